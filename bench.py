@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""Benchmark of the Genz QMC multivariate-normal path on one MI355X.
+
+Workload (the headline configuration): dense factor model, d = 2500,
+Sigma = standardize(B B^T + diag(psi)) with B 2500 x 8, lower orthant
+b = Phi^{-1}(u), u ~ U(0.01, 0.10); Richtmyer lattice, N = 2^18 points x 10
+random shifts.  One step is one whole job with the inputs resident in HBM:
+the device Genz-Bretz Cholesky of Sigma, truncate/pack, the fused
+point-generation + integration kernel over all N x 10 samples and the FP64
+shift reductions.
+
+Reported: sample*dims/s of the FP32-mixed path (the value), the plain FP64
+path on the same job, their ratio, the relative error of each against the
+FP64 CPU oracle on a bounded sample of the same points and shifts, the
+roofline of the integration kernel (device time from HIP events on the
+plan's stream) and the CPU oracle's own throughput on that sample.
+
+The FP64 path runs the erfc form of the normal CDF (QMCData_Distributions):
+at this shape the erf form's 1 + erf(x) is exactly 0 in Float64 for many
+dimensions, so its estimate is 0 (as in the reference).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 matrix
+PEAK_FP64_TFLOPS = 78.6
+
+
+def _log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--d", type=int, default=2500)
+    ap.add_argument("--m", type=int, default=2 ** 18)
+    ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--cpu-points", type=int, default=2 ** 14,
+                    help="points of shift 1 in the CPU oracle sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the path would shard "
+              "by (shift, point range) with one reduction of the shift sums; see DESIGN.md)",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    import dbgenzpaper_amd as dbg
+    from dbgenzpaper_amd import workloads as W
+
+    d, m, R = args.d, args.m, args.reps
+    _log(f"workload factor_model d={d} k=8 m={m} reps={R}")
+    P = W.factor_model(d, 8, m=m, n_reps=R, seed=1)
+
+    def make(T, m_, R_, shifts):
+        opts = dbg.QMC_opts(T, m=m_, n_reps=R_, max_pts=m_, max_abs_err=0.0)
+        D = dbg.QMCData(P.Sigma, P.a, P.b, opts, None, "Richtmyer", shifts=shifts)
+        if T == np.float64:
+            D = dbg.QMCData_Distributions(D)
+        return D
+
+    def run(D, steps, warmup):
+        for _ in range(warmup):
+            D.refactor()
+            D.loop(0)
+            D.sync()
+        kms = []
+        D.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            D.refactor()
+            D.loop(0)
+            kms.append(D.last_kernel_ms())
+        D.sync()
+        t1 = time.perf_counter()
+        return (t1 - t0) / steps, sum(kms) / len(kms), D.summary()
+
+    Dm = make(np.float32, m, R, P.shifts)
+    _log("mixed plan ready")
+    t_mixed, k_mixed, s_mixed = run(Dm, args.steps, args.warmup)
+    _log(f"mixed: {t_mixed * 1e3:.1f} ms/step, kernel {k_mixed:.1f} ms, log P {s_mixed.log_value:.6f}")
+    sdims = float(m) * R * d
+    value = sdims / t_mixed
+    out = {
+        "metric": "sample*dims/s, FP32-mixed Genz QMC MVN probability at d=2500 (N=2^18 x 10 shifts)",
+        "value": value,
+        "unit": "sample*dims/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_mixed * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (contraction, special functions) + fp64 (factor, points, reductions)",
+        "data": "synthetic (seeded factor model, random-init inputs of the configured shape)",
+        "config": {"workload": f"factor_model d={d} k=8 lower orthant u~U(0.01,0.10), Richtmyer "
+                               f"N={m} x {R} shifts, step = Cholesky + integration + reductions",
+                   "d": d, "points": m, "shifts": R, "parallelism": "single GPU"},
+        "log_p_mixed": s_mixed.log_value,
+        "kernel_ms_mixed": k_mixed,
+    }
+    flops = float(m) * R * d * (d - 1)
+    achieved = flops / (k_mixed * 1e-3) / 1e12
+    out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                       "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                       "kernel": "dense integration (mixed)", "flops_per_launch": flops}
+    Dm.close()
+
+    if not args.no_fp64:
+        D64 = make(np.float64, m, R, P.shifts)
+        _log("fp64 plan ready")
+        t64, k64, s64 = run(D64, max(1, args.steps), min(args.warmup, 1))
+        _log(f"fp64: {t64 * 1e3:.1f} ms/step, kernel {k64:.1f} ms, log P {s64.log_value:.6f}")
+        out["fp64_value"] = sdims / t64
+        out["fp64_ms_per_step"] = t64 * 1e3
+        out["kernel_ms_fp64"] = k64
+        out["log_p_fp64"] = s64.log_value
+        out["speedup_mixed_vs_fp64"] = t64 / t_mixed
+        out["rel_err_mixed_vs_fp64_full"] = abs(math.expm1(s_mixed.log_value - s64.log_value))
+        D64.close()
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_leg(P, d, args.cpu_points, out)
+
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def cpu_leg(P, d, n_pts, out):
+    """FP64 oracle (erfc form, the reference's blocking bs_j=2^7, bs_i=2^10,
+    bs_i2=2^6) on the first n_pts points of shift 1, all host cores; the GPU
+    paths run the same sample for their error against it."""
+    import numpy as np
+
+    import dbgenzpaper_amd as dbg
+    from oracle import oracle as O
+    from tests._helpers import oracle_same_inputs
+
+    sh = np.asfortranarray(P.shifts[:, :1])
+    o64 = dbg.QMC_opts(np.float64, m=n_pts, n_reps=1, max_pts=n_pts, max_abs_err=0.0,
+                       block_size_i=2 ** 10, block_size_i2=2 ** 6, block_size_j=2 ** 7)
+    D64 = dbg.QMCData_Distributions(dbg.QMCData(P.Sigma, P.a, P.b, o64, None, "Richtmyer", shifts=sh))
+    o32 = dbg.QMC_opts(np.float32, m=n_pts, n_reps=1, max_pts=n_pts, max_abs_err=0.0)
+    Dm = dbg.QMCData(P.Sigma, P.a, P.b, o32, None, "Richtmyer", shifts=sh)
+    dbg.qmc_pnorm(D64)
+    dbg.qmc_pnorm(Dm)
+    threads = O.num_threads()
+    _log(f"cpu oracle: {n_pts} points, {threads} threads")
+    t0 = time.perf_counter()
+    r = oracle_same_inputs(O, D64)
+    dt = time.perf_counter() - t0
+    lg64 = D64.last_result.log_value
+    lgm = Dm.last_result.log_value
+    out["rel_err_fp64_vs_oracle"] = abs(math.expm1(lg64 - r.log_value))
+    out["rel_err_mixed_vs_oracle"] = abs(math.expm1(lgm - r.log_value))
+    out["oracle_sample_log_p"] = r.log_value
+    D64.close()
+    Dm.close()
+    return {"value": n_pts * d / dt, "unit": "sample*dims/s", "cores": threads, "kind": "port",
+            "sample": f"first {n_pts} Richtmyer points of shift 1 of the same d={d} problem, FP64 erfc form, "
+                      f"{dt:.2f} s wall", "seconds": dt}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

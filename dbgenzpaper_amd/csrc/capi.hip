@@ -1,0 +1,667 @@
+// C ABI of libdbgenz.so: plan objects, the factorization drivers and the
+// qmc_pnorm! driver (src/qmc.jl:783-850) on top of the kernels.
+#include <float.h>
+#include <math.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/dbgenz.h"
+#include "dbg_internal.h"
+
+using namespace dbg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(DBG_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+template <typename P>
+hipError_t dalloc(P** p, size_t bytes)
+{
+    *p = nullptr;
+    return hipMalloc((void**)p, bytes > 0 ? bytes : 16);
+}
+
+// Window boundaries of the reference's loop nest (iv -> iii -> ii blocks),
+// src/cholesky.jl:482-491.
+std::vector<std::pair<int, int>> pivot_windows(int n, int bs2, int bs3, int bs4)
+{
+    std::vector<std::pair<int, int>> w;
+    for (int iv0 = 0; iv0 < n; iv0 += bs4) {
+        const int iv1 = std::min(n, iv0 + bs4);
+        for (int iii0 = iv0; iii0 < iv1; iii0 += bs3) {
+            const int iii1 = std::min(iv1, iii0 + bs3);
+            for (int ii0 = iii0; ii0 < iii1; ii0 += bs2) w.push_back({ii0, std::min(iii1, ii0 + bs2)});
+        }
+    }
+    return w;
+}
+
+struct CholBuffers {
+    double* M = nullptr;  // n x n, lda = n
+    double* a = nullptr;
+    double* b = nullptr;
+    double* scale = nullptr;
+    double* s_vec = nullptr;
+    double* eps0 = nullptr;
+    int32_t* perm = nullptr;
+    int32_t* rank = nullptr;
+
+    int alloc(int n)
+    {
+        const size_t nn = (size_t)n * n;
+        HIPCHK(dalloc(&M, nn * sizeof(double)));
+        HIPCHK(dalloc(&a, n * sizeof(double)));
+        HIPCHK(dalloc(&b, n * sizeof(double)));
+        HIPCHK(dalloc(&scale, n * sizeof(double)));
+        HIPCHK(dalloc(&s_vec, n * sizeof(double)));
+        HIPCHK(dalloc(&eps0, sizeof(double)));
+        HIPCHK(dalloc(&perm, n * sizeof(int32_t)));
+        HIPCHK(dalloc(&rank, sizeof(int32_t)));
+        return DBG_OK;
+    }
+    void release()
+    {
+        (void)hipFree(M); (void)hipFree(a); (void)hipFree(b); (void)hipFree(scale);
+        (void)hipFree(s_vec); (void)hipFree(eps0); (void)hipFree(perm); (void)hipFree(rank);
+        M = a = b = scale = s_vec = eps0 = nullptr;
+        perm = rank = nullptr;
+    }
+};
+
+int check_block_sizes(int bs1, int bs2, int bs3, int bs4)
+{
+    if (bs1 < 1 || bs2 < 1 || bs3 < 1 || bs4 < 1)
+        return fail(DBG_ERR_ARG, "Cholesky block sizes must be positive");
+    return DBG_OK;
+}
+
+// Copy the inputs (host or device memory, hipMemcpyDefault) and factor.
+// pivot: cholesky_genz!, else cholesky_classic!.
+int factor_on_device(CholBuffers& B, int n, const double* M_in, int64_t lda, const double* a_in,
+                     const double* b_in, int bs1, int bs2, int bs3, int bs4, bool pivot,
+                     hipStream_t s, int32_t* rank_out)
+{
+    int rc = check_block_sizes(bs1, bs2, bs3, bs4);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy2DAsync(B.M, (size_t)n * sizeof(double), M_in, (size_t)lda * sizeof(double),
+                            (size_t)n * sizeof(double), n, hipMemcpyDefault, s));
+    HIPCHK(hipMemcpyAsync(B.a, a_in, n * sizeof(double), hipMemcpyDefault, s));
+    HIPCHK(hipMemcpyAsync(B.b, b_in, n * sizeof(double), hipMemcpyDefault, s));
+    std::vector<int32_t> iota(n);
+    for (int i = 0; i < n; ++i) iota[i] = i;
+    HIPCHK(hipMemcpyAsync(B.perm, iota.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemsetAsync(B.s_vec, 0, n * sizeof(double), s));
+    CholArgs C{B.M, (int64_t)n, n, B.a, B.b, B.scale, B.perm, B.s_vec, B.eps0, B.rank};
+    HIPCHK(chol_standardize(C, false, s));
+    const int nb = std::max(1, std::min(bs1, 64));
+    for (auto& w : pivot_windows(n, bs2, bs3, bs4)) {
+        if (w.second - w.first > chol_max_window())
+            return fail(DBG_ERR_ARG, "Cholesky pivot window (min(chol_block_size2, 1024-aligned blocks)) above " +
+                                         std::to_string(chol_max_window()));
+        HIPCHK(chol_window(C, w.first, w.second, nb, pivot, s));
+        HIPCHK(chol_trsm_syrk(C, w.first, w.second, s));
+    }
+    HIPCHK(chol_finish(C, s));
+    int32_t rank = n;
+    HIPCHK(hipMemcpyAsync(&rank, B.rank, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    *rank_out = rank;
+    return DBG_OK;
+}
+
+int chol_host(double* M, int64_t lda, int32_t n, double* a, double* b, int32_t bs1, int32_t bs2,
+              int32_t bs3, int32_t bs4, int32_t* perm, double* scale, int32_t* rank, bool pivot)
+{
+    if (n < 1 || lda < n || !M || !a || !b) return fail(DBG_ERR_ARG, "invalid matrix arguments");
+    hipStream_t s;
+    HIPCHK(hipStreamCreate(&s));
+    CholBuffers B;
+    int rc = B.alloc(n);
+    int32_t rk = n;
+    if (!rc) rc = factor_on_device(B, n, M, lda, a, b, bs1, bs2, bs3, bs4, pivot, s, &rk);
+    if (!rc) {
+        hipError_t e = hipMemcpy2D(M, (size_t)lda * sizeof(double), B.M, (size_t)n * sizeof(double),
+                                   (size_t)n * sizeof(double), n, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(a, B.a, n * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(b, B.b, n * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess && scale) e = hipMemcpy(scale, B.scale, n * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess && perm) e = hipMemcpy(perm, B.perm, n * sizeof(int32_t), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(DBG_ERR_DEVICE, std::string("copy back: ") + hipGetErrorString(e));
+        if (rank) *rank = rk;
+    }
+    B.release();
+    (void)hipStreamDestroy(s);
+    return rc;
+}
+
+}  // namespace
+
+struct dbg_plan {
+    dbg_opts o{};
+    int n = 0, nd = 0;
+    bool mixed = false;
+    size_t tsz = 8;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double* sig = nullptr;   // device copies of the inputs Sigma (lda = n), a, b
+    double* a0 = nullptr;
+    double* b0 = nullptr;
+    CholBuffers F;
+    int32_t rank = 0;
+    // packed layouts
+    void* Uk = nullptr;
+    int64_t ldu = 0;
+    void* ak = nullptr;
+    void* bk = nullptr;
+    void* invk = nullptr;
+    double* first = nullptr;
+    double first_h[6] = {0, 0, 0, 0, 0, 0};
+    int64_t* colcnt = nullptr;
+    int64_t* colptr = nullptr;
+    int32_t* rowval = nullptr;
+    void* nzval = nullptr;
+    int64_t nnz = 0;
+    // points
+    double* q = nullptr;
+    double* shifts_f = nullptr;
+    uint64_t* shifts_u = nullptr;
+    uint32_t* V = nullptr;
+    bool shifts_set = false, dirs_set = false;
+    int64_t state = 0;
+    uint64_t sobol_skip = 0;
+    // integration workspace
+    void* Y = nullptr;
+    int64_t y_slot = 0;
+    int grid = 0;
+    int tile = 256;
+    int64_t tiles_per_rep = 0, n_tiles = 0;
+    double* part_m = nullptr;
+    int* part_e = nullptr;
+    double* rep_m = nullptr;
+    int* rep_e = nullptr;
+    double* out = nullptr;
+
+    void release()
+    {
+        F.release();
+        void* ptrs[] = {sig, a0, b0, Uk, ak, bk, invk, first, colcnt, colptr, rowval, nzval, q, shifts_f,
+                        shifts_u, V, Y, part_m, part_e, rep_m, rep_e, out};
+        for (void* p : ptrs) (void)hipFree(p);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// Factor the device-resident inputs and rebuild the kernel layouts (the
+// QMCData constructor's work, src/qmc.jl:182-199).
+int plan_factor(dbg_plan* P)
+{
+    const dbg_opts& o = P->o;
+    const int n = P->n;
+    hipStream_t s = P->stream;
+    const bool pivot = (o.kind == DBG_KIND_DENSE);
+    int rc = factor_on_device(P->F, n, P->sig, n, P->a0, P->b0, o.chol_block_size, o.chol_block_size2,
+                              o.chol_block_size3, o.chol_block_size4, pivot, s, &P->rank);
+    if (rc) return rc;
+    if (pivot && P->rank != n)
+        return fail(DBG_ERR_RANK, "QMCData requires a full-rank Cholesky factorization.");
+    // truncate_matrixU!: |U_ij| < eps(T) / (9n) -> 0
+    const double min_val = P->mixed ? (double)(FLT_EPSILON / (float)(n * 9)) : DBL_EPSILON / (double)(n * 9);
+    HIPCHK(truncate_upper(P->F.M, n, n, min_val, s));
+    const size_t T = P->tsz;
+    if (o.kind != DBG_KIND_DENSE) {
+        HIPCHK(csc_count(P->F.M, n, n, P->colcnt, s));
+        HIPCHK(exclusive_scan_i64(P->colcnt, P->colptr, n, s));
+        int64_t nnz = 0;
+        HIPCHK(hipMemcpyAsync(&nnz, P->colptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (nnz > P->nnz || P->rowval == nullptr) {
+            (void)hipFree(P->rowval);
+            (void)hipFree(P->nzval);
+            P->rowval = nullptr;
+            P->nzval = nullptr;
+            HIPCHK(dalloc(&P->rowval, (size_t)std::max<int64_t>(nnz, 1) * sizeof(int32_t)));
+            HIPCHK(dalloc(&P->nzval, (size_t)std::max<int64_t>(nnz, 1) * T));
+        }
+        P->nnz = nnz;
+        HIPCHK(csc_fill(P->F.M, n, n, P->colptr, P->mixed, P->rowval, P->nzval, s));
+    }
+    HIPCHK(pack_dense(P->F.M, n, n, P->F.a, P->F.b, P->mixed, P->Uk, P->ldu, P->ak, P->bk, P->invk,
+                      P->first, o.kind == DBG_KIND_DENSE ? o.phi_form : DBG_PHI_ERF, s));
+    HIPCHK(hipMemcpyAsync(P->first_h, P->first, 6 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return DBG_OK;
+}
+
+int plan_build(dbg_plan* P, const double* Sigma, int64_t lda, const double* a, const double* b)
+{
+    const dbg_opts& o = P->o;
+    const int n = P->n;
+    hipStream_t s = P->stream;
+    int rc = P->F.alloc(n);
+    if (rc) return rc;
+    HIPCHK(dalloc(&P->sig, (size_t)n * n * sizeof(double)));
+    HIPCHK(dalloc(&P->a0, n * sizeof(double)));
+    HIPCHK(dalloc(&P->b0, n * sizeof(double)));
+    HIPCHK(hipMemcpy2DAsync(P->sig, (size_t)n * sizeof(double), Sigma, (size_t)lda * sizeof(double),
+                            (size_t)n * sizeof(double), n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(P->a0, a, n * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(P->b0, b, n * sizeof(double), hipMemcpyHostToDevice, s));
+
+    const int nd = P->nd;
+    const size_t T = P->tsz;
+    HIPCHK(dalloc(&P->ak, n * T));
+    HIPCHK(dalloc(&P->bk, n * T));
+    HIPCHK(dalloc(&P->invk, n * T));
+    HIPCHK(dalloc(&P->first, 8 * sizeof(double)));
+    if (o.kind == DBG_KIND_DENSE) {
+        P->ldu = (int64_t)n + 64;
+        HIPCHK(dalloc(&P->Uk, (size_t)std::max(nd, 1) * P->ldu * T));
+    } else {
+        HIPCHK(dalloc(&P->colcnt, (size_t)n * sizeof(int64_t)));
+        HIPCHK(dalloc(&P->colptr, (size_t)(n + 1) * sizeof(int64_t)));
+    }
+    rc = plan_factor(P);
+    if (rc) return rc;
+
+    // lattice generating vector frac(sqrt(p_g)) (src/qmc_generators.jl:300-310)
+    std::vector<double> qh(std::max(nd, 1), 0.0);
+    {
+        int found = 0;
+        long p = 1;
+        while (found < nd) {
+            long c = p + 1;
+            for (;; ++c) {
+                bool is_p = c >= 2;
+                for (long f = 2; f * f <= c && is_p; ++f)
+                    if (c % f == 0) is_p = false;
+                if (is_p) break;
+            }
+            p = c;
+            const double sq = sqrt((double)p);
+            qh[found++] = sq - floor(sq);
+        }
+    }
+    HIPCHK(dalloc(&P->q, qh.size() * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(P->q, qh.data(), qh.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(dalloc(&P->shifts_f, (size_t)std::max(nd, 1) * o.n_reps * sizeof(double)));
+    HIPCHK(dalloc(&P->shifts_u, (size_t)std::max(nd, 1) * o.n_reps * sizeof(uint64_t)));
+    HIPCHK(dalloc(&P->V, (size_t)std::max(nd, 1) * 32 * sizeof(uint32_t)));
+
+    // integration workspace: one y slot per resident block
+    int dev = 0, ncu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    P->tile = (o.kind == DBG_KIND_DENSE) ? dense_valu_tile() : sparse_tile();
+    P->tiles_per_rep = (o.m + P->tile - 1) / P->tile;
+    P->n_tiles = P->tiles_per_rep * o.n_reps;
+    P->grid = (int)std::min<int64_t>(P->n_tiles, (int64_t)ncu * 2);
+    P->y_slot = (int64_t)std::max(nd, 1) * P->tile;
+    HIPCHK(dalloc(&P->Y, (size_t)P->grid * P->y_slot * T));
+    HIPCHK(dalloc(&P->part_m, (size_t)P->n_tiles * sizeof(double)));
+    HIPCHK(dalloc(&P->part_e, (size_t)P->n_tiles * sizeof(int)));
+    HIPCHK(dalloc(&P->rep_m, (size_t)o.n_reps * sizeof(double)));
+    HIPCHK(dalloc(&P->rep_e, (size_t)o.n_reps * sizeof(int)));
+    HIPCHK(dalloc(&P->out, 8 * sizeof(double)));
+    HIPCHK(hipMemsetAsync(P->rep_m, 0, (size_t)o.n_reps * sizeof(double), s));
+    HIPCHK(hipMemsetAsync(P->rep_e, 0, (size_t)o.n_reps * sizeof(int), s));
+    HIPCHK(hipStreamSynchronize(s));
+    return DBG_OK;
+}
+
+PointArgs point_args(const dbg_plan* P, int64_t block)
+{
+    PointArgs pa{};
+    pa.nd = P->nd;
+    pa.q = P->q;
+    pa.shifts_f = P->shifts_f;
+    pa.n_base = block * P->o.m;
+    pa.V = P->V;
+    pa.shifts_u = P->shifts_u;
+    pa.sobol_first = P->sobol_skip + (uint64_t)block * (uint64_t)P->o.m;
+    const int L = P->o.n_bits;
+    pa.mask = (L >= 64) ? ~0ull : ((1ull << L) - 1ull);
+    pa.invscale = ldexp(1.0, -L);
+    return pa;
+}
+
+int run_loop(dbg_plan* P, int64_t block, int64_t n_pts0)
+{
+    const dbg_opts& o = P->o;
+    if (P->nd > 0 && !P->shifts_set) return fail(DBG_ERR_STATE, "shifts not set");
+    if (P->nd > 0 && o.generator == DBG_GEN_SOBOL && !P->dirs_set)
+        return fail(DBG_ERR_STATE, "Sobol direction integers not set");
+    const int gen = (o.kind == DBG_KIND_SPARSEQMC) ? kGenSobol : o.generator;
+    HIPCHK(hipEventRecord(P->ev0, P->stream));
+    if (o.kind == DBG_KIND_DENSE) {
+        DenseArgs A{};
+        A.n = P->n;
+        A.m = o.m;
+        A.n_reps = o.n_reps;
+        A.tiles_per_rep = P->tiles_per_rep;
+        A.n_tiles = P->n_tiles;
+        A.Uk = P->Uk;
+        A.ldu = P->ldu;
+        A.ak = P->ak;
+        A.bk = P->bk;
+        A.invk = P->invk;
+        A.c1 = P->first_h[0];
+        A.dc1 = P->first_h[1];
+        A.ea1 = (float)P->first_h[2];
+        A.eb1 = (float)P->first_h[3];
+        A.d1 = (float)P->first_h[4];
+        A.mode1 = (int)P->first_h[5];
+        A.pts = point_args(P, block);
+        A.Y = P->Y;
+        A.y_slot = P->y_slot;
+        A.part_m = P->part_m;
+        A.part_e = P->part_e;
+        HIPCHK(launch_dense_valu(A, P->mixed, gen, o.phi_form, P->grid, P->stream));
+    } else {
+        SparseArgs A{};
+        A.n = P->n;
+        A.m = o.m;
+        A.n_reps = o.n_reps;
+        A.tiles_per_rep = P->tiles_per_rep;
+        A.n_tiles = P->n_tiles;
+        A.colptr = P->colptr;
+        A.rowval = P->rowval;
+        A.nzval = P->nzval;
+        A.ak = P->ak;
+        A.bk = P->bk;
+        A.invk = P->invk;
+        A.c1 = P->first_h[0];
+        A.dc1 = P->first_h[1];
+        A.ea1 = (float)P->first_h[2];
+        A.eb1 = (float)P->first_h[3];
+        A.d1 = (float)P->first_h[4];
+        A.mode1 = (int)P->first_h[5];
+        A.tent = (o.kind == DBG_KIND_SPARSEQMC) ? 0 : 1;
+        A.pts = point_args(P, block);
+        A.Y = P->Y;
+        A.y_slot = P->y_slot;
+        A.part_m = P->part_m;
+        A.part_e = P->part_e;
+        HIPCHK(launch_sparse(A, P->mixed, gen, P->grid, P->stream));
+    }
+    HIPCHK(hipEventRecord(P->ev1, P->stream));
+    HIPCHK(launch_reduce_reps(P->part_m, P->part_e, P->tiles_per_rep, o.n_reps, n_pts0, o.m,
+                              o.kind == DBG_KIND_SPARSEQMC ? 1 : 0, P->rep_m, P->rep_e, P->out,
+                              P->stream));
+    return DBG_OK;
+}
+
+int read_summary(dbg_plan* P, dbg_result* r)
+{
+    double h[8];
+    HIPCHK(hipMemcpyAsync(h, P->out, 7 * sizeof(double), hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    r->value = h[0];
+    r->log_value = h[1];
+    r->err3 = h[2];
+    r->rel_err3 = h[3];
+    return DBG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dbg_last_error(void) { return g_err.c_str(); }
+
+int32_t dbg_abi_version(void) { return 1; }
+
+void dbg_opts_default(dbg_opts* o, int32_t precision)
+{
+    if (!o) return;
+    memset(o, 0, sizeof(*o));
+    o->chol_block_size = 1 << 6;
+    o->chol_block_size2 = 1 << 9;
+    o->chol_block_size3 = 1 << 10;
+    o->chol_block_size4 = 1 << 11;
+    o->m = 10000;
+    o->block_size_i = 1 << 6;
+    o->block_size_i2 = 1 << 5;
+    o->block_size_j = 1 << 6;
+    o->n_blocks = 1;
+    o->n_reps = 12;
+    o->n_bits = 32;
+    o->max_pts = 10000;
+    o->max_abs_err = 1e-6;
+    o->precision = precision;
+    o->generator = DBG_GEN_RICHTMYER;
+    o->phi_form = DBG_PHI_ERF;
+    o->kind = DBG_KIND_DENSE;
+    o->sobol_skip = -1;
+}
+
+int dbg_chol_genz_f64(double* M, int64_t lda, int32_t n, double* a, double* b, int32_t bs1,
+                      int32_t bs2, int32_t bs3, int32_t bs4, int32_t* perm, double* scale,
+                      int32_t* rank)
+{
+    g_err.clear();
+    return chol_host(M, lda, n, a, b, bs1, bs2, bs3, bs4, perm, scale, rank, true);
+}
+
+int dbg_chol_classic_f64(double* M, int64_t lda, int32_t n, double* a, double* b, int32_t bs1,
+                         int32_t bs2, int32_t bs3, int32_t bs4, double* scale)
+{
+    g_err.clear();
+    return chol_host(M, lda, n, a, b, bs1, bs2, bs3, bs4, nullptr, scale, nullptr, false);
+}
+
+int dbg_plan_create(dbg_plan** plan, const dbg_opts* opts, int32_t n, const double* Sigma,
+                    int64_t lda, const double* a, const double* b)
+{
+    g_err.clear();
+    if (!plan || !opts || !Sigma || !a || !b) return fail(DBG_ERR_ARG, "null argument");
+    *plan = nullptr;
+    const dbg_opts& o = *opts;
+    if (n < 1 || lda < n) return fail(DBG_ERR_ARG, "invalid dimension or leading dimension");
+    if (o.m < 1 || o.n_reps < 1) return fail(DBG_ERR_ARG, "m and n_reps must be positive");
+    if (o.precision != DBG_PREC_FP64 && o.precision != DBG_PREC_MIXED)
+        return fail(DBG_ERR_ARG, "unknown precision");
+    if (o.generator != DBG_GEN_RICHTMYER && o.generator != DBG_GEN_SOBOL)
+        return fail(DBG_ERR_ARG, "unknown generator");
+    if (o.kind < DBG_KIND_DENSE || o.kind > DBG_KIND_SPARSEQMC) return fail(DBG_ERR_ARG, "unknown kind");
+    if (o.phi_form != DBG_PHI_ERF && o.phi_form != DBG_PHI_ERFC) return fail(DBG_ERR_ARG, "unknown phi_form");
+    if (o.n_bits < 1 || o.n_bits > 64) return fail(DBG_ERR_ARG, "n_bits out of range");
+    dbg_plan* P = new dbg_plan();
+    P->o = o;
+    P->n = n;
+    P->nd = n - 1;
+    P->mixed = (o.precision == DBG_PREC_MIXED);
+    P->tsz = P->mixed ? sizeof(float) : sizeof(double);
+    const int64_t skip_n = 2 * o.m - 1;
+    if (o.sobol_skip >= 0) {
+        P->sobol_skip = (uint64_t)o.sobol_skip;
+    } else {
+        int k = 0;
+        while ((1ll << (k + 1)) <= skip_n + 1) ++k;
+        P->sobol_skip = (uint64_t)((1ll << k) - 1);
+    }
+    P->state = (o.kind == DBG_KIND_SPARSEQMC) ? 1 : 0;
+    int rc = DBG_OK;
+    hipError_t e = hipStreamCreate(&P->stream);
+    if (e == hipSuccess) e = hipEventCreate(&P->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&P->ev1);
+    if (e != hipSuccess) rc = fail(DBG_ERR_DEVICE, std::string("stream: ") + hipGetErrorString(e));
+    if (!rc) rc = plan_build(P, Sigma, lda, a, b);
+    if (rc) {
+        std::string msg = g_err;
+        P->release();
+        delete P;
+        g_err = msg;
+        return rc;
+    }
+    *plan = P;
+    return DBG_OK;
+}
+
+int dbg_plan_destroy(dbg_plan* plan)
+{
+    if (!plan) return DBG_OK;
+    (void)hipStreamSynchronize(plan->stream);
+    plan->release();
+    delete plan;
+    return DBG_OK;
+}
+
+int dbg_plan_refactor(dbg_plan* P)
+{
+    g_err.clear();
+    if (!P) return fail(DBG_ERR_ARG, "null plan");
+    return plan_factor(P);
+}
+
+int dbg_plan_set_shifts(dbg_plan* P, const void* shifts)
+{
+    g_err.clear();
+    if (!P || (!shifts && P->nd > 0)) return fail(DBG_ERR_ARG, "null argument");
+    const size_t cnt = (size_t)P->nd * P->o.n_reps;
+    if (cnt > 0) {
+        const bool sob = P->o.generator == DBG_GEN_SOBOL || P->o.kind == DBG_KIND_SPARSEQMC;
+        if (sob)
+            HIPCHK(hipMemcpyAsync(P->shifts_u, shifts, cnt * sizeof(uint64_t), hipMemcpyHostToDevice, P->stream));
+        else
+            HIPCHK(hipMemcpyAsync(P->shifts_f, shifts, cnt * sizeof(double), hipMemcpyHostToDevice, P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+    }
+    P->shifts_set = true;
+    return DBG_OK;
+}
+
+int dbg_plan_set_sobol_directions(dbg_plan* P, const uint32_t* V)
+{
+    g_err.clear();
+    if (!P || (!V && P->nd > 0)) return fail(DBG_ERR_ARG, "null argument");
+    if (P->nd > 0) {
+        HIPCHK(hipMemcpyAsync(P->V, V, (size_t)P->nd * 32 * sizeof(uint32_t), hipMemcpyHostToDevice, P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+    }
+    P->dirs_set = true;
+    return DBG_OK;
+}
+
+int dbg_qmc_loop(dbg_plan* P, int64_t n_pts0)
+{
+    g_err.clear();
+    if (!P) return fail(DBG_ERR_ARG, "null plan");
+    const int64_t block = (P->o.kind == DBG_KIND_SPARSEQMC) ? 0 : P->state;
+    return run_loop(P, block, n_pts0);
+}
+
+int dbg_plan_sync(dbg_plan* P)
+{
+    if (!P) return fail(DBG_ERR_ARG, "null plan");
+    HIPCHK(hipStreamSynchronize(P->stream));
+    return DBG_OK;
+}
+
+int dbg_plan_last_kernel_ms(dbg_plan* P, float* ms)
+{
+    if (!P || !ms) return fail(DBG_ERR_ARG, "null argument");
+    HIPCHK(hipEventSynchronize(P->ev1));
+    HIPCHK(hipEventElapsedTime(ms, P->ev0, P->ev1));
+    return DBG_OK;
+}
+
+int dbg_plan_summary(dbg_plan* P, dbg_result* out)
+{
+    if (!P || !out) return fail(DBG_ERR_ARG, "null argument");
+    return read_summary(P, out);
+}
+
+int dbg_qmc_pnorm(dbg_plan* P, dbg_result* out)
+{
+    g_err.clear();
+    if (!P || !out) return fail(DBG_ERR_ARG, "null argument");
+    const dbg_opts& o = P->o;
+    HIPCHK(hipMemsetAsync(P->rep_m, 0, (size_t)o.n_reps * sizeof(double), P->stream));
+    HIPCHK(hipMemsetAsync(P->rep_e, 0, (size_t)o.n_reps * sizeof(int), P->stream));
+    int64_t n_pts = 0;
+    int rounds = 0;
+    dbg_result r{};
+    for (;;) {
+        int64_t block;
+        if (o.kind == DBG_KIND_SPARSEQMC)
+            block = rounds == 0 ? 0 : P->state++;
+        else
+            block = P->state;
+        int rc = run_loop(P, block, n_pts);
+        if (rc) return rc;
+        rounds++;
+        rc = read_summary(P, &r);
+        if (rc) return rc;
+        n_pts += o.m;
+        const double err_acc = r.err3;
+        if (rounds == 1 && ((n_pts >= o.max_pts) || (err_acc < o.max_abs_err))) break;
+        if (!((n_pts < o.max_pts) && (err_acc > o.max_abs_err))) break;
+        if (o.kind != DBG_KIND_SPARSEQMC) P->state++;  // next_points!
+    }
+    r.n_pts = n_pts;
+    r.rounds = rounds;
+    *out = r;
+    if (isnan(r.value) && isnan(r.log_value)) return fail(DBG_ERR_NONFINITE, "result is NaN");
+    return DBG_OK;
+}
+
+int dbg_plan_get_factor(dbg_plan* P, double* U, int32_t* perm, double* a, double* b, double* scale,
+                        int32_t* rank)
+{
+    g_err.clear();
+    if (!P) return fail(DBG_ERR_ARG, "null plan");
+    const int n = P->n;
+    HIPCHK(hipStreamSynchronize(P->stream));
+    if (U) HIPCHK(hipMemcpy(U, P->F.M, (size_t)n * n * sizeof(double), hipMemcpyDeviceToHost));
+    if (perm) HIPCHK(hipMemcpy(perm, P->F.perm, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (a) HIPCHK(hipMemcpy(a, P->F.a, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (b) HIPCHK(hipMemcpy(b, P->F.b, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (scale) HIPCHK(hipMemcpy(scale, P->F.scale, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (rank) *rank = P->rank;
+    return DBG_OK;
+}
+
+int dbg_plan_get_reps(dbg_plan* P, double* rep_mant, int32_t* rep_exp)
+{
+    g_err.clear();
+    if (!P) return fail(DBG_ERR_ARG, "null plan");
+    HIPCHK(hipStreamSynchronize(P->stream));
+    if (rep_mant) HIPCHK(hipMemcpy(rep_mant, P->rep_m, P->o.n_reps * sizeof(double), hipMemcpyDeviceToHost));
+    if (rep_exp) HIPCHK(hipMemcpy(rep_exp, P->rep_e, P->o.n_reps * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return DBG_OK;
+}
+
+int dbg_plan_get_state(dbg_plan* P, int64_t* state)
+{
+    if (!P || !state) return fail(DBG_ERR_ARG, "null argument");
+    *state = P->state;
+    return DBG_OK;
+}
+
+int dbg_plan_set_state(dbg_plan* P, int64_t state)
+{
+    if (!P || state < 0) return fail(DBG_ERR_ARG, "invalid state");
+    P->state = state;
+    return DBG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,373 @@
+// FP64 Cholesky factorizations on the device.
+//
+// cholesky_genz!  (src/cholesky.jl:445-628) and cholesky_classic!
+// (:144-242).  The reference's four-level blocking only changes rounding,
+// with one exception that changes results: the Genz-Bretz pivot is searched
+// inside the current bs2-window ("ii" block) only, and the conditional means
+// s_vec only collect contributions of pivots of the same window.  The device
+// version therefore factors window by window:
+//
+//   chol_window_kernel  one workgroup factors the window's diagonal block with
+//                       the block-local pivot (argmin of erf(b')-erf(a'),
+//                       first index, NaN first), swapping rows and columns of
+//                       the whole upper-stored matrix, and the window's rows
+//                       of U inside the window; inner blocks of nb pivots are
+//                       left-looking with a deferred trailing update, like
+//                       the reference's bs1 blocks (d_vec reset per block);
+//   chol_trsm_kernel    the window's rows of U beyond the window
+//                       (ldiv!(UpperTriangular(.)', .) of :589-623);
+//   chol_syrk_kernel    the trailing Schur complement update (the mul! of
+//                       :592-595), upper triangle only.
+//
+// The matrix is column-major with leading dimension lda.  Only the upper
+// triangle is kept up to date; chol_finish zeroes the strict lower triangle.
+#include <float.h>
+
+#include "dbg_device.h"
+
+namespace dbg {
+
+constexpr int kCholThreads = 1024;
+constexpr int kMaxWin = 2048;
+
+int chol_max_window() { return kMaxWin; }
+
+#define MM(i, j) M[(size_t)(i) + (size_t)(j) * (size_t)lda]
+
+// scale_j = sqrt(M_jj) (1 if M_jj <= 0), eps0 = eps * max|diag| of the input
+// (src/cholesky.jl:450-456), rank = n.
+__global__ __launch_bounds__(kCholThreads) void chol_diag_kernel(const double* M, int64_t lda, int n,
+                                                                 double* scale, double* eps0,
+                                                                 int32_t* rank)
+{
+    __shared__ double s_m[kCholThreads / 64];
+    double mx = 0.0;
+    for (int j = threadIdx.x; j < n; j += kCholThreads) {
+        const double v = MM(j, j);
+        const double av = fabs(v);
+        mx = (isnan(av) || av > mx) ? av : mx;
+        scale[j] = v > 0 ? sqrt(v) : 1.0;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(mx, off, 64);
+        mx = (isnan(o) || o > mx) ? o : mx;
+    }
+    if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = s_m[0];
+        for (int w = 1; w < kCholThreads / 64; ++w) m = (isnan(s_m[w]) || s_m[w] > m) ? s_m[w] : m;
+        *eps0 = DBL_EPSILON * m;
+        *rank = n;
+    }
+}
+
+// Standardize to a correlation matrix and scale the bounds
+// (src/cholesky.jl:458-473).
+__global__ void chol_scale_kernel(double* M, int64_t lda, int n, const double* __restrict__ scale,
+                                  double* a, double* b, int mirror)
+{
+    const int j = blockIdx.y;
+    const double inv_s_j = 1.0 / scale[j];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i <= j; i += gridDim.x * blockDim.x) {
+        if (i == j) {
+            MM(j, j) = 1.0;
+            a[j] *= inv_s_j;
+            b[j] *= inv_s_j;
+        } else {
+            const double val = MM(i, j) * (inv_s_j / scale[i]);
+            MM(i, j) = val;
+            if (mirror) MM(j, i) = val;
+        }
+    }
+}
+
+hipError_t chol_standardize(const CholArgs& C, bool mirror, hipStream_t s)
+{
+    hipLaunchKernelGGL(chol_diag_kernel, dim3(1), dim3(kCholThreads), 0, s, C.M, C.lda, C.n,
+                       C.scale, C.eps0, C.rank);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    dim3 g((C.n + 255) / 256, C.n);
+    hipLaunchKernelGGL(chol_scale_kernel, g, dim3(256), 0, s, C.M, C.lda, C.n, C.scale, C.a, C.b,
+                       mirror ? 1 : 0);
+    return hipGetLastError();
+}
+
+// argmin key: NaN before any number, then smaller value, then smaller index
+__device__ __forceinline__ bool better(double v, int i, double bv, int bi)
+{
+    const bool vn = isnan(v), bn = isnan(bv);
+    if (vn != bn) return vn;
+    if (!vn && v != bv) return v < bv;
+    return i < bi;
+}
+
+__global__ __launch_bounds__(kCholThreads) void chol_window_kernel(CholArgs C, int w0, int w1, int nb,
+                                                                   int pivot)
+{
+    __shared__ double s_dsum[kMaxWin], s_sv[kMaxWin], s_a[kMaxWin], s_b[kMaxWin];
+    __shared__ int s_perm[kMaxWin];
+    __shared__ double s_rv[kCholThreads / 64];
+    __shared__ int s_ri[kCholThreads / 64];
+    __shared__ double s_uii, s_yi;
+    __shared__ int s_im, s_stop;
+
+    double* M = C.M;
+    const int64_t lda = C.lda;
+    const int n = C.n;
+    const int tid = threadIdx.x;
+    const double c2 = 1.0 / sqrt(2.0);
+    const double is2pi = 2.0 / sqrt(2.0 * 3.14159265358979323846);
+    if (*C.rank < n) return;  // an earlier window found the matrix rank deficient
+    const double eps0 = *C.eps0;
+
+    for (int k = tid; k < w1 - w0; k += kCholThreads) {
+        s_sv[k] = C.s_vec[w0 + k];
+        s_a[k] = C.a[w0 + k];
+        s_b[k] = C.b[w0 + k];
+        s_perm[k] = C.perm[w0 + k];
+    }
+    if (tid == 0) s_stop = 0;
+    __syncthreads();
+
+    for (int ib0 = w0; ib0 < w1; ib0 += nb) {
+        const int ib1 = min(w1, ib0 + nb);
+        for (int k = ib0 + tid; k < w1; k += kCholThreads) s_dsum[k - w0] = 0.0;
+        __syncthreads();
+        for (int i = ib0; i < ib1; ++i) {
+            // ---- pivot search over the rest of the window (src/cholesky.jl:500-516)
+            double bv = NAN;
+            int bi = INT32_MAX;
+            double best_order = 0.0;
+            if (pivot) {
+                bv = INFINITY;
+                for (int k = i + tid; k < w1; k += kCholThreads) {
+                    double m_kk = MM(k, k) - s_dsum[k - w0];
+                    m_kk = m_kk < 0 ? 0.0 : sqrt(m_kk);
+                    const double a_k = (s_a[k - w0] - s_sv[k - w0]) / m_kk * c2;
+                    const double b_k = (s_b[k - w0] - s_sv[k - w0]) / m_kk * c2;
+                    const double o = erf(b_k) - erf(a_k);
+                    if (better(o, k, bv, bi)) {
+                        bv = o;
+                        bi = k;
+                    }
+                }
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double ov = __shfl_xor(bv, off, 64);
+                    const int oi = __shfl_xor(bi, off, 64);
+                    if (better(ov, oi, bv, bi)) {
+                        bv = ov;
+                        bi = oi;
+                    }
+                }
+                if ((tid & 63) == 0) {
+                    s_rv[tid >> 6] = bv;
+                    s_ri[tid >> 6] = bi;
+                }
+                __syncthreads();
+            }
+            if (tid == 0) {
+                int im = i;
+                if (pivot) {
+                    bv = s_rv[0];
+                    bi = s_ri[0];
+                    for (int w = 1; w < kCholThreads / 64; ++w)
+                        if (better(s_rv[w], s_ri[w], bv, bi)) {
+                            bv = s_rv[w];
+                            bi = s_ri[w];
+                        }
+                    im = bi;
+                    best_order = bv;
+                }
+                double m_ii = MM(im, im) - s_dsum[im - w0];
+                if (im != i) {
+                    const int a0 = i - w0, a1 = im - w0;
+                    int tp = s_perm[a0]; s_perm[a0] = s_perm[a1]; s_perm[a1] = tp;
+                    double t;
+                    t = s_dsum[a0]; s_dsum[a0] = s_dsum[a1]; s_dsum[a1] = t;
+                    t = s_sv[a0]; s_sv[a0] = s_sv[a1]; s_sv[a1] = t;
+                    t = s_a[a0]; s_a[a0] = s_a[a1]; s_a[a1] = t;
+                    t = s_b[a0]; s_b[a0] = s_b[a1]; s_b[a1] = t;
+                }
+                s_im = im;
+                if (pivot && !(m_ii > eps0)) {
+                    s_stop = 1;
+                    *C.rank = i;
+                } else {
+                    const double uii = sqrt(m_ii);
+                    s_uii = uii;
+                    const double a_norm = (s_a[i - w0] - s_sv[i - w0]) / uii;
+                    const double b_norm = (s_b[i - w0] - s_sv[i - w0]) / uii;
+                    s_yi = pivot ? (exp(-(a_norm * a_norm) / 2) - exp(-(b_norm * b_norm) / 2)) * is2pi / best_order
+                                 : 0.0;
+                }
+            }
+            __syncthreads();
+            const int im = s_im;
+            // ---- symmetric row/column swap i <-> im of the upper-stored matrix
+            if (im != i) {
+                // rows r < i: columns i and im
+                for (int r = tid; r < i; r += kCholThreads) {
+                    const double t = MM(r, i);
+                    MM(r, i) = MM(r, im);
+                    MM(r, im) = t;
+                }
+                // i < r < im: M[i, r] <-> M[r, im]
+                for (int r = i + 1 + tid; r < im; r += kCholThreads) {
+                    const double t = MM(i, r);
+                    MM(i, r) = MM(r, im);
+                    MM(r, im) = t;
+                }
+                // c > im: M[i, c] <-> M[im, c]
+                for (int c = im + 1 + tid; c < n; c += kCholThreads) {
+                    const double t = MM(i, c);
+                    MM(i, c) = MM(im, c);
+                    MM(im, c) = t;
+                }
+                if (tid == 0) {
+                    const double t = MM(i, i);
+                    MM(i, i) = MM(im, im);
+                    MM(im, im) = t;
+                }
+            }
+            __syncthreads();
+            if (s_stop) break;
+            const double uii = s_uii;
+            const double yi = s_yi;
+            if (tid == 0) MM(i, i) = uii;
+            // ---- row i of U inside the window (src/cholesky.jl:547-570)
+            for (int c = i + 1 + tid; c < w1; c += kCholThreads) {
+                double acc = MM(i, c);
+                for (int l = ib0; l < i; ++l) acc -= MM(l, i) * MM(l, c);
+                const double u = acc / uii;
+                MM(i, c) = u;
+                s_dsum[c - w0] += u * u;
+                s_sv[c - w0] += yi * u;
+            }
+            __syncthreads();
+        }
+        if (s_stop) break;
+        // ---- deferred update of the rest of the window (src/cholesky.jl:578-582)
+        if (ib1 < w1) {
+            const int wr = w1 - ib1;
+            for (int64_t idx = tid; idx < (int64_t)wr * wr; idx += kCholThreads) {
+                const int x = ib1 + (int)(idx % wr);
+                const int y = ib1 + (int)(idx / wr);
+                if (x <= y) {
+                    double acc = MM(x, y);
+                    for (int l = ib0; l < ib1; ++l) acc -= MM(l, x) * MM(l, y);
+                    MM(x, y) = acc;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int k = tid; k < w1 - w0; k += kCholThreads) {
+        C.s_vec[w0 + k] = s_sv[k];
+        C.a[w0 + k] = s_a[k];
+        C.b[w0 + k] = s_b[k];
+        C.perm[w0 + k] = s_perm[k];
+    }
+}
+
+hipError_t chol_window(const CholArgs& C, int w0, int w1, int nb, bool pivot, hipStream_t s)
+{
+    hipLaunchKernelGGL(chol_window_kernel, dim3(1), dim3(kCholThreads), 0, s, C, w0, w1, nb,
+                       pivot ? 1 : 0);
+    return hipGetLastError();
+}
+
+// rows [w0, w1) of U beyond the window: solve U_ww^T X = M[w0:w1, c]
+__global__ __launch_bounds__(64) void chol_trsm_kernel(double* M, int64_t lda, int n, int w0, int w1,
+                                                       const int32_t* rank)
+{
+    if (*rank < n) return;
+    const int c = w1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    for (int r = w0; r < w1; ++r) {
+        double acc = MM(r, c);
+        for (int l = w0; l < r; ++l) acc -= MM(l, r) * MM(l, c);
+        MM(r, c) = acc / MM(r, r);
+    }
+}
+
+// trailing update M[x, y] -= sum_l U[l, x] U[l, y] for w1 <= x <= y < n
+constexpr int kSyTile = 64;
+constexpr int kSyK = 16;
+__global__ __launch_bounds__(256) void chol_syrk_kernel(double* M, int64_t lda, int n, int w0, int w1,
+                                                        const int32_t* rank)
+{
+    if (*rank < n) return;
+    const int tx = blockIdx.x, ty = blockIdx.y;
+    if (tx > ty) return;
+    const int x0 = w1 + tx * kSyTile, y0 = w1 + ty * kSyTile;
+    __shared__ double Ax[kSyK][kSyTile + 1], Ay[kSyK][kSyTile + 1];
+    const int t = threadIdx.x;
+    const int tr = t % 16, tc = t / 16;
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+    for (int l0 = w0; l0 < w1; l0 += kSyK) {
+        for (int e = t; e < kSyK * kSyTile; e += 256) {
+            const int ll = e % kSyK, cc = e / kSyK;
+            const int l = l0 + ll;
+            Ax[ll][cc] = (l < w1 && x0 + cc < n) ? MM(l, x0 + cc) : 0.0;
+            Ay[ll][cc] = (l < w1 && y0 + cc < n) ? MM(l, y0 + cc) : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int ll = 0; ll < kSyK; ++ll) {
+            double xa[4], ya[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) xa[i] = Ax[ll][tr * 4 + i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) ya[j] = Ay[ll][tc * 4 + j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = fma(xa[i], ya[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int x = x0 + tr * 4 + i, y = y0 + tc * 4 + j;
+            if (x <= y && y < n) MM(x, y) -= acc[i][j];
+        }
+}
+
+hipError_t chol_trsm_syrk(const CholArgs& C, int w0, int w1, hipStream_t s)
+{
+    const int rest = C.n - w1;
+    if (rest <= 0) return hipSuccess;
+    hipLaunchKernelGGL(chol_trsm_kernel, dim3((rest + 63) / 64), dim3(64), 0, s, C.M, C.lda, C.n, w0,
+                       w1, C.rank);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int nt = (rest + kSyTile - 1) / kSyTile;
+    hipLaunchKernelGGL(chol_syrk_kernel, dim3(nt, nt), dim3(256), 0, s, C.M, C.lda, C.n, w0, w1,
+                       C.rank);
+    return hipGetLastError();
+}
+
+__global__ void chol_zero_lower_kernel(double* M, int64_t lda, int n)
+{
+    const int j = blockIdx.y;
+    for (int i = j + 1 + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        MM(i, j) = 0.0;
+}
+
+hipError_t chol_finish(const CholArgs& C, hipStream_t s)
+{
+    dim3 g((C.n + 255) / 256, C.n);
+    hipLaunchKernelGGL(chol_zero_lower_kernel, g, dim3(256), 0, s, C.M, C.lda, C.n);
+    return hipGetLastError();
+}
+
+#undef MM
+}  // namespace dbg

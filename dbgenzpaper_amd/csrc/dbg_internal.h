@@ -1,0 +1,115 @@
+// Internal (non-ABI) declarations shared by the kernel translation units and
+// the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dbg {
+
+enum { kGenRichtmyer = 0, kGenSobol = 1 };
+enum { kPhiErf = 0, kPhiErfc = 1 };
+
+// Where the points of one round (one block of m points per shift) come from.
+struct PointArgs {
+    int nd;                    // sampled dimensions (n - 1)
+    const double* q;           // Richtmyer: frac(sqrt(p_g)), nd entries
+    const double* shifts_f;    // Richtmyer: nd x n_reps shifts in [0, 1)
+    int64_t n_base;            // Richtmyer: point j uses n_base + j (j 1-based)
+    const uint32_t* V;         // Sobol: nd x 32 direction integers
+    const uint64_t* shifts_u;  // Sobol: nd x n_reps digital shifts
+    uint64_t sobol_first;      // Sobol: point j uses sequence index sobol_first + j
+    uint64_t mask;             // Sobol: 2^L - 1
+    double invscale;           // Sobol: 2^-L
+};
+
+// Arguments of the dense integration kernels.
+struct DenseArgs {
+    int n;                     // dimension d
+    int64_t m;                 // points per shift in this round
+    int n_reps;
+    int64_t tiles_per_rep;     // ceil(m / tile)
+    int64_t n_tiles;           // tiles_per_rep * n_reps
+    const void* Uk;            // packed strict-upper rows (T), Uk[l*ldu + j] = U[l, j] for j > l else 0
+    int64_t ldu;
+    const void* ak;            // per target j: a_j (T)
+    const void* bk;            // per target j: b_j (T)
+    const void* invk;          // per target j: 1 / (U_jj * sqrt 2) (T)
+    double c1, dc1;            // first dimension (FP64 path)
+    float ea1, eb1, d1;        // first dimension (mixed path)
+    int mode1;
+    PointArgs pts;
+    void* Y;                   // workspace: one slot of nd x tile samples per resident block
+    int64_t y_slot;            // elements per slot
+    double* part_m;            // per-tile partial sums (scaled)
+    int* part_e;
+};
+
+// Arguments of the sparse (CSC) integration kernel.
+struct SparseArgs {
+    int n;
+    int64_t m;
+    int n_reps;
+    int64_t tiles_per_rep;
+    int64_t n_tiles;
+    const int64_t* colptr;     // n + 1, 0-based
+    const int32_t* rowval;     // 0-based rows
+    const void* nzval;         // T
+    const void* ak;
+    const void* bk;
+    const void* invk;
+    double c1, dc1;
+    float ea1, eb1, d1;
+    int mode1;
+    int tent;                  // 1: src/qmc.jl form, 0: SparseQMC (no tent)
+    PointArgs pts;
+    void* Y;
+    int64_t y_slot;
+    double* part_m;
+    int* part_e;
+};
+
+hipError_t launch_dense_valu(const DenseArgs& A, bool mixed, int gen, int form, int grid,
+                             hipStream_t s);
+int dense_valu_tile();
+int dense_valu_width();
+
+hipError_t launch_sparse(const SparseArgs& A, bool mixed, int gen, int grid, hipStream_t s);
+int sparse_tile();
+
+// Per-shift reduction of the tile partials and running-mean merge
+// (src/qmc.jl:434-442); then mean and 3*sd over shifts (:303-312, 814-815).
+hipError_t launch_reduce_reps(const double* part_m, const int* part_e, int64_t tiles_per_rep,
+                              int n_reps, int64_t n_pts0, int64_t m, int sparseqmc_merge,
+                              double* rep_m, int* rep_e, double* out, hipStream_t s);
+
+// Cholesky building blocks (FP64, column-major, lda).
+struct CholArgs {
+    double* M;
+    int64_t lda;
+    int n;
+    double* a;
+    double* b;
+    double* scale;
+    int32_t* perm;
+    double* s_vec;
+    double* eps0;      // device scalar
+    int32_t* rank;     // device scalar (n while full rank)
+};
+hipError_t chol_standardize(const CholArgs& C, bool mirror, hipStream_t s);
+hipError_t chol_window(const CholArgs& C, int w0, int w1, int nb, bool pivot, hipStream_t s);
+hipError_t chol_trsm_syrk(const CholArgs& C, int w0, int w1, hipStream_t s);
+hipError_t chol_finish(const CholArgs& C, hipStream_t s);
+int chol_max_window();
+
+// Post-factor preparation.
+hipError_t truncate_upper(double* U, int64_t lda, int n, double min_val, hipStream_t s);
+hipError_t pack_dense(const double* U, int64_t lda, int n, const double* a, const double* b,
+                      bool mixed, void* Uk, int64_t ldu, void* ak, void* bk, void* invk,
+                      double* first /* c1, dc1, ea1, eb1, d1, mode1 */, int phi_form,
+                      hipStream_t s);
+hipError_t csc_count(const double* U, int64_t lda, int n, int64_t* colcount, hipStream_t s);
+hipError_t csc_fill(const double* U, int64_t lda, int n, const int64_t* colptr, bool mixed,
+                    int32_t* rowval, void* nzval, hipStream_t s);
+hipError_t exclusive_scan_i64(const int64_t* in, int64_t* out, int n, hipStream_t s);
+
+}  // namespace dbg

@@ -1,0 +1,163 @@
+/*
+ * dbgenz.h -- C ABI of the MI355X-native Genz quasi-Monte Carlo
+ * multivariate-normal probability library (libdbgenz.so).
+ *
+ * The entry points replace the hot path of JakobPeterlin/DBGenzPaper, a pure
+ * Julia package.  Each declaration names the reference routine it stands in
+ * for (paths relative to the reference repository root).  A Julia caller binds
+ * them with ccall; the binding is spelled out in INTEGRATION.md.
+ *
+ * Conventions
+ *   - Matrices are column-major with an explicit leading dimension, like a
+ *     Julia Matrix{Float64}.  Permutations are 0-based (Julia adds 1).
+ *   - Host pointers unless the name ends in _dev.
+ *   - Every function returns a status code (DBG_OK on success);
+ *     dbg_last_error() returns a message for the calling thread.
+ *   - A plan owns its device buffers and its HIP stream.  Calls on one plan
+ *     are ordered; distinct plans are independent and may be used from
+ *     different host threads.  No call keeps global mutable state.
+ */
+#ifndef DBGENZ_H
+#define DBGENZ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define DBG_OK 0
+#define DBG_ERR_ARG 1        /* invalid argument (size, option, null pointer) */
+#define DBG_ERR_RANK 2       /* factor not of full rank: QMCData's ArgumentError (src/qmc.jl:195-197) */
+#define DBG_ERR_DEVICE 3     /* HIP runtime error or out of device memory */
+#define DBG_ERR_NONFINITE 4  /* result is NaN */
+#define DBG_ERR_STATE 5      /* call out of order (e.g. no shifts set) */
+
+/* precision of the integration (the Julia type parameter T) */
+#define DBG_PREC_FP64 0   /* T = Float64: the reference arithmetic in FP64 */
+#define DBG_PREC_MIXED 1  /* T = Float32: FP32 contraction and special functions,
+                             FP64 factor, points and reductions */
+/* point set (qmc_type) */
+#define DBG_GEN_RICHTMYER 0
+#define DBG_GEN_SOBOL 1
+/* form of the normal CDF in the loop */
+#define DBG_PHI_ERF 0   /* QMCData (src/qmc.jl:405-429) */
+#define DBG_PHI_ERFC 1  /* QMCData_Distributions (src/qmc.jl:579-593) */
+/* which data structure / loop */
+#define DBG_KIND_DENSE 0      /* QMCData: pivoted factor, dense loop */
+#define DBG_KIND_SPARSE 1     /* QMCDataSparse: unpivoted factor, CSC loop */
+#define DBG_KIND_SPARSEQMC 2  /* SparseQMC.QMCDataSparse: Sobol, no tent map */
+
+/* Options: QMCOpts{T} (src/qmc.jl:15-27, defaults of QMC_opts :111-125) plus
+ * the choices the Julia constructors take as arguments. */
+typedef struct dbg_opts {
+    int32_t chol_block_size;  /* bs1 of cholesky_genz!/cholesky_classic! */
+    int32_t chol_block_size2; /* bs2: the pivot window */
+    int32_t chol_block_size3; /* bs3 (Julia default 2^10) */
+    int32_t chol_block_size4; /* bs4 (Julia default 2^11) */
+    int64_t m;                /* points per shift and round */
+    int32_t block_size_i;     /* accepted for API parity; the device picks its own tiling */
+    int32_t block_size_i2;    /* accepted for API parity */
+    int32_t block_size_j;     /* accepted for API parity */
+    int32_t n_blocks;         /* accepted for API parity (CPU thread slots) */
+    int32_t n_reps;           /* number of random shifts */
+    int32_t n_bits;           /* Sobol bits L */
+    int64_t max_pts;          /* adaptive stop: total points per shift */
+    double max_abs_err;       /* adaptive stop: 3 * standard error */
+    int32_t precision;        /* DBG_PREC_* */
+    int32_t generator;        /* DBG_GEN_* */
+    int32_t phi_form;         /* DBG_PHI_* */
+    int32_t kind;             /* DBG_KIND_* */
+    int64_t sobol_skip;       /* next! calls before the first Sobol point; < 0: as skip(s, 2m-1) */
+} dbg_opts;
+
+/* Result of qmc_pnorm!: (value, err, n_pts) of src/qmc.jl:829,849 plus the
+ * log of the value, which stays finite when P is below the Float64 range. */
+typedef struct dbg_result {
+    double value;     /* P, 0 if it underflows Float64 */
+    double log_value; /* log P */
+    double err3;      /* 3 * standard error over the shifts (src/qmc.jl:815) */
+    double rel_err3;  /* err3 / value, finite when both underflow */
+    int64_t n_pts;    /* points per shift used */
+    int32_t rounds;   /* qmc_loop! calls made */
+} dbg_result;
+
+typedef struct dbg_plan dbg_plan;
+
+/* message of the last failed call on this thread ("" if none) */
+const char* dbg_last_error(void);
+/* ABI version, bumped on incompatible changes */
+int32_t dbg_abi_version(void);
+/* QMC_opts(T) defaults (src/qmc.jl:111-125) */
+void dbg_opts_default(dbg_opts* opts, int32_t precision);
+
+/* cholesky_genz!(M, a, b, bs1, bs2, bs3, bs4)  src/cholesky.jl:445-628
+ * In place on host arrays: M (n x n, lda) receives the standardized pivoted
+ * factor U in its upper triangle (strict lower triangle zeroed), a and b are
+ * standardized and permuted.  perm (0-based), scale (scale_vec) and rank are
+ * outputs.  Runs on the device in FP64. */
+int dbg_chol_genz_f64(double* M, int64_t lda, int32_t n, double* a, double* b, int32_t bs1,
+                      int32_t bs2, int32_t bs3, int32_t bs4, int32_t* perm, double* scale,
+                      int32_t* rank);
+
+/* cholesky_classic!(M, a, b, bs1, bs2, bs3, bs4)  src/cholesky.jl:144-242
+ * Unpivoted, in place on host arrays, FP64 on the device. */
+int dbg_chol_classic_f64(double* M, int64_t lda, int32_t n, double* a, double* b, int32_t bs1,
+                         int32_t bs2, int32_t bs3, int32_t bs4, double* scale);
+
+/* QMCData(C, a, b, opts, rng, qmc_type)        src/qmc.jl:149-202
+ * QMCDataSparse(C, a, b, opts, rng, qmc_type)  src/qmc.jl:225-282
+ * SparseQMC.QMCDataSparse(C, a, b; opts)       src/qmc_sparse.jl:58-90
+ * Copies Sigma (n x n, lda), a and b to the device, factors (pivoted for
+ * DBG_KIND_DENSE, unpivoted otherwise), applies truncate_matrixU! and builds
+ * the packed / CSC layouts.  The random shifts are set separately (the RNG
+ * stays with the caller).  Fails with DBG_ERR_RANK for a rank-deficient
+ * pivoted factor. */
+int dbg_plan_create(dbg_plan** plan, const dbg_opts* opts, int32_t n, const double* Sigma,
+                    int64_t lda, const double* a, const double* b);
+int dbg_plan_destroy(dbg_plan* plan);
+
+/* Redo the constructor's factorization (cholesky_genz! / cholesky_classic!,
+ * truncate_matrixU!, packed and CSC layouts) from the device-resident copies
+ * of Sigma, a and b: the factor part of QMCData (src/qmc.jl:189-199) for
+ * callers that time or repeat the whole job with its inputs in HBM. */
+int dbg_plan_refactor(dbg_plan* plan);
+
+/* Random shifts: rand(rng, T, n-1, n_reps) (src/qmc_generators.jl:52) as
+ * double, or rand(rng, UInt64, n-1, n_reps) (:63) for Sobol; column-major
+ * (n-1) x n_reps.  For DBG_PREC_MIXED the double shifts are used as given. */
+int dbg_plan_set_shifts(dbg_plan* plan, const void* shifts);
+/* Sobol direction integers, (n-1) x 32 row-major: V[d*32+k] = m_{k+1} << (31-k) */
+int dbg_plan_set_sobol_directions(dbg_plan* plan, const uint32_t* V);
+
+/* qmc_pnorm!(D)  src/qmc.jl:783-850 (SparseQMC.qmc_pnorm! src/qmc_sparse.jl:272-317
+ * for DBG_KIND_SPARSEQMC).  Runs rounds of the integration until
+ * n_pts >= max_pts or err3 < max_abs_err; the generator state carries over
+ * between calls as in the reference. */
+int dbg_qmc_pnorm(dbg_plan* plan, dbg_result* out);
+
+/* One qmc_loop!(D, n_pts0, c_1, dc_1) (src/qmc.jl:315-444) over the current
+ * block of points, queued on the plan's stream without waiting: the timed
+ * hot path.  dbg_plan_sync waits for it. */
+int dbg_qmc_loop(dbg_plan* plan, int64_t n_pts0);
+int dbg_plan_sync(dbg_plan* plan);
+/* device time of the integration kernel of the last dbg_qmc_loop (HIP events
+ * on the plan's stream) */
+int dbg_plan_last_kernel_ms(dbg_plan* plan, float* ms);
+/* summary of the current per-shift means (after dbg_qmc_loop) */
+int dbg_plan_summary(dbg_plan* plan, dbg_result* out);
+
+/* The factor and the shift means held by the plan. */
+int dbg_plan_get_factor(dbg_plan* plan, double* U, int32_t* perm, double* a, double* b,
+                        double* scale, int32_t* rank);
+int dbg_plan_get_reps(dbg_plan* plan, double* rep_mant, int32_t* rep_exp);
+/* generator block state: Richtmyer n_base = state * m, Sobol first index =
+ * skip + state * m; for DBG_KIND_SPARSEQMC the number of blocks generated */
+int dbg_plan_get_state(dbg_plan* plan, int64_t* state);
+int dbg_plan_set_state(dbg_plan* plan, int64_t state);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DBGENZ_H */

@@ -1,0 +1,46 @@
+"""Shared test helpers: run the CPU oracle on exactly the inputs a device
+plan holds (same factor, bounds, points, shifts and generator state)."""
+from __future__ import annotations
+
+import numpy as np
+
+from dbgenzpaper_amd import _lib as L
+from dbgenzpaper_amd.sobol import direction_integers, skip_count
+
+
+def oracle_same_inputs(O, D, *, kind=None, phi=None, gen_state=None, max_pts=None, max_abs_err=None,
+                       U=None, a=None, b=None):
+    """qmc_pnorm! of the oracle on the plan's own factor and shifts."""
+    F = D.C
+    opts = D.qmc_opts
+    n = D.n
+    if kind is None:
+        kind = {L.KIND_DENSE: "dense", L.KIND_SPARSE: "sparse", L.KIND_SPARSEQMC: "sparseqmc"}[D._kind]
+    if phi is None:
+        phi = "erfc" if D._phi == L.PHI_ERFC else "erf"
+    gen = "Sobol" if (D.qmc_type == "Sobol" or kind == "sparseqmc") else "Richtmyer"
+    V = direction_integers(max(n - 1, 1)) if gen == "Sobol" else None
+    skip = D._sobol_skip if D._sobol_skip is not None else skip_count(2 * opts.m - 1)
+    bs_i, bs_i2 = max(opts.block_size_i, 1), max(opts.block_size_i2, 1)
+    return O.qmc_pnorm(F.U if U is None else U, F.a if a is None else a, F.b if b is None else b,
+                       m=opts.m, n_reps=opts.n_reps, gen=gen, phi_form=phi, kind=kind,
+                       bs_i=bs_i, bs_i2=bs_i2, bs_j=opts.block_size_j, n_bits=opts.n_bits,
+                       max_pts=opts.max_pts if max_pts is None else max_pts,
+                       max_abs_err=opts.max_abs_err if max_abs_err is None else max_abs_err,
+                       shifts=D.shifts, V=V, sobol_skip=skip, gen_state=gen_state)
+
+
+def rel(x, y):
+    return abs(x - y) / max(abs(y), 1e-300)
+
+
+def random_spd(n, seed, jitter=100.0):
+    """C = A A^T + jitter I with A ~ U(0,1)^{n x n} (test/runtests.jl:16-19)."""
+    rng = np.random.default_rng(seed)
+    A = rng.random((n, n))
+    return np.asfortranarray(A @ A.T + jitter * np.eye(n))
+
+
+def standardized(C):
+    d = np.sqrt(np.diag(C))
+    return C / np.outer(d, d)

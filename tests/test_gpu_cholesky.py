@@ -1,0 +1,101 @@
+"""Device Cholesky factorizations against the oracle and the reference's own
+invariants (test/runtests.jl:15-59)."""
+import numpy as np
+import pytest
+
+from tests._helpers import random_spd, standardized
+
+pytestmark = pytest.mark.gpu
+
+
+def _genz_pair(dbg, oracle, C, a, b, bs):
+    Fo = oracle.cholesky_genz(C, a, b, *bs)
+    Fg = dbg.cholesky_genz(C.copy(order="F"), a.copy(), b.copy(), *bs)
+    return Fo, Fg
+
+
+@pytest.mark.parametrize("n,bs,seed", [
+    (16, (2 ** 7, 2 ** 9, 2 ** 10, 2 ** 11), 1),
+    (100, (8, 32, 64, 128), 2),           # several windows, inner blocks, iii/iv nesting
+    (300, (16, 64, 128, 256), 3),
+    (1024, (2 ** 7, 2 ** 9, 2 ** 10, 2 ** 11), 101),  # the reference test's shape
+])
+def test_genz_matches_oracle(dbg, oracle, n, bs, seed):
+    C = random_spd(n, seed)
+    rng = np.random.default_rng(seed + 7)
+    a = -rng.uniform(0.5, 2.0, n)
+    b = rng.uniform(0.5, 2.0, n)
+    print(f"genz n={n} bs={bs} seed={seed}", flush=True)
+    Fo, Fg = _genz_pair(dbg, oracle, C, a, b, bs)
+    assert Fg.rank == n == Fo.rank
+    if not np.array_equal(Fg.perm, Fo.perm):
+        k = int(np.argmax(Fg.perm != Fo.perm))
+        pytest.fail(f"perm differs first at step {k}; oracle best/second gap {Fo.gaps[k]:.3e}")
+    Uo, Ug = np.triu(Fo.U), np.triu(Fg.U)
+    assert np.abs(Ug - Uo).max() < 1e-12
+    assert np.allclose(Fg.a, Fo.a, rtol=1e-14, atol=0) and np.allclose(Fg.b, Fo.b, rtol=1e-14, atol=0)
+    assert np.allclose(Fg.scale_vec, Fo.scale_vec, rtol=0, atol=0)
+    Cs = standardized(C)
+    p = Fg.perm
+    assert np.abs(Cs[np.ix_(p, p)] - Ug.T @ Ug).max() < 1e-10   # test/runtests.jl:35
+
+
+def test_genz_reference_invariant_ones_bounds(dbg):
+    """test/runtests.jl:15-36 with a = -1, b = 1."""
+    n = 1024
+    C = random_spd(n, 101)
+    F = dbg.cholesky_genz(C.copy(order="F"), -np.ones(n), np.ones(n))
+    U = np.triu(F.U)
+    assert np.abs(standardized(C)[np.ix_(F.perm, F.perm)] - U.T @ U).max() < 1e-10
+
+
+def test_genz_identity_is_windowed_selection_sort(dbg):
+    n = 40
+    rng = np.random.default_rng(5)
+    b = rng.uniform(0.1, 2.0, n)
+    a = -b * rng.uniform(0.5, 1.5, n)
+    F = dbg.cholesky_genz(np.eye(n), a.copy(), b.copy(), 4, 16, 32, 64)
+    from scipy.special import ndtr
+
+    mass = ndtr(b) - ndtr(a)
+    exp = []
+    for w0 in range(0, n, 16):
+        exp += sorted(range(w0, min(n, w0 + 16)), key=lambda k: (mass[k], k))
+    assert list(F.perm) == exp
+
+
+def test_genz_exchangeable_identity_perm(dbg):
+    n = 64
+    C = np.eye(n) + np.ones((n, n))
+    F = dbg.cholesky_genz(C, -np.ones(n), np.ones(n), 8, 32)
+    assert list(F.perm) == list(range(n))
+
+
+def test_genz_rank_deficient(dbg, oracle):
+    n = 30
+    rng = np.random.default_rng(9)
+    B = rng.standard_normal((n, 20))
+    C = B @ B.T
+    Fo = oracle.cholesky_genz(C, -np.ones(n), np.ones(n))
+    Fg = dbg.cholesky_genz(C.copy(order="F"), -np.ones(n), np.ones(n))
+    assert Fg.rank == Fo.rank < n
+
+
+def test_qmcdata_rank_error(dbg):
+    n = 10
+    C = np.ones((n, n))
+    with pytest.raises(ValueError, match="full-rank"):
+        dbg.QMCData(C, -np.ones(n), np.ones(n), dbg.QMC_opts(m=256, max_pts=256), 0)
+
+
+@pytest.mark.parametrize("bs", [8, 16, 32, 64])
+def test_classic_matches_oracle(dbg, oracle, bs):
+    """test/runtests.jl:38-59 (n = 100, equal block sizes at all levels)."""
+    n = 100
+    C = random_spd(n, 102)
+    a, b = -np.ones(n), np.ones(n)
+    Fo = oracle.cholesky_classic(C, a, b, bs, bs, bs, bs)
+    Fg = dbg.cholesky_classic(C.copy(order="F"), a.copy(), b.copy(), bs, bs, bs, bs)
+    Ug = np.triu(Fg.U)
+    assert np.abs(standardized(C) - Ug.T @ Ug).max() < 1e-10
+    assert np.abs(Ug - np.triu(Fo.U)).max() < 1e-12

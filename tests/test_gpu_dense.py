@@ -1,0 +1,114 @@
+"""Dense integration on the device against the oracle on the same factor,
+points and shifts (FP64 path: reference arithmetic; mixed path: within a
+stated tolerance)."""
+import numpy as np
+import pytest
+from scipy.special import ndtr
+
+from dbgenzpaper_amd import workloads as W
+from tests._helpers import oracle_same_inputs, rel
+
+pytestmark = pytest.mark.gpu
+
+F64_TOL = 1e-12      # relative, same factor / points / shifts
+
+
+def _plan(dbg, P, T=np.float64, gen="Richtmyer", **ow):
+    kw = dict(m=P.m, n_reps=P.n_reps, max_pts=P.m, max_abs_err=0.0)
+    kw.update(ow)
+    opts = dbg.QMC_opts(T, **kw)
+    return dbg.QMCData(P.Sigma, P.a, P.b, opts, None, gen, shifts=P.shifts)
+
+
+def test_config0_fp64(dbg, oracle):
+    """d = 16 random correlation, a = -inf, b ~ U(0, 2), N = 2^14 x 8."""
+    P = W.dense_random(16)
+    print("config0 fp64", flush=True)
+    D = _plan(dbg, P)
+    v, e, npts = dbg.qmc_pnorm(D)
+    r = oracle_same_inputs(oracle, D)
+    assert npts == P.m
+    assert rel(v, r.value) < F64_TOL, (v, r.value)
+    assert rel(e, r.err3) < 1e-8
+    assert np.allclose(D.qmc_reps, r.rep_means, rtol=F64_TOL, atol=0)
+
+
+@pytest.mark.parametrize("d,m,R", [(2, 1000, 3), (3, 257, 2), (17, 999, 4), (33, 2048, 2), (100, 700, 3)])
+def test_ragged_shapes_fp64(dbg, oracle, d, m, R):
+    P = W.dense_random(d, m=m, n_reps=R, seed=d)
+    print(f"ragged d={d} m={m} R={R}", flush=True)
+    D = _plan(dbg, P)
+    v, _, _ = dbg.qmc_pnorm(D)
+    r = oracle_same_inputs(oracle, D)
+    assert rel(v, r.value) < F64_TOL
+
+
+def test_erfc_form(dbg, oracle):
+    P = W.dense_random(24, m=4096, n_reps=4, seed=11)
+    D = _plan(dbg, P)
+    De = dbg.QMCData_Distributions(D)
+    v, _, _ = dbg.qmc_pnorm(De)
+    r = oracle_same_inputs(oracle, De)
+    assert rel(v, r.value) < F64_TOL
+    v0, _, _ = dbg.qmc_pnorm(D)
+    assert abs(v - v0) < 4e-14   # erf vs erfc form on the same points (data/sims/table_precision.tex)
+
+
+def test_sobol(dbg, oracle):
+    P = W.dense_random(20, m=2048, n_reps=3, seed=4, gen="Sobol")
+    D = _plan(dbg, P, gen="Sobol")
+    v, _, _ = dbg.qmc_pnorm(D)
+    r = oracle_same_inputs(oracle, D)
+    assert rel(v, r.value) < F64_TOL
+
+
+def test_adaptive_rounds_and_state(dbg, oracle):
+    """max_abs_err = 0 forces max_pts/m rounds (test/runtests.jl:157-167);
+    the generator state carries over to the next call like the reference's."""
+    P = W.dense_random(8, m=2 ** 11, n_reps=12, seed=42)
+    D = _plan(dbg, P, max_pts=2 ** 11 * 4)
+    v, e, npts = dbg.qmc_pnorm(D)
+    assert npts == 2 ** 13 and D.last_result.rounds == 4
+    r = oracle_same_inputs(oracle, D, gen_state=0)
+    assert r.rounds == 4 and rel(v, r.value) < F64_TOL and rel(e, r.err3) < 1e-8
+    assert D.gen_state == 3
+    v2, _, _ = dbg.qmc_pnorm(D)
+    r2 = oracle_same_inputs(oracle, D, gen_state=3)
+    assert rel(v2, r2.value) < F64_TOL and D.gen_state == 6
+
+
+def test_identity_closed_form(dbg):
+    n = 12
+    rng = np.random.default_rng(3)
+    b = rng.uniform(0.2, 2, n)
+    a = -rng.uniform(0.2, 2, n)
+    opts = dbg.QMC_opts(np.float64, m=4096, n_reps=4, max_pts=4096)
+    v, e, _ = dbg.qmc_pnorm(dbg.QMCData(np.eye(n), a, b, opts, 1))
+    exact = np.prod(ndtr(b) - ndtr(a))
+    assert rel(v, exact) < 1e-13 and e < 1e-15
+
+
+def test_underflow_log_domain(dbg, oracle):
+    """Lower orthant of a 500-dim factor model: P is far below 1e-308; the
+    log of the estimate matches the oracle's.  The erfc form is used: with
+    the erf form, 1 + erf(x) is exactly 0 in Float64 once x < -6, so every
+    weight of this problem becomes 0 (in the oracle as in the reference)."""
+    P = W.factor_model(500, m=1024, n_reps=3)
+    D = dbg.QMCData_Distributions(_plan(dbg, P))
+    dbg.qmc_pnorm(D)
+    r = oracle_same_inputs(oracle, D)
+    lv = D.last_result.log_value
+    print("log P", lv, r.log_value, flush=True)
+    assert np.isfinite(lv) and abs(lv - r.log_value) < 1e-9 * abs(r.log_value)
+
+
+@pytest.mark.parametrize("gen", ["Richtmyer", "Sobol"])
+def test_mixed_within_tolerance(dbg, oracle, gen):
+    P = W.dense_random(64, m=8192, n_reps=4, seed=21, gen=gen)
+    D = _plan(dbg, P, T=np.float32, gen=gen)
+    v, e, _ = dbg.qmc_pnorm(D)
+    # FP64 oracle on the same points, shifts and (FP64) factor of an FP64 plan
+    D64 = _plan(dbg, P, gen=gen)
+    r = oracle_same_inputs(oracle, D64)
+    print(f"mixed {gen} rel err {rel(v, r.value):.3e} (se {r.err3 / 3 / r.value:.3e})", flush=True)
+    assert rel(v, r.value) < 2e-5
