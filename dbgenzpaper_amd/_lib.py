@@ -20,7 +20,7 @@ DBG_ERR_STATE = 5
 
 PREC_FP64, PREC_MIXED = 0, 1
 GEN_RICHTMYER, GEN_SOBOL = 0, 1
-PHI_ERF, PHI_ERFC = 0, 1
+PHI_ERF, PHI_ERFC, PHI_TAIL = 0, 1, 2
 KIND_DENSE, KIND_SPARSE, KIND_SPARSEQMC = 0, 1, 2
 
 # every exported symbol of include/dbgenz.h
@@ -29,7 +29,8 @@ EXPORTS = (
     "dbg_chol_classic_f64", "dbg_plan_create", "dbg_plan_destroy", "dbg_plan_set_shifts",
     "dbg_plan_set_sobol_directions", "dbg_qmc_pnorm", "dbg_qmc_loop", "dbg_plan_sync",
     "dbg_plan_last_kernel_ms", "dbg_plan_summary", "dbg_plan_get_factor", "dbg_plan_get_reps",
-    "dbg_plan_get_state", "dbg_plan_set_state", "dbg_plan_refactor",
+    "dbg_plan_get_state", "dbg_plan_set_state", "dbg_plan_refactor", "dbg_batch_create",
+    "dbg_batch_run", "dbg_batch_last_ms", "dbg_batch_tuning", "dbg_batch_destroy",
 )
 
 
@@ -92,6 +93,11 @@ def lib():
     L.dbg_plan_get_state.argtypes = [vp, i64p]
     L.dbg_plan_set_state.argtypes = [vp, C.c_int64]
     L.dbg_plan_refactor.argtypes = [vp]
+    L.dbg_batch_create.argtypes = [C.POINTER(vp), C.POINTER(Opts), C.c_int32, ip32, dp, dp, dp, dp]
+    L.dbg_batch_run.argtypes = [vp, dp]
+    L.dbg_batch_last_ms.argtypes = [vp, C.POINTER(C.c_float)]
+    L.dbg_batch_tuning.argtypes = [vp, ip32, C.POINTER(C.c_float)]
+    L.dbg_batch_destroy.argtypes = [vp]
     for name in EXPORTS:
         if name not in ("dbg_last_error", "dbg_abi_version", "dbg_opts_default"):
             getattr(L, name).restype = C.c_int

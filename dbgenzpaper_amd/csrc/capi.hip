@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/dbgenz.h"
@@ -49,6 +50,29 @@ std::vector<std::pair<int, int>> pivot_windows(int n, int bs2, int bs3, int bs4)
         }
     }
     return w;
+}
+
+// Lattice generating vector q_g = frac(sqrt(p_g)), p_1 = 2, p_2 = 3, ...
+// (get_sqrt_primes!, src/qmc_generators.jl:300-310).  Setup-time host work,
+// like the reference's constructor; at least one entry.
+std::vector<double> sqrt_prime_fracs(int nd)
+{
+    std::vector<double> qh(std::max(nd, 1), 0.0);
+    int found = 0;
+    long p = 1;
+    while (found < nd) {
+        long c = p + 1;
+        for (;; ++c) {
+            bool is_p = c >= 2;
+            for (long f = 2; f * f <= c && is_p; ++f)
+                if (c % f == 0) is_p = false;
+            if (is_p) break;
+        }
+        p = c;
+        const double sq = sqrt((double)p);
+        qh[found++] = sq - floor(sq);
+    }
+    return qh;
 }
 
 struct CholBuffers {
@@ -165,6 +189,7 @@ struct dbg_plan {
     int32_t rank = 0;
     // packed layouts
     void* Uk = nullptr;
+    float* ucol32 = nullptr;  // FP32 copy of the factor for the mixed walk
     int64_t ldu = 0;
     void* ak = nullptr;
     void* bk = nullptr;
@@ -189,6 +214,7 @@ struct dbg_plan {
     int64_t y_slot = 0;
     int grid = 0;
     int tile = 256;
+    bool use_mfma = false;
     int64_t tiles_per_rep = 0, n_tiles = 0;
     double* part_m = nullptr;
     int* part_e = nullptr;
@@ -199,7 +225,7 @@ struct dbg_plan {
     void release()
     {
         F.release();
-        void* ptrs[] = {sig, a0, b0, Uk, ak, bk, invk, first, colcnt, colptr, rowval, nzval, q, shifts_f,
+        void* ptrs[] = {sig, a0, b0, Uk, ucol32, ak, bk, invk, first, colcnt, colptr, rowval, nzval, q, shifts_f,
                         shifts_u, V, Y, part_m, part_e, rep_m, rep_e, out};
         for (void* p : ptrs) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -244,8 +270,12 @@ int plan_factor(dbg_plan* P)
         P->nnz = nnz;
         HIPCHK(csc_fill(P->F.M, n, n, P->colptr, P->mixed, P->rowval, P->nzval, s));
     }
+    if (o.kind == DBG_KIND_DENSE && P->mixed) {
+        if (P->ucol32 == nullptr) HIPCHK(dalloc(&P->ucol32, (size_t)n * n * sizeof(float)));
+        HIPCHK(convert_f32(P->F.M, (size_t)n * n, P->ucol32, s));
+    }
     HIPCHK(pack_dense(P->F.M, n, n, P->F.a, P->F.b, P->mixed, P->Uk, P->ldu, P->ak, P->bk, P->invk,
-                      P->first, o.kind == DBG_KIND_DENSE ? o.phi_form : DBG_PHI_ERF, s));
+                      P->first, o.phi_form == DBG_PHI_ERFC ? DBG_PHI_ERFC : DBG_PHI_ERF, s));
     HIPCHK(hipMemcpyAsync(P->first_h, P->first, 6 * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     return DBG_OK;
@@ -273,7 +303,8 @@ int plan_build(dbg_plan* P, const double* Sigma, int64_t lda, const double* a, c
     HIPCHK(dalloc(&P->invk, n * T));
     HIPCHK(dalloc(&P->first, 8 * sizeof(double)));
     if (o.kind == DBG_KIND_DENSE) {
-        P->ldu = (int64_t)n + 64;
+        // rows padded so a panel of targets past n reads zeros
+        P->ldu = (int64_t)n + dense_mfma_panel(P->mixed) + 32;
         HIPCHK(dalloc(&P->Uk, (size_t)std::max(nd, 1) * P->ldu * T));
     } else {
         HIPCHK(dalloc(&P->colcnt, (size_t)n * sizeof(int64_t)));
@@ -282,24 +313,7 @@ int plan_build(dbg_plan* P, const double* Sigma, int64_t lda, const double* a, c
     rc = plan_factor(P);
     if (rc) return rc;
 
-    // lattice generating vector frac(sqrt(p_g)) (src/qmc_generators.jl:300-310)
-    std::vector<double> qh(std::max(nd, 1), 0.0);
-    {
-        int found = 0;
-        long p = 1;
-        while (found < nd) {
-            long c = p + 1;
-            for (;; ++c) {
-                bool is_p = c >= 2;
-                for (long f = 2; f * f <= c && is_p; ++f)
-                    if (c % f == 0) is_p = false;
-                if (is_p) break;
-            }
-            p = c;
-            const double sq = sqrt((double)p);
-            qh[found++] = sq - floor(sq);
-        }
-    }
+    const std::vector<double> qh = sqrt_prime_fracs(nd);
     HIPCHK(dalloc(&P->q, qh.size() * sizeof(double)));
     HIPCHK(hipMemcpyAsync(P->q, qh.data(), qh.size() * sizeof(double), hipMemcpyHostToDevice, s));
     HIPCHK(dalloc(&P->shifts_f, (size_t)std::max(nd, 1) * o.n_reps * sizeof(double)));
@@ -310,7 +324,9 @@ int plan_build(dbg_plan* P, const double* Sigma, int64_t lda, const double* a, c
     int dev = 0, ncu = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    P->tile = (o.kind == DBG_KIND_DENSE) ? dense_valu_tile() : sparse_tile();
+    P->use_mfma = (o.kind == DBG_KIND_DENSE) && n >= kMfmaMinDim;
+    P->tile = (o.kind != DBG_KIND_DENSE) ? sparse_tile()
+                                          : (P->use_mfma ? dense_mfma_tile(P->mixed) : dense_valu_tile());
     P->tiles_per_rep = (o.m + P->tile - 1) / P->tile;
     P->n_tiles = P->tiles_per_rep * o.n_reps;
     P->grid = (int)std::min<int64_t>(P->n_tiles, (int64_t)ncu * 2);
@@ -326,6 +342,8 @@ int plan_build(dbg_plan* P, const double* Sigma, int64_t lda, const double* a, c
     HIPCHK(hipStreamSynchronize(s));
     return DBG_OK;
 }
+
+int64_t n_of(const dbg_plan* P) { return P->n; }
 
 PointArgs point_args(const dbg_plan* P, int64_t block)
 {
@@ -360,21 +378,21 @@ int run_loop(dbg_plan* P, int64_t block, int64_t n_pts0)
         A.n_tiles = P->n_tiles;
         A.Uk = P->Uk;
         A.ldu = P->ldu;
+        A.Ucol = P->mixed ? (const void*)P->ucol32 : (const void*)P->F.M;
+        A.ldc = n_of(P);
         A.ak = P->ak;
         A.bk = P->bk;
         A.invk = P->invk;
-        A.c1 = P->first_h[0];
-        A.dc1 = P->first_h[1];
-        A.ea1 = (float)P->first_h[2];
-        A.eb1 = (float)P->first_h[3];
-        A.d1 = (float)P->first_h[4];
-        A.mode1 = (int)P->first_h[5];
+        for (int q = 0; q < 6; ++q) A.first[q] = P->first_h[q];
         A.pts = point_args(P, block);
         A.Y = P->Y;
         A.y_slot = P->y_slot;
         A.part_m = P->part_m;
         A.part_e = P->part_e;
-        HIPCHK(launch_dense_valu(A, P->mixed, gen, o.phi_form, P->grid, P->stream));
+        if (P->use_mfma)
+            HIPCHK(launch_dense_mfma(A, P->mixed, gen, o.phi_form, P->grid, P->stream));
+        else
+            HIPCHK(launch_dense_valu(A, P->mixed, gen, o.phi_form, P->grid, P->stream));
     } else {
         SparseArgs A{};
         A.n = P->n;
@@ -388,13 +406,9 @@ int run_loop(dbg_plan* P, int64_t block, int64_t n_pts0)
         A.ak = P->ak;
         A.bk = P->bk;
         A.invk = P->invk;
-        A.c1 = P->first_h[0];
-        A.dc1 = P->first_h[1];
-        A.ea1 = (float)P->first_h[2];
-        A.eb1 = (float)P->first_h[3];
-        A.d1 = (float)P->first_h[4];
-        A.mode1 = (int)P->first_h[5];
+        for (int q = 0; q < 6; ++q) A.first[q] = P->first_h[q];
         A.tent = (o.kind == DBG_KIND_SPARSEQMC) ? 0 : 1;
+        A.form = (o.phi_form == DBG_PHI_TAIL) ? kPhiTail : kPhiErf;
         A.pts = point_args(P, block);
         A.Y = P->Y;
         A.y_slot = P->y_slot;
@@ -482,7 +496,10 @@ int dbg_plan_create(dbg_plan** plan, const dbg_opts* opts, int32_t n, const doub
     if (o.generator != DBG_GEN_RICHTMYER && o.generator != DBG_GEN_SOBOL)
         return fail(DBG_ERR_ARG, "unknown generator");
     if (o.kind < DBG_KIND_DENSE || o.kind > DBG_KIND_SPARSEQMC) return fail(DBG_ERR_ARG, "unknown kind");
-    if (o.phi_form != DBG_PHI_ERF && o.phi_form != DBG_PHI_ERFC) return fail(DBG_ERR_ARG, "unknown phi_form");
+    if (o.phi_form != DBG_PHI_ERF && o.phi_form != DBG_PHI_ERFC && o.phi_form != DBG_PHI_TAIL)
+        return fail(DBG_ERR_ARG, "unknown phi_form");
+    if (o.kind != DBG_KIND_DENSE && o.phi_form == DBG_PHI_ERFC)
+        return fail(DBG_ERR_ARG, "the erfc form exists for the dense loop only");
     if (o.n_bits < 1 || o.n_bits > 64) return fail(DBG_ERR_ARG, "n_bits out of range");
     dbg_plan* P = new dbg_plan();
     P->o = o;
@@ -661,6 +678,285 @@ int dbg_plan_set_state(dbg_plan* P, int64_t state)
 {
     if (!P || state < 0) return fail(DBG_ERR_ARG, "invalid state");
     P->state = state;
+    return DBG_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batched problems of uneven dimension
+// ===========================================================================
+struct dbg_batch {
+    dbg_opts o{};
+    int K = 0;
+    bool mixed = false;
+    size_t tsz = 8;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    BatchArgs B{};
+    std::vector<void*> owned;
+    void* Uk = nullptr;
+    void* Ucol = nullptr;
+    void* Y = nullptr;
+    double* part_m = nullptr;
+    int* part_e = nullptr;
+    int2* sample_items = nullptr;
+    int64_t n_sample = 0;
+    int grid = 0;
+    int W = 16;
+    float tune_ms[3] = {0, 0, 0};
+
+    void release()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (void* p : owned) (void)hipFree(p);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    template <typename P>
+    int alloc(P** p, size_t bytes)
+    {
+        HIPCHK(dalloc(p, bytes));
+        owned.push_back((void*)*p);
+        return DBG_OK;
+    }
+};
+
+namespace {
+
+int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const double* a,
+                const double* b, const double* shifts)
+{
+    const dbg_opts& o = Bt->o;
+    const int K = Bt->K, R = o.n_reps;
+    const size_t T = Bt->tsz;
+    hipStream_t s = Bt->stream;
+    BatchArgs& B = Bt->B;
+    std::vector<BatchProblem> probs(K);
+    int64_t sig = 0, vec = 0, uk = 0, sh = 0, part = 0;
+    int dmax = 1;
+    const int tile = batch_tile();
+    const int64_t tpr = (o.m + tile - 1) / tile;
+    for (int k = 0; k < K; ++k) {
+        const int d = dims[k];
+        if (d < 1 || d > kBatchMaxDim)
+            return fail(DBG_ERR_ARG, "batched dimensions must be in [1, " + std::to_string(kBatchMaxDim) + "]");
+        if (d > o.chol_block_size2 || d > o.chol_block_size3 || d > o.chol_block_size4)
+            return fail(DBG_ERR_ARG, "batched problems must fit one pivot window (d <= chol_block_size2)");
+        BatchProblem& P = probs[k];
+        P.d = d;
+        P.tiles_per_rep = (int32_t)tpr;
+        P.ldu = d + 32;
+        P.pad = 0;
+        P.sig_off = sig;
+        P.ucol_off = sig;
+        P.vec_off = vec;
+        P.uk_off = uk;
+        P.shift_off = sh;
+        P.part_off = part;
+        sig += (int64_t)d * d;
+        vec += d;
+        uk += (int64_t)std::max(d - 1, 0) * P.ldu;
+        sh += (int64_t)(d - 1) * R;
+        part += tpr * R;
+        dmax = std::max(dmax, d);
+    }
+    // work items, largest dimension first (stable in the input order)
+    std::vector<int> order(K);
+    for (int k = 0; k < K; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return dims[x] > dims[y]; });
+    std::vector<int2> items;
+    items.reserve((size_t)K * tpr * R);
+    for (int k : order)
+        for (int64_t lt = 0; lt < tpr * R; ++lt) items.push_back(make_int2(k, (int)lt));
+    std::vector<int2> sample;
+    for (size_t i = 0; i < items.size(); i += 16) sample.push_back(items[i]);
+
+    BatchProblem* dprobs;
+    int rc;
+#define BALLOC(p, bytes)                       \
+    do {                                       \
+        rc = Bt->alloc(&(p), (bytes));         \
+        if (rc) return rc;                     \
+    } while (0)
+    BALLOC(dprobs, sizeof(BatchProblem) * K);
+    double *sig_in, *a_in, *b_in, *sh_d, *q_d;
+    BALLOC(sig_in, sig * sizeof(double));
+    BALLOC(a_in, vec * sizeof(double));
+    BALLOC(b_in, vec * sizeof(double));
+    BALLOC(sh_d, std::max<int64_t>(sh, 1) * sizeof(double));
+    BALLOC(B.M, sig * sizeof(double));
+    BALLOC(B.a, vec * sizeof(double));
+    BALLOC(B.b, vec * sizeof(double));
+    BALLOC(B.scale, vec * sizeof(double));
+    BALLOC(B.svec, vec * sizeof(double));
+    BALLOC(B.perm, vec * sizeof(int32_t));
+    BALLOC(B.rank, K * sizeof(int32_t));
+    BALLOC(B.ak, vec * T);
+    BALLOC(B.bk, vec * T);
+    BALLOC(B.invk, vec * T);
+    BALLOC(B.first, (size_t)8 * K * sizeof(double));
+    BALLOC(Bt->Uk, std::max<int64_t>(uk, 1) * T);
+    BALLOC(Bt->Ucol, sig * T);
+    int2* items_d;
+    BALLOC(items_d, items.size() * sizeof(int2));
+    BALLOC(Bt->sample_items, std::max<size_t>(sample.size(), 1) * sizeof(int2));
+    BALLOC(B.counter, sizeof(unsigned long long));
+    const std::vector<double> qh = sqrt_prime_fracs(dmax - 1);
+    BALLOC(q_d, qh.size() * sizeof(double));
+    int dev = 0, ncu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    Bt->grid = ncu * 2;
+    B.y_slot = (int64_t)std::max(dmax - 1, 1) * tile;
+    BALLOC(Bt->Y, (size_t)Bt->grid * B.y_slot * T);
+    BALLOC(Bt->part_m, std::max<int64_t>(part, 1) * sizeof(double));
+    BALLOC(Bt->part_e, std::max<int64_t>(part, 1) * sizeof(int));
+    BALLOC(B.rep_m, (size_t)K * R * sizeof(double));
+    BALLOC(B.rep_e, (size_t)K * R * sizeof(int));
+    BALLOC(B.out, (size_t)8 * K * sizeof(double));
+#undef BALLOC
+    HIPCHK(hipMemcpyAsync(dprobs, probs.data(), sizeof(BatchProblem) * K, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(sig_in, sigmas, sig * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(a_in, a, vec * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b_in, b, vec * sizeof(double), hipMemcpyHostToDevice, s));
+    if (sh > 0) HIPCHK(hipMemcpyAsync(sh_d, shifts, sh * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(items_d, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+    if (!sample.empty())
+        HIPCHK(hipMemcpyAsync(Bt->sample_items, sample.data(), sample.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(q_d, qh.data(), qh.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    B.K = K;
+    B.n_reps = R;
+    B.m = o.m;
+    B.bs1 = o.chol_block_size;
+    B.probs = dprobs;
+    B.sig_in = sig_in;
+    B.a_in = a_in;
+    B.b_in = b_in;
+    B.items = items_d;
+    B.n_items = (int64_t)items.size();
+    B.q = q_d;
+    B.shifts = sh_d;
+    Bt->n_sample = (int64_t)sample.size();
+
+    // ---- autotune the dimension-block width on a sample of the work items
+    HIPCHK(batch_factor(B, Bt->mixed, Bt->Uk, Bt->Ucol, s));
+    const int widths[3] = {8, 16, 32};
+    float best = 1e30f;
+    BatchArgs Bs = B;
+    Bs.items = Bt->sample_items;
+    Bs.n_items = Bt->n_sample;
+    for (int w = 0; w < 3; ++w) {
+        float msum = 0.f;
+        for (int rep = 0; rep < 2; ++rep) {  // first pass warms up
+            HIPCHK(fill_u64(B.counter, 0ull, s));
+            HIPCHK(hipEventRecord(Bt->ev0, s));
+            HIPCHK(batch_integrate(Bs, Bt->mixed, widths[w], Bt->grid, Bt->Uk, Bt->Ucol, Bt->Y, Bt->part_m,
+                                   Bt->part_e, s));
+            HIPCHK(hipEventRecord(Bt->ev1, s));
+            HIPCHK(hipEventSynchronize(Bt->ev1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, Bt->ev0, Bt->ev1));
+            if (rep == 1) msum = ms;
+        }
+        Bt->tune_ms[w] = msum;
+        if (msum < best) {
+            best = msum;
+            Bt->W = widths[w];
+        }
+    }
+    // ---- the whole job as one graph: factor, queue reset, integration, reductions
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t e = batch_factor(B, Bt->mixed, Bt->Uk, Bt->Ucol, s);
+    if (e == hipSuccess) e = fill_u64(B.counter, 0ull, s);
+    if (e == hipSuccess)
+        e = batch_integrate(B, Bt->mixed, Bt->W, Bt->grid, Bt->Uk, Bt->Ucol, Bt->Y, Bt->part_m, Bt->part_e, s);
+    if (e == hipSuccess) e = batch_reduce(B, Bt->part_m, Bt->part_e, s);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(s, &g);
+    Bt->graph = g;
+    HIPCHK(e);
+    HIPCHK(e2);
+    HIPCHK(hipGraphInstantiate(&Bt->exec, Bt->graph, nullptr, nullptr, 0));
+    HIPCHK(hipStreamSynchronize(s));
+    return DBG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dbg_batch_create(dbg_batch** out, const dbg_opts* opts, int32_t K, const int32_t* dims,
+                     const double* sigmas, const double* a, const double* b, const double* shifts)
+{
+    g_err.clear();
+    if (!out || !opts || !dims || !sigmas || !a || !b || K < 1) return fail(DBG_ERR_ARG, "invalid argument");
+    *out = nullptr;
+    if (opts->m < 1 || opts->n_reps < 1) return fail(DBG_ERR_ARG, "m and n_reps must be positive");
+    if (opts->generator != DBG_GEN_RICHTMYER || opts->kind != DBG_KIND_DENSE || opts->phi_form != DBG_PHI_ERF)
+        return fail(DBG_ERR_ARG, "batched problems use the dense Richtmyer erf-form path");
+    dbg_batch* Bt = new dbg_batch();
+    Bt->o = *opts;
+    Bt->K = K;
+    Bt->mixed = opts->precision == DBG_PREC_MIXED;
+    Bt->tsz = Bt->mixed ? sizeof(float) : sizeof(double);
+    int rc = DBG_OK;
+    hipError_t e = hipStreamCreate(&Bt->stream);
+    if (e == hipSuccess) e = hipEventCreate(&Bt->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&Bt->ev1);
+    if (e != hipSuccess) rc = fail(DBG_ERR_DEVICE, std::string("stream: ") + hipGetErrorString(e));
+    if (!rc) rc = batch_build(Bt, dims, sigmas, a, b, shifts);
+    if (rc) {
+        std::string msg = g_err;
+        Bt->release();
+        delete Bt;
+        g_err = msg;
+        return rc;
+    }
+    *out = Bt;
+    return DBG_OK;
+}
+
+int dbg_batch_run(dbg_batch* Bt, double* results)
+{
+    g_err.clear();
+    if (!Bt) return fail(DBG_ERR_ARG, "null batch");
+    HIPCHK(hipEventRecord(Bt->ev0, Bt->stream));
+    HIPCHK(hipGraphLaunch(Bt->exec, Bt->stream));
+    HIPCHK(hipEventRecord(Bt->ev1, Bt->stream));
+    if (results)
+        HIPCHK(hipMemcpyAsync(results, Bt->B.out, (size_t)8 * Bt->K * sizeof(double), hipMemcpyDeviceToHost,
+                              Bt->stream));
+    HIPCHK(hipStreamSynchronize(Bt->stream));
+    return DBG_OK;
+}
+
+int dbg_batch_last_ms(dbg_batch* Bt, float* ms)
+{
+    if (!Bt || !ms) return fail(DBG_ERR_ARG, "null argument");
+    HIPCHK(hipEventElapsedTime(ms, Bt->ev0, Bt->ev1));
+    return DBG_OK;
+}
+
+int dbg_batch_tuning(dbg_batch* Bt, int32_t* width, float* ms_w8_w16_w32)
+{
+    if (!Bt) return fail(DBG_ERR_ARG, "null batch");
+    if (width) *width = Bt->W;
+    if (ms_w8_w16_w32)
+        for (int i = 0; i < 3; ++i) ms_w8_w16_w32[i] = Bt->tune_ms[i];
+    return DBG_OK;
+}
+
+int dbg_batch_destroy(dbg_batch* Bt)
+{
+    if (!Bt) return DBG_OK;
+    (void)hipStreamSynchronize(Bt->stream);
+    Bt->release();
+    delete Bt;
     return DBG_OK;
 }
 

@@ -23,6 +23,8 @@
 // triangle is kept up to date; chol_finish zeroes the strict lower triangle.
 #include <float.h>
 
+#include <type_traits>
+
 #include "dbg_device.h"
 
 namespace dbg {
@@ -103,9 +105,14 @@ __device__ __forceinline__ bool better(double v, int i, double bv, int bi)
     return i < bi;
 }
 
-__global__ __launch_bounds__(kCholThreads) void chol_window_kernel(CholArgs C, int w0, int w1, int nb,
-                                                                   int pivot)
+// Factor the window [w0, w1) with one workgroup of NT threads (window size at
+// most WMAX).  On a rank failure *C.rank is set and the function returns.
+template <int NT, int WMAX>
+__device__ __forceinline__ void chol_window_body(const CholArgs& C, int w0, int w1, int nb, int pivot,
+                                                 double eps0)
 {
+    constexpr int kCholThreads = NT;
+    constexpr int kMaxWin = WMAX;
     __shared__ double s_dsum[kMaxWin], s_sv[kMaxWin], s_a[kMaxWin], s_b[kMaxWin];
     __shared__ int s_perm[kMaxWin];
     __shared__ double s_rv[kCholThreads / 64];
@@ -119,8 +126,6 @@ __global__ __launch_bounds__(kCholThreads) void chol_window_kernel(CholArgs C, i
     const int tid = threadIdx.x;
     const double c2 = 1.0 / sqrt(2.0);
     const double is2pi = 2.0 / sqrt(2.0 * 3.14159265358979323846);
-    if (*C.rank < n) return;  // an earlier window found the matrix rank deficient
-    const double eps0 = *C.eps0;
 
     for (int k = tid; k < w1 - w0; k += kCholThreads) {
         s_sv[k] = C.s_vec[w0 + k];
@@ -269,6 +274,14 @@ __global__ __launch_bounds__(kCholThreads) void chol_window_kernel(CholArgs C, i
         C.b[w0 + k] = s_b[k];
         C.perm[w0 + k] = s_perm[k];
     }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kCholThreads) void chol_window_kernel(CholArgs C, int w0, int w1, int nb,
+                                                                   int pivot)
+{
+    if (*C.rank < C.n) return;  // an earlier window found the matrix rank deficient
+    chol_window_body<kCholThreads, kMaxWin>(C, w0, w1, nb, pivot, *C.eps0);
 }
 
 hipError_t chol_window(const CholArgs& C, int w0, int w1, int nb, bool pivot, hipStream_t s)
@@ -370,4 +383,130 @@ hipError_t chol_finish(const CholArgs& C, hipStream_t s)
 }
 
 #undef MM
+}  // namespace dbg
+
+// ---------------------------------------------------------------------------
+// Batched constructor: one workgroup per problem of a batch of uneven
+// dimension (d <= kBatchMaxDim, a single pivot window).  It runs the whole
+// QMCData constructor for its problem on the device: standardize, pivoted
+// factor, truncate_matrixU!, and the layouts read by the integration kernel
+// (packed rows, column-major copy, bounds, 1/(U_jj sqrt 2), first interval).
+// ---------------------------------------------------------------------------
+namespace dbg {
+
+template <bool MIXED>
+__global__ __launch_bounds__(256) void batch_factor_kernel(BatchArgs B,
+                                                          typename std::conditional<MIXED, float, double>::type* __restrict__ Uk,
+                                                          typename std::conditional<MIXED, float, double>::type* __restrict__ Ucol)
+{
+    using T = typename std::conditional<MIXED, float, double>::type;
+    __shared__ double s_red[4];
+    const int p = blockIdx.x;
+    const BatchProblem P = B.probs[p];
+    const int d = P.d;
+    const int tid = threadIdx.x;
+    double* M = B.M + P.sig_off;
+    double* a = B.a + P.vec_off;
+    double* b = B.b + P.vec_off;
+    double* scale = B.scale + P.vec_off;
+    double* sv = B.svec + P.vec_off;
+    int32_t* perm = B.perm + P.vec_off;
+    const int64_t lda = d;
+    for (int64_t e = tid; e < (int64_t)d * d; e += 256) M[e] = B.sig_in[P.sig_off + e];
+    double mx = 0.0;
+    for (int j = tid; j < d; j += 256) {
+        a[j] = B.a_in[P.vec_off + j];
+        b[j] = B.b_in[P.vec_off + j];
+        sv[j] = 0.0;
+        perm[j] = j;
+        const double v = M[(size_t)j * (d + 1)];
+        const double av = fabs(v);
+        mx = (isnan(av) || av > mx) ? av : mx;
+        scale[j] = v > 0 ? sqrt(v) : 1.0;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(mx, off, 64);
+        mx = (isnan(o) || o > mx) ? o : mx;
+    }
+    if ((tid & 63) == 0) s_red[tid >> 6] = mx;
+    if (tid == 0) B.rank[p] = d;
+    __syncthreads();
+    double m4 = s_red[0];
+    for (int w = 1; w < 4; ++w) m4 = (isnan(s_red[w]) || s_red[w] > m4) ? s_red[w] : m4;
+    const double eps0 = DBL_EPSILON * m4;
+    // standardize (src/cholesky.jl:458-473), upper triangle
+    for (int64_t e = tid; e < (int64_t)d * d; e += 256) {
+        const int i = (int)(e % d), j = (int)(e / d);
+        if (i < j) M[e] = M[e] * ((1.0 / scale[j]) / scale[i]);
+        else if (i == j) M[e] = 1.0;
+    }
+    __syncthreads();
+    for (int j = tid; j < d; j += 256) {
+        const double inv = 1.0 / scale[j];
+        a[j] *= inv;
+        b[j] *= inv;
+    }
+    __syncthreads();
+    CholArgs C{M, lda, d, a, b, scale, perm, sv, nullptr, B.rank + p};
+    chol_window_body<256, kBatchMaxDim>(C, 0, d, max(1, min(B.bs1, 64)), 1, eps0);
+    const bool ok = B.rank[p] == d;
+    // zero the strict lower triangle, truncate_matrixU! (src/qmc.jl:133-142)
+    const double min_val = MIXED ? (double)(FLT_EPSILON / (float)(d * 9)) : DBL_EPSILON / (double)(d * 9);
+    for (int64_t e = tid; e < (int64_t)d * d; e += 256) {
+        const int i = (int)(e % d), j = (int)(e / d);
+        double v = (i > j) ? 0.0 : M[e];
+        if (fabs(v) < min_val) v = 0.0;
+        M[e] = v;
+    }
+    __syncthreads();
+    T* uk = Uk + P.uk_off;
+    T* uc = Ucol + P.ucol_off;
+    for (int64_t e = tid; e < (int64_t)d * d; e += 256) uc[e] = (T)M[e];
+    for (int64_t e = tid; e < (int64_t)max(d - 1, 0) * P.ldu; e += 256) {
+        const int l = (int)(e / P.ldu), j = (int)(e % P.ldu);
+        uk[e] = (T)((j > l && j < d) ? M[(size_t)l + (size_t)j * lda] : 0.0);
+    }
+    T* ak = (T*)B.ak + P.vec_off;
+    T* bk = (T*)B.bk + P.vec_off;
+    T* ik = (T*)B.invk + P.vec_off;
+    for (int j = tid; j < d; j += 256) {
+        const double ujj = M[(size_t)j * (d + 1)];
+        ak[j] = (T)a[j];
+        bk[j] = (T)b[j];
+        ik[j] = (T)(1.0 / (ujj * 1.4142135623730951));
+    }
+    if (tid == 0) {
+        double* f = B.first + 8 * (int64_t)p;
+        const double u11 = M[0];
+        const double arg_a = a[0] / (u11 * 1.4142135623730951);
+        const double arg_b = b[0] / (u11 * 1.4142135623730951);
+        f[0] = 0.5 * (1.0 + erf(arg_a));
+        f[1] = 0.5 * (erf(arg_b) - erf(arg_a));
+        const double ea = 0.5 * erfc(fabs(arg_a));
+        const double eb = 0.5 * erfc(fabs(arg_b));
+        if (arg_b <= 0.0) {
+            f[4] = eb - ea;
+            f[5] = 0;
+        } else if (arg_a >= 0.0) {
+            f[4] = ea - eb;
+            f[5] = 1;
+        } else {
+            f[4] = 0.5 * (erf(arg_b) - erf(arg_a));
+            f[5] = 2;
+        }
+        f[2] = ea;
+        f[3] = eb;
+        f[6] = ok ? 1.0 : 0.0;
+    }
+}
+
+hipError_t batch_factor(const BatchArgs& B, bool mixed, void* Uk, void* Ucol, hipStream_t s)
+{
+    if (mixed)
+        hipLaunchKernelGGL(batch_factor_kernel<true>, dim3(B.K), dim3(256), 0, s, B, (float*)Uk, (float*)Ucol);
+    else
+        hipLaunchKernelGGL(batch_factor_kernel<false>, dim3(B.K), dim3(256), 0, s, B, (double*)Uk, (double*)Ucol);
+    return hipGetLastError();
+}
+
 }  // namespace dbg

@@ -14,6 +14,26 @@ constexpr double kSqrt2 = 1.4142135623730951;       // Float64(sqrt(2.0))
 constexpr double kEps64 = 2.220446049250313e-16;    // eps(Float64)
 
 // ---------------------------------------------------------------------------
+// Special functions behind call boundaries.  Inlined into the dimension loop,
+// their polynomial constants are hoisted out of the loop into vector
+// registers (two per FP64 constant), which exhausts the register file; as
+// calls they keep the constants in scalar instructions local to each call.
+// ---------------------------------------------------------------------------
+__device__ __attribute__((noinline)) double dbg_erfinv(double x);
+__device__ __attribute__((noinline)) double dbg_erf(double x);
+__device__ __attribute__((noinline)) double dbg_erfc(double x);
+__device__ __attribute__((noinline)) double dbg_erfcinv(double x);
+__device__ __attribute__((noinline)) float dbg_erfcf(float x);
+__device__ __attribute__((noinline)) float dbg_erfcinvf(float x);
+
+__device__ __attribute__((noinline)) double dbg_erfinv(double x) { return erfinv(x); }
+__device__ __attribute__((noinline)) double dbg_erf(double x) { return erf(x); }
+__device__ __attribute__((noinline)) double dbg_erfc(double x) { return erfc(x); }
+__device__ __attribute__((noinline)) double dbg_erfcinv(double x) { return erfcinv(x); }
+__device__ __attribute__((noinline)) float dbg_erfcf(float x) { return erfcf(x); }
+__device__ __attribute__((noinline)) float dbg_erfcinvf(float x) { return erfcinvf(x); }
+
+// ---------------------------------------------------------------------------
 // Point generators (src/qmc_generators.jl)
 // ---------------------------------------------------------------------------
 
@@ -81,116 +101,260 @@ struct Weight {
 };
 
 // ---------------------------------------------------------------------------
-// FP64 per-dimension state: reference arithmetic of qmc_loop!.
+// Per-dimension arithmetic.  Three forms share one interface (struct Path):
+//
+//   reference FP64 (kPhiErf / kPhiErfc): the arithmetic of qmc_loop! in
+//     Float64 -- c = (1 + erf(x))/2 or erfc(-x)/2 with x clamped to [-9, 9],
+//     u = muladd(w, dc, c) clamped to [eps, 1 - eps], y = sqrt2 erfinv(2u - 1)
+//     (src/qmc.jl:385-429, :579-593, src/qmc_generators.jl:156-164);
+//   tail (kPhiTail; FP64, and the FP32 mixed path): no clamp; with za, zb the
+//     erf arguments of the interval, ea = erfc(|za|)/2 and eb = erfc(|zb|)/2
+//     are the normal tail masses beyond |za|, |zb|:
+//        zb <= 0 (lower half):  Phi(a') = ea, dc = eb - ea
+//        za >= 0 (upper half):  1 - Phi(b') = eb, dc = ea - eb
+//        otherwise:             dc = 1 - ea - eb
+//     and the sample's tail mass t is formed from whichever tail it is nearer
+//     to (u_lo = ea + w dc or u_hi = eb + (1 - w) dc), y = -+sqrt2 erfcinv(2t).
+//     Nothing cancels in the tails and no interval is cut at 9.
+//   In FP32 the three masses carry a binary scale 2^-k: past |z| = 9 (where
+//   erfcf leaves the normal FP32 range) they are formed in FP64 and scaled
+//   into FP32; Phi^{-1} of masses below 1e-6 (or scaled) is taken in FP64.
 // ---------------------------------------------------------------------------
 struct StateF64 {
     double c, dc;
 };
 
-// rand_points! + Y = sqrt2 * erfinv(Y)   (src/qmc_generators.jl:156-164,
-// src/qmc.jl:385-387).  u is the shifted uniform of point_u.
-// sample_f64_w takes the already tent-mapped point w (the SparseQMC module
-// has no tent map: src/qmc_sparse.jl:202-207).
-__device__ __forceinline__ double sample_f64_w(const StateF64& st, double w)
-{
-    double u = fma(w, st.dc, st.c);
-    u = fmin(fmax(u, kEps64), 1.0 - kEps64);
-    const double Yv = fma(2.0, u, -1.0);
-    return __dmul_rn(kSqrt2, erfinv(Yv));
-}
-
-__device__ __forceinline__ double sample_f64(const StateF64& st, double u)
-{
-    return sample_f64_w(st, fabs(fma(u, 2.0, -1.0)));
-}
-
-// c, dc of the next dimension from s  (src/qmc.jl:405-429 erf form,
-// :579-593 erfc form).
-template <int FORM>
-__device__ __forceinline__ void phi_f64(StateF64& st, double s, double a, double b, double inv)
-{
-    double x = __dmul_rn(__dsub_rn(a, s), inv);
-    double xb = __dmul_rn(__dsub_rn(b, s), inv);
-    x = fmax(fmin(x, 9.0), -9.0);
-    xb = fmax(fmin(xb, 9.0), -9.0);
-    if constexpr (FORM == kPhiErf) {
-        const double e1 = erf(x), e2 = erf(xb);
-        st.c = __dmul_rn(0.5, __dadd_rn(1.0, e1));
-        st.dc = __dsub_rn(__dmul_rn(0.5, __dadd_rn(1.0, e2)), st.c);
-    } else {
-        st.c = __dmul_rn(0.5, erfc(-x));
-        st.dc = __dsub_rn(__dmul_rn(0.5, erfc(-xb)), st.c);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Mixed per-dimension state (FP32 special functions, complementary form).
-//
-// With za, zb the clamped erf arguments of the interval, ea = erfc(|za|)/2 and
-// eb = erfc(|zb|)/2 are the normal tail masses beyond |za|, |zb|.  Then
-//   zb <= 0 (interval in the lower half):  Phi(a') = ea, dc = eb - ea
-//   za >= 0 (upper half):                   1-Phi(b') = eb, dc = ea - eb
-//   otherwise (straddles 0):                dc = 1 - ea - eb
-// and the sample u = Phi(a') + w*dc is formed from whichever tail it is
-// nearer to, so Phi^{-1} never sees a value close to 1 and the difference
-// never cancels in the tails.
-// ---------------------------------------------------------------------------
-struct StateMixed {
-    float ea, eb, dc;
-    int mode;  // 0 lower, 1 upper, 2 straddle
+template <typename T>
+struct StateTail {
+    T ea, eb, dc;
+    int mode;  // 0 lower half, 1 upper half, 2 straddles 0
+    int k;     // binary scale of ea, eb, dc (FP32 only)
 };
 
-__device__ __forceinline__ void mixed_from_z(StateMixed& st, float za, float zb)
+// first-dimension values computed in FP64: {c1, dc1, ea1, eb1, d1, mode1}
+__device__ __forceinline__ void tail_set64(double ea, double eb, double dc, int mode, StateTail<double>& st)
 {
-    st.ea = 0.5f * erfcf(fabsf(za));
-    st.eb = 0.5f * erfcf(fabsf(zb));
-    if (zb <= 0.0f) {
-        st.mode = 0;
-        st.dc = st.eb - st.ea;
-    } else if (za >= 0.0f) {
-        st.mode = 1;
-        st.dc = st.ea - st.eb;
+    st.ea = ea;
+    st.eb = eb;
+    st.dc = dc;
+    st.mode = mode;
+    st.k = 0;
+}
+
+__device__ __forceinline__ void tail_set64(double ea, double eb, double dc, int mode, StateTail<float>& st)
+{
+    const double mx = fmax(fmax(ea, eb), fabs(dc));
+    int k = 0;
+    if (mx > 0.0 && mx < 1e-30) {
+        int e = 0;
+        (void)frexp(mx, &e);
+        k = -e;  // mx * 2^k in [0.5, 1)
+    }
+    st.ea = (float)ldexp(ea, k);
+    st.eb = (float)ldexp(eb, k);
+    st.dc = (float)ldexp(dc, k);
+    st.mode = mode;
+    st.k = k;
+}
+
+__device__ __forceinline__ void tail_masses64(double za, double zb, double& ea, double& eb, double& dc,
+                                              int& mode)
+{
+    ea = 0.5 * dbg_erfc(fabs(za));
+    eb = 0.5 * dbg_erfc(fabs(zb));
+    if (zb <= 0.0) {
+        mode = 0;
+        dc = eb - ea;
+    } else if (za >= 0.0) {
+        mode = 1;
+        dc = ea - eb;
     } else {
-        st.mode = 2;
-        st.dc = (1.0f - st.ea) - st.eb;
+        mode = 2;
+        dc = (1.0 - ea) - eb;
     }
 }
 
-// Phi^{-1} of a tail mass t <= 1/2 (lower tail), FP32 with an FP64 fallback
-// for t < 1e-6.
-__device__ __forceinline__ float ndtri_lower(float t)
+__device__ __forceinline__ void tail_from_z(StateTail<double>& st, double za, double zb)
 {
-    if (t < 1e-6f) return (float)(-kSqrt2 * erfcinv(2.0 * (double)t));
-    return -1.41421356f * erfcinvf(2.0f * t);
+    double ea, eb, dc;
+    int mode;
+    tail_masses64(za, zb, ea, eb, dc, mode);
+    tail_set64(ea, eb, dc, mode, st);
 }
 
-// The tent-mapped point w in FP64, its complement, the clamp of the reference
-// (u in [eps64, 1 - eps64]) and Phi^{-1}.
-__device__ __forceinline__ float sample_mixed_w(const StateMixed& st, double w)
+__device__ __forceinline__ void tail_from_z(StateTail<float>& st, float za, float zb)
 {
-    const float wf = (float)w;
-    const float wcf = (float)(1.0 - w);
-    const float u_lo = fmaf(wf, st.dc, st.ea);   // Phi(a') + w dc          (modes 0, 2)
-    const float u_hi = fmaf(wcf, st.dc, st.eb);  // 1 - (Phi(a') + w dc)    (modes 1, 2)
+    if (fabsf(za) <= 9.0f && fabsf(zb) <= 9.0f) {
+        st.k = 0;
+        st.ea = 0.5f * dbg_erfcf(fabsf(za));
+        st.eb = 0.5f * dbg_erfcf(fabsf(zb));
+        if (zb <= 0.0f) {
+            st.mode = 0;
+            st.dc = st.eb - st.ea;
+        } else if (za >= 0.0f) {
+            st.mode = 1;
+            st.dc = st.ea - st.eb;
+        } else {
+            st.mode = 2;
+            st.dc = (1.0f - st.ea) - st.eb;
+        }
+    } else {
+        double ea, eb, dc;
+        int mode;
+        tail_masses64((double)za, (double)zb, ea, eb, dc, mode);
+        tail_set64(ea, eb, dc, mode, st);
+    }
+}
+
+__device__ __forceinline__ double tail_sample_w(const StateTail<double>& st, double w)
+{
+    const double u_lo = fma(w, st.dc, st.ea);
+    const double u_hi = fma(1.0 - w, st.dc, st.eb);
     const bool upper = (st.mode == 1) || (st.mode == 2 && u_hi < u_lo);
-    float t = upper ? u_hi : u_lo;
-    t = fmaxf(t, (float)kEps64);
-    const float y = ndtri_lower(t);
+    const double t = fmax(upper ? u_hi : u_lo, 1e-300);
+    const double y = -kSqrt2 * dbg_erfcinv(2.0 * t);
     return upper ? -y : y;
 }
 
-__device__ __forceinline__ float sample_mixed(const StateMixed& st, double u)
+__device__ __forceinline__ float tail_sample_w(const StateTail<float>& st, double w)
 {
-    return sample_mixed_w(st, fabs(fma(u, 2.0, -1.0)));
+    const float u_lo = fmaf((float)w, st.dc, st.ea);
+    const float u_hi = fmaf((float)(1.0 - w), st.dc, st.eb);
+    const bool upper = (st.mode == 1) || (st.mode == 2 && u_hi < u_lo);
+    const float t = upper ? u_hi : u_lo;
+    float y;
+    if (st.k != 0 || t < 1e-6f) {
+        const double tt = fmax(ldexp((double)t, -st.k), 1e-300);
+        y = (float)(-kSqrt2 * dbg_erfcinv(2.0 * tt));
+    } else {
+        y = -1.41421356f * dbg_erfcinvf(2.0f * t);
+    }
+    return upper ? -y : y;
 }
 
-__device__ __forceinline__ void phi_mixed(StateMixed& st, float s, float a, float b, float inv)
+template <bool MIXED, int FORM>
+struct Path;
+
+// reference FP64 arithmetic (erf or erfc form)
+template <int FORM>
+struct Path<false, FORM> {
+    using T = double;
+    using State = StateF64;
+    static __device__ __forceinline__ void init(State& st, Weight<T>& w, const double* f)
+    {
+        st.c = f[0];
+        st.dc = f[1];
+        w.init(f[1]);
+    }
+    // sample from the already tent-mapped point w (the SparseQMC module has
+    // no tent map: src/qmc_sparse.jl:202-207)
+    static __device__ __forceinline__ T sample_w(const State& st, double w)
+    {
+        double u = fma(w, st.dc, st.c);
+        u = fmin(fmax(u, kEps64), 1.0 - kEps64);
+        const double Yv = fma(2.0, u, -1.0);
+        return __dmul_rn(kSqrt2, dbg_erfinv(Yv));
+    }
+    static __device__ __forceinline__ void phi(State& st, T s, T a, T b, T inv)
+    {
+        double x = __dmul_rn(__dsub_rn(a, s), inv);
+        double xb = __dmul_rn(__dsub_rn(b, s), inv);
+        x = fmax(fmin(x, 9.0), -9.0);
+        xb = fmax(fmin(xb, 9.0), -9.0);
+        if constexpr (FORM == kPhiErf) {
+            const double e1 = dbg_erf(x), e2 = dbg_erf(xb);
+            st.c = __dmul_rn(0.5, __dadd_rn(1.0, e1));
+            st.dc = __dsub_rn(__dmul_rn(0.5, __dadd_rn(1.0, e2)), st.c);
+        } else {
+            st.c = __dmul_rn(0.5, dbg_erfc(-x));
+            st.dc = __dsub_rn(__dmul_rn(0.5, dbg_erfc(-xb)), st.c);
+        }
+    }
+    static __device__ __forceinline__ void weigh(Weight<T>& w, const State& st) { w.mul(st.dc); }
+};
+
+// tail form in FP64
+template <>
+struct Path<false, kPhiTail> {
+    using T = double;
+    using State = StateTail<double>;
+    static __device__ __forceinline__ void init(State& st, Weight<T>& w, const double* f)
+    {
+        tail_set64(f[2], f[3], f[4], (int)f[5], st);
+        w.init(st.dc);
+    }
+    static __device__ __forceinline__ T sample_w(const State& st, double w) { return tail_sample_w(st, w); }
+    static __device__ __forceinline__ void phi(State& st, T s, T a, T b, T inv)
+    {
+        tail_from_z(st, (a - s) * inv, (b - s) * inv);
+    }
+    static __device__ __forceinline__ void weigh(Weight<T>& w, const State& st) { w.mul(st.dc); }
+};
+
+// the FP32 mixed path (tail form, any FORM)
+template <int FORM>
+struct Path<true, FORM> {
+    using T = float;
+    using State = StateTail<float>;
+    static __device__ __forceinline__ void init(State& st, Weight<T>& w, const double* f)
+    {
+        tail_set64(f[2], f[3], f[4], (int)f[5], st);
+        w.init(st.dc);
+        w.pe -= st.k;
+    }
+    static __device__ __forceinline__ T sample_w(const State& st, double w) { return tail_sample_w(st, w); }
+    static __device__ __forceinline__ void phi(State& st, T s, T a, T b, T inv)
+    {
+        tail_from_z(st, (a - s) * inv, (b - s) * inv);
+    }
+    static __device__ __forceinline__ void weigh(Weight<T>& w, const State& st)
+    {
+        w.mul(st.dc);
+        w.pe -= st.k;
+    }
+};
+
+template <bool MIXED, int FORM>
+__device__ __forceinline__ typename Path<MIXED, FORM>::T path_sample(const typename Path<MIXED, FORM>::State& st,
+                                                                     double u)
 {
-    float x = (a - s) * inv;
-    float xb = (b - s) * inv;
-    x = fmaxf(fminf(x, 9.0f), -9.0f);
-    xb = fmaxf(fminf(xb, 9.0f), -9.0f);
-    mixed_from_z(st, x, xb);
+    return Path<MIXED, FORM>::sample_w(st, fabs(fma(u, 2.0, -1.0)));
+}
+
+// ---------------------------------------------------------------------------
+// Walk of one block of up to 16 sampled dimensions [gb, gb + cnt) for the
+// lane's sample (lane t of a tile of TS samples).
+//
+// On entry s_x[t * SXS + c] holds the conditional mean of target gb + 1 + c
+// accumulated from every dimension before gb.  For each dimension g the lane
+// draws the point, forms y_g, stores it (LDS row s_y[g - gb], global
+// workspace row Ys[g]), completes the mean of target g + 1 with the block's
+// own y values in ascending order (the in-block dot of src/qmc.jl:390-403),
+// and updates its interval and weight (:405-429).  Ucol is the factor in
+// column-major order (Ucol[j * ldc + l] = U[l, j]), read through the scalar
+// cache since every lane needs the same column.
+// ---------------------------------------------------------------------------
+template <bool MIXED, int GEN, int FORM, int TS, int SXS, typename T, typename State>
+__device__ __forceinline__ void walk_block(const PointArgs& pts, int gb, int cnt, int k, int64_t j,
+                                           int t, State& st, Weight<T>& w, const T* __restrict__ s_x,
+                                           T* __restrict__ s_y, T* __restrict__ Ys,
+                                           const T* __restrict__ Ucol, int64_t ldc,
+                                           const T* __restrict__ ak, const T* __restrict__ bk,
+                                           const T* __restrict__ invk)
+{
+    using PT = Path<MIXED, FORM>;
+    for (int cc = 0; cc < cnt; ++cc) {
+        const int g = gb + cc;
+        const double u = point_u<GEN>(pts, g, k, j);
+        const T y = path_sample<MIXED, FORM>(st, u);
+        s_y[cc * TS + t] = y;
+        Ys[(size_t)g * TS + t] = y;
+        T acc = s_x[t * SXS + cc];
+        const T* __restrict__ ucol = Ucol + (size_t)(g + 1) * ldc + gb;
+        for (int l = 0; l <= cc; ++l) acc = fma(s_y[l * TS + t], ucol[l], acc);
+        PT::phi(st, acc, ak[g + 1], bk[g + 1], invk[g + 1]);
+        PT::weigh(w, st);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -7,7 +7,7 @@
 namespace dbg {
 
 enum { kGenRichtmyer = 0, kGenSobol = 1 };
-enum { kPhiErf = 0, kPhiErfc = 1 };
+enum { kPhiErf = 0, kPhiErfc = 1, kPhiTail = 2 };
 
 // Where the points of one round (one block of m points per shift) come from.
 struct PointArgs {
@@ -31,12 +31,12 @@ struct DenseArgs {
     int64_t n_tiles;           // tiles_per_rep * n_reps
     const void* Uk;            // packed strict-upper rows (T), Uk[l*ldu + j] = U[l, j] for j > l else 0
     int64_t ldu;
+    const void* Ucol;          // the factor column-major (T), Ucol[j*ldc + l] = U[l, j]
+    int64_t ldc;
     const void* ak;            // per target j: a_j (T)
     const void* bk;            // per target j: b_j (T)
     const void* invk;          // per target j: 1 / (U_jj * sqrt 2) (T)
-    double c1, dc1;            // first dimension (FP64 path)
-    float ea1, eb1, d1;        // first dimension (mixed path)
-    int mode1;
+    double first[6];           // first dimension: c1, dc1, ea1, eb1, d1, mode1 (FP64)
     PointArgs pts;
     void* Y;                   // workspace: one slot of nd x tile samples per resident block
     int64_t y_slot;            // elements per slot
@@ -57,9 +57,8 @@ struct SparseArgs {
     const void* ak;
     const void* bk;
     const void* invk;
-    double c1, dc1;
-    float ea1, eb1, d1;
-    int mode1;
+    double first[6];
+    int form;                  // kPhiErf or kPhiTail
     int tent;                  // 1: src/qmc.jl form, 0: SparseQMC (no tent)
     PointArgs pts;
     void* Y;
@@ -72,9 +71,68 @@ hipError_t launch_dense_valu(const DenseArgs& A, bool mixed, int gen, int form, 
                              hipStream_t s);
 int dense_valu_tile();
 int dense_valu_width();
+hipError_t launch_dense_mfma(const DenseArgs& A, bool mixed, int gen, int form, int grid,
+                             hipStream_t s);
+int dense_mfma_tile(bool mixed);
+int dense_mfma_panel(bool mixed);
+// dimension from which the dense path uses the matrix-core kernel
+constexpr int kMfmaMinDim = 96;
 
 hipError_t launch_sparse(const SparseArgs& A, bool mixed, int gen, int grid, hipStream_t s);
 int sparse_tile();
+
+// ---- batched problems of uneven dimension --------------------------------
+struct BatchProblem {
+    int32_t d;
+    int32_t tiles_per_rep;
+    int64_t sig_off;    // offset of Sigma_k / the factor (d x d doubles)
+    int64_t vec_off;    // offset of a, b, scale, perm, ak, bk, invk (d entries)
+    int64_t uk_off;     // offset of the packed rows (d - 1 rows of ldu, working precision)
+    int64_t ucol_off;   // offset of the column-major factor copy (d x d, working precision)
+    int64_t shift_off;  // offset of the shifts ((d - 1) x n_reps doubles)
+    int64_t part_off;   // offset of the problem's tile partials (n_reps x tiles_per_rep)
+    int32_t ldu;
+    int32_t pad;
+};
+
+struct BatchArgs {
+    int K;
+    int n_reps;
+    int64_t m;
+    int bs1;
+    const BatchProblem* probs;
+    const double* sig_in;
+    const double* a_in;
+    const double* b_in;
+    double* M;
+    double* a;
+    double* b;
+    double* scale;
+    double* svec;
+    int32_t* perm;
+    int32_t* rank;
+    void* ak;
+    void* bk;
+    void* invk;
+    double* first;                 // 8 per problem
+    const int2* items;             // (problem, rep * tiles_per_rep + tile), largest d first
+    int64_t n_items;
+    unsigned long long* counter;   // work-queue head
+    const double* q;
+    const double* shifts;
+    int64_t y_slot;
+    double* rep_m;                 // n_reps per problem
+    int* rep_e;
+    double* out;                   // 8 per problem: value, log, err3, rel, mean_m, err_m, E, rank
+};
+
+constexpr int kBatchMaxDim = 512;
+hipError_t batch_factor(const BatchArgs& B, bool mixed, void* Uk, void* Ucol, hipStream_t s);
+hipError_t batch_integrate(const BatchArgs& B, bool mixed, int W, int grid, const void* Uk,
+                           const void* Ucol, void* Y, double* part_m, int* part_e, hipStream_t s);
+hipError_t batch_reduce(const BatchArgs& B, const double* part_m, const int* part_e, hipStream_t s);
+hipError_t fill_u64(unsigned long long* p, unsigned long long v, hipStream_t s);
+int batch_tile();
 
 // Per-shift reduction of the tile partials and running-mean merge
 // (src/qmc.jl:434-442); then mean and 3*sd over shifts (:303-312, 814-815).
@@ -107,6 +165,7 @@ hipError_t pack_dense(const double* U, int64_t lda, int n, const double* a, cons
                       bool mixed, void* Uk, int64_t ldu, void* ak, void* bk, void* invk,
                       double* first /* c1, dc1, ea1, eb1, d1, mode1 */, int phi_form,
                       hipStream_t s);
+hipError_t convert_f32(const double* in, size_t count, float* out, hipStream_t s);
 hipError_t csc_count(const double* U, int64_t lda, int n, int64_t* colcount, hipStream_t s);
 hipError_t csc_fill(const double* U, int64_t lda, int n, const int64_t* colptr, bool mixed,
                     int32_t* rowval, void* nzval, hipStream_t s);

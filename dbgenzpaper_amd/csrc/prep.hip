@@ -111,6 +111,20 @@ hipError_t pack_dense(const double* U, int64_t lda, int n, const double* a, cons
     return hipGetLastError();
 }
 
+__global__ void convert_f32_kernel(const double* __restrict__ in, size_t count, float* out)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = (float)in[i];
+}
+
+hipError_t convert_f32(const double* in, size_t count, float* out, hipStream_t s)
+{
+    const size_t blocks = std::min<size_t>((count + 255) / 256, 65536);
+    hipLaunchKernelGGL(convert_f32_kernel, dim3((unsigned)std::max<size_t>(blocks, 1)), dim3(256), 0, s,
+                       in, count, out);
+    return hipGetLastError();
+}
+
 // --- CSC of triu(U, 1): rows ascending within each column ------------------
 __global__ void csc_count_kernel(const double* __restrict__ U, int64_t lda, int n, int64_t* cnt)
 {

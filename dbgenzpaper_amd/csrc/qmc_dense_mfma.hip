@@ -1,0 +1,224 @@
+// Dense integration kernel, matrix-core form (large d).
+//
+// The same computation as qmc_loop! (src/qmc.jl:315-444) with the point map
+// of src/qmc_generators.jl:137-204 fused in, organised around the MI355X
+// matrix cores:
+//
+//   * a workgroup (4 wavefronts) owns TS samples of one shift; its sampled
+//     y values live in a per-block workspace slot, layout [dimension][sample];
+//   * the sampled dimensions are swept in panels of PW targets.  For each
+//     panel the conditional means S (TS x PW) of all earlier dimensions are one
+//     dense contraction S = Y[:, 0:g0] * U[0:g0, panel] on v_mfma_*_16x16x4,
+//     with the accumulators held in registers (the outer mul! of :354);
+//   * the panel is then walked in sub-blocks of 16 dimensions: the
+//     sub-block's 16 accumulator columns go through LDS to a lane-per-sample
+//     layout, each lane walks its sample through the 16 dimensions in
+//     registers (point, Phi^{-1}, the in-block triangle of :390-403, Phi,
+//     weight), and the 16 new y columns update the panel's later targets with
+//     one more K = 16 matrix-core contraction (the incremental mul! of :374).
+//
+// Operands of the contraction are read straight from global memory: Y rows
+// are private to the workgroup (coalesced 64-byte segments), U rows are shared
+// by all workgroups and stay in L2 / MALL.
+#include <type_traits>
+
+#include "dbg_device.h"
+
+namespace dbg {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+// Result layout of the 16x16x4 instructions (checked on the device): lane l
+// holds column l % 16; element r of its accumulator is row 4 * (l / 16) + r
+// for f32 and row (l / 16) + 4 * r for f64.
+template <>
+struct Mfma<float> {
+    using V = floatx4;
+    static __device__ __forceinline__ int row(int lq, int r) { return 4 * lq + r; }
+    static __device__ __forceinline__ V mac(float a, float b, V c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct Mfma<double> {
+    using V = doublex4;
+    static __device__ __forceinline__ int row(int lq, int r) { return lq + 4 * r; }
+    static __device__ __forceinline__ V mac(double a, double b, V c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+
+constexpr int kSB = 16;      // walk sub-block (dimensions)
+constexpr int kSXS = kSB + 4;  // LDS row stride of the accumulator hand-off
+
+template <bool MIXED>
+struct MfmaCfg;
+template <>
+struct MfmaCfg<true> {   // FP32: 256 samples x 128 targets, 128 accumulators per lane
+    static constexpr int TS = 256, PW = 128;
+};
+template <>
+struct MfmaCfg<false> {  // FP64: 128 samples x 128 targets, 64 accumulators per lane
+    static constexpr int TS = 128, PW = 128;
+};
+
+int dense_mfma_tile(bool mixed) { return mixed ? MfmaCfg<true>::TS : MfmaCfg<false>::TS; }
+int dense_mfma_panel(bool mixed) { return mixed ? MfmaCfg<true>::PW : MfmaCfg<false>::PW; }
+
+template <bool MIXED, int GEN, int FORM, typename T = typename std::conditional<MIXED, float, double>::type>
+__global__ __launch_bounds__(256) void dense_mfma_kernel(DenseArgs A, const T* __restrict__ Uk,
+                                                         const T* __restrict__ Ucol,
+                                                         const T* __restrict__ ak,
+                                                         const T* __restrict__ bk,
+                                                         const T* __restrict__ invk,
+                                                         T* __restrict__ Yall,
+                                                         double* __restrict__ part_m,
+                                                         int* __restrict__ part_e)
+{
+    using V = typename Mfma<T>::V;
+    constexpr int TS = MfmaCfg<MIXED>::TS;
+    constexpr int PW = MfmaCfg<MIXED>::PW;
+    constexpr int WS = TS / 4;         // samples per wavefront
+    constexpr int NSB = WS / 16;       // 16-sample blocks per wavefront
+    constexpr int NTB = PW / 16;       // 16-target blocks per panel
+
+    __shared__ T s_x[TS * kSXS];       // accumulator hand-off, [sample][16]
+    __shared__ T s_y[kSB * TS];        // y of the current sub-block, [k][sample]
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wv = tid >> 6;
+    const int lr = lane & 15;          // row/column inside a 16x16 block
+    const int lq = lane >> 4;          // k (operands) or row quad (result)
+    const int nd = A.n - 1;
+    const int64_t ldu = A.ldu;
+    T* __restrict__ Ys = Yall + (size_t)blockIdx.x * (size_t)A.y_slot;
+    const bool walker = tid < TS;
+
+    for (int64_t tile = blockIdx.x; tile < A.n_tiles; tile += gridDim.x) {
+        const int k = (int)(tile / A.tiles_per_rep);
+        const int64_t jt = tile - (int64_t)k * A.tiles_per_rep;
+        const int64_t j = jt * TS + tid + 1;  // 1-based point index of the walker's sample
+        const bool valid = walker && j <= A.m;
+
+        typename Path<MIXED, FORM>::State st;
+        Weight<T> w;
+        Path<MIXED, FORM>::init(st, w, A.first);
+
+        for (int g0 = 0; g0 < nd; g0 += PW) {
+            // ---- S = Y[:, 0:g0] * U[0:g0, g0+1 : g0+PW+1]
+            V acc[NSB][NTB];
+#pragma unroll
+            for (int sb = 0; sb < NSB; ++sb)
+#pragma unroll
+                for (int tb = 0; tb < NTB; ++tb) acc[sb][tb] = V{0, 0, 0, 0};
+            {
+                const T* __restrict__ ya = Ys + (size_t)lq * TS + wv * WS + lr;
+                const T* __restrict__ ub = Uk + (size_t)lq * ldu + g0 + 1 + lr;
+                for (int k0 = 0; k0 < g0; k0 += 4) {
+                    T av[NSB], bv[NTB];
+#pragma unroll
+                    for (int sb = 0; sb < NSB; ++sb) av[sb] = ya[(size_t)k0 * TS + 16 * sb];
+#pragma unroll
+                    for (int tb = 0; tb < NTB; ++tb) bv[tb] = ub[(size_t)k0 * ldu + 16 * tb];
+#pragma unroll
+                    for (int sb = 0; sb < NSB; ++sb)
+#pragma unroll
+                        for (int tb = 0; tb < NTB; ++tb) acc[sb][tb] = Mfma<T>::mac(av[sb], bv[tb], acc[sb][tb]);
+                }
+            }
+            // ---- walk the panel in sub-blocks of 16 dimensions
+            for (int b = 0; b < NTB; ++b) {
+                const int gb = g0 + kSB * b;
+                if (gb >= nd) break;  // uniform
+                // accumulator columns of this sub-block -> LDS [sample][col]
+#pragma unroll
+                for (int tb = 0; tb < NTB; ++tb) {
+                    if (tb == b) {
+#pragma unroll
+                        for (int sb = 0; sb < NSB; ++sb)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r)
+                                s_x[(wv * WS + 16 * sb + Mfma<T>::row(lq, r)) * kSXS + lr] = acc[sb][tb][r];
+                    }
+                }
+                __syncthreads();
+                const int cnt = min(kSB, nd - gb);
+                if (walker)
+                    walk_block<MIXED, GEN, FORM, TS, kSXS>(A.pts, gb, cnt, k, j, tid, st, w, s_x, s_y,
+                                                             Ys, Ucol, A.ldc, ak, bk, invk);
+                __syncthreads();
+                // later targets of the panel += Y_sub * U[sub, later]
+                if (b + 1 < NTB && gb + kSB < nd) {
+#pragma unroll
+                    for (int ks = 0; ks < kSB / 4; ++ks) {
+                        T av[NSB];
+#pragma unroll
+                        for (int sb = 0; sb < NSB; ++sb)
+                            av[sb] = s_y[(4 * ks + lq) * TS + wv * WS + 16 * sb + lr];
+                        const T* __restrict__ ub = Uk + (size_t)(gb + 4 * ks + lq) * ldu + g0 + 1 + lr;
+#pragma unroll
+                        for (int tb = 0; tb < NTB; ++tb) {
+                            if (tb > b) {
+                                const T bv = ub[16 * tb];
+#pragma unroll
+                                for (int sb = 0; sb < NSB; ++sb) acc[sb][tb] = Mfma<T>::mac(av[sb], bv, acc[sb][tb]);
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        const double pm = valid ? (double)w.pm : 0.0;
+        double S;
+        int E;
+        block_scaled_sum<256>(pm, w.pe, S, E);
+        if (tid == 0) {
+            part_m[tile] = S;
+            part_e[tile] = E;
+        }
+    }
+}
+
+template <bool MIXED, int GEN, int FORM>
+static void launch_m(const DenseArgs& A, int grid, hipStream_t s)
+{
+    using T = typename std::conditional<MIXED, float, double>::type;
+    hipLaunchKernelGGL((dense_mfma_kernel<MIXED, GEN, FORM>), dim3(grid), dim3(256), 0, s, A,
+                       (const T*)A.Uk, (const T*)A.Ucol, (const T*)A.ak, (const T*)A.bk,
+                       (const T*)A.invk, (T*)A.Y, A.part_m, A.part_e);
+}
+
+hipError_t launch_dense_mfma(const DenseArgs& A, bool mixed, int gen, int form, int grid,
+                             hipStream_t s)
+{
+    if (mixed) {
+        if (gen == kGenRichtmyer)
+            launch_m<true, kGenRichtmyer, kPhiTail>(A, grid, s);
+        else
+            launch_m<true, kGenSobol, kPhiTail>(A, grid, s);
+    } else if (gen == kGenRichtmyer) {
+        if (form == kPhiErf)
+            launch_m<false, kGenRichtmyer, kPhiErf>(A, grid, s);
+        else if (form == kPhiErfc)
+            launch_m<false, kGenRichtmyer, kPhiErfc>(A, grid, s);
+        else
+            launch_m<false, kGenRichtmyer, kPhiTail>(A, grid, s);
+    } else {
+        if (form == kPhiErf)
+            launch_m<false, kGenSobol, kPhiErf>(A, grid, s);
+        else if (form == kPhiErfc)
+            launch_m<false, kGenSobol, kPhiErfc>(A, grid, s);
+        else
+            launch_m<false, kGenSobol, kPhiTail>(A, grid, s);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace dbg

@@ -17,11 +17,11 @@ namespace dbg {
 constexpr int kTileS = 256;
 int sparse_tile() { return kTileS; }
 
-template <bool MIXED, int GEN>
+template <bool MIXED, int GEN, int FORM>
 __global__ __launch_bounds__(kTileS) void sparse_kernel(SparseArgs A)
 {
     using T = typename std::conditional<MIXED, float, double>::type;
-    using State = typename std::conditional<MIXED, StateMixed, StateF64>::type;
+    using PT = Path<MIXED, FORM>;
     const int tid = threadIdx.x;
     const int nd = A.n - 1;
     const int64_t* __restrict__ colptr = A.colptr;
@@ -37,43 +37,20 @@ __global__ __launch_bounds__(kTileS) void sparse_kernel(SparseArgs A)
         const int64_t jt = tile - (int64_t)k * A.tiles_per_rep;
         const int64_t j = jt * kTileS + tid + 1;
         const bool valid = j <= A.m;
-        State st;
+        typename PT::State st;
         Weight<T> w;
-        if constexpr (MIXED) {
-            st.ea = A.ea1;
-            st.eb = A.eb1;
-            st.dc = A.d1;
-            st.mode = A.mode1;
-            w.init(A.d1);
-        } else {
-            st.c = A.c1;
-            st.dc = A.dc1;
-            w.init(A.dc1);
-        }
+        PT::init(st, w, A.first);
         for (int g = 0; g < nd; ++g) {
             const double u = point_u<GEN>(A.pts, g, k, j);
-            T y;
-            if (A.tent) {
-                if constexpr (MIXED)
-                    y = sample_mixed(st, u);
-                else
-                    y = sample_f64(st, u);
-            } else {
-                if constexpr (MIXED)
-                    y = sample_mixed_w(st, u);
-                else
-                    y = sample_f64_w(st, u);
-            }
+            // the SparseQMC module uses the shifted point without the tent map
+            const T y = A.tent ? path_sample<MIXED, FORM>(st, u) : PT::sample_w(st, u);
             Ys[(size_t)g * kTileS + tid] = y;
             T sv = T(0);
             const int64_t p1 = colptr[g + 2];
             for (int64_t p = colptr[g + 1]; p < p1; ++p)
                 sv = fma(Ys[(size_t)rowval[p] * kTileS + tid], nzval[p], sv);
-            if constexpr (MIXED)
-                phi_mixed(st, sv, ak[g + 1], bk[g + 1], invk[g + 1]);
-            else
-                phi_f64<kPhiErf>(st, sv, ak[g + 1], bk[g + 1], invk[g + 1]);
-            w.mul(st.dc);
+            PT::phi(st, sv, ak[g + 1], bk[g + 1], invk[g + 1]);
+            PT::weigh(w, st);
         }
         const double pm = valid ? (double)w.pm : 0.0;
         double S;
@@ -91,14 +68,19 @@ hipError_t launch_sparse(const SparseArgs& A, bool mixed, int gen, int grid, hip
     dim3 g(grid), b(kTileS);
     if (mixed) {
         if (gen == kGenRichtmyer)
-            hipLaunchKernelGGL((sparse_kernel<true, kGenRichtmyer>), g, b, 0, s, A);
+            hipLaunchKernelGGL((sparse_kernel<true, kGenRichtmyer, kPhiTail>), g, b, 0, s, A);
         else
-            hipLaunchKernelGGL((sparse_kernel<true, kGenSobol>), g, b, 0, s, A);
+            hipLaunchKernelGGL((sparse_kernel<true, kGenSobol, kPhiTail>), g, b, 0, s, A);
+    } else if (A.form == kPhiTail) {
+        if (gen == kGenRichtmyer)
+            hipLaunchKernelGGL((sparse_kernel<false, kGenRichtmyer, kPhiTail>), g, b, 0, s, A);
+        else
+            hipLaunchKernelGGL((sparse_kernel<false, kGenSobol, kPhiTail>), g, b, 0, s, A);
     } else {
         if (gen == kGenRichtmyer)
-            hipLaunchKernelGGL((sparse_kernel<false, kGenRichtmyer>), g, b, 0, s, A);
+            hipLaunchKernelGGL((sparse_kernel<false, kGenRichtmyer, kPhiErf>), g, b, 0, s, A);
         else
-            hipLaunchKernelGGL((sparse_kernel<false, kGenSobol>), g, b, 0, s, A);
+            hipLaunchKernelGGL((sparse_kernel<false, kGenSobol, kPhiErf>), g, b, 0, s, A);
     }
     return hipGetLastError();
 }

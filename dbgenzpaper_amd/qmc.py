@@ -141,7 +141,7 @@ class _QMCBase:
     _phi = L.PHI_ERF
 
     def __init__(self, C_mat, a, b, opts: QMCOpts | None, rng, qmc_type, *, shifts=None,
-                 sobol_skip=None, chol_sizes=None, _clamp=True):
+                 sobol_skip=None, chol_sizes=None, phi_form=None, _clamp=True):
         Cm = _as_matrix(C_mat)
         n = Cm.shape[0]
         if opts is None:
@@ -171,6 +171,8 @@ class _QMCBase:
         o.max_pts = opts.max_pts
         o.max_abs_err = opts.max_abs_err
         o.generator = L.GEN_SOBOL if qmc_type == "Sobol" else L.GEN_RICHTMYER
+        if phi_form is not None:
+            self._phi = {"erf": L.PHI_ERF, "erfc": L.PHI_ERFC, "tail": L.PHI_TAIL}[phi_form]
         o.phi_form = self._phi
         o.kind = self._kind
         o.sobol_skip = -1 if sobol_skip is None else int(sobol_skip)
@@ -314,7 +316,12 @@ def draw_shifts(rng, n: int, n_reps: int, qmc_type: str, T=np.float64) -> np.nda
 
 class QMCData(_QMCBase):
     """QMCData(C, a, b, opts=QMC_opts(T), rng=default_rng(), qmc_type="Richtmyer")
-    src/qmc.jl:149-202.  Pivoted Genz factor, dense loop, erf-form CDF."""
+    src/qmc.jl:149-202.  Pivoted Genz factor, dense loop, erf-form CDF.
+
+    Keyword ``phi_form="tail"`` (no reference counterpart) selects the tail
+    form for T = Float64: normal tail masses without the [-9, 9] clamp and
+    Phi^{-1} from the nearer tail, which keeps problems whose per-dimension
+    masses fall below 1e-37 finite.  T = Float32 always uses it."""
     _kind = L.KIND_DENSE
     _phi = L.PHI_ERF
 
@@ -331,7 +338,7 @@ class QMCData_Distributions(_QMCBase):
 
     def __init__(self, D: QMCData):
         super().__init__(D._Sigma, D._a0, D._b0, D.qmc_opts, None, D.qmc_type, shifts=D.shifts,
-                         sobol_skip=D._sobol_skip, _clamp=False)
+                         sobol_skip=D._sobol_skip, phi_form="erfc", _clamp=False)
         self.gen_state = D.gen_state
 
 

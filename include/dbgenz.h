@@ -44,6 +44,10 @@ extern "C" {
 /* form of the normal CDF in the loop */
 #define DBG_PHI_ERF 0   /* QMCData (src/qmc.jl:405-429) */
 #define DBG_PHI_ERFC 1  /* QMCData_Distributions (src/qmc.jl:579-593) */
+#define DBG_PHI_TAIL 2  /* no reference counterpart: complementary tail masses without the
+                           [-9, 9] clamp, Phi^{-1} from the nearer tail (erfcinv); keeps
+                           problems whose per-dimension masses fall below 1e-37 finite.
+                           The FP32 mixed path always uses this form. */
 /* which data structure / loop */
 #define DBG_KIND_DENSE 0      /* QMCData: pivoted factor, dense loop */
 #define DBG_KIND_SPARSE 1     /* QMCDataSparse: unpivoted factor, CSC loop */
@@ -156,6 +160,31 @@ int dbg_plan_get_reps(dbg_plan* plan, double* rep_mant, int32_t* rep_exp);
  * skip + state * m; for DBG_KIND_SPARSEQMC the number of blocks generated */
 int dbg_plan_get_state(dbg_plan* plan, int64_t* state);
 int dbg_plan_set_state(dbg_plan* plan, int64_t state);
+
+/* ---- batched problems of uneven dimension (no reference counterpart: the
+ * reference calls qmc_pnorm!(QMCData(...)) once per problem) ----
+ * K independent problems, each the QMCData(Sigma_k, a_k, b_k) constructor
+ * plus qmc_pnorm! with max_pts = m (one round), Richtmyer points, erf form,
+ * d_k <= 512 and inside one pivot window.  Inputs are concatenated:
+ * sigmas = [Sigma_1 (d_1 x d_1, column-major), Sigma_2, ...], a and b =
+ * [a_1, a_2, ...], shifts = [(d_1 - 1) x n_reps, ...].  dbg_batch_create
+ * uploads, picks the dimension-block width with a timed trial on a sample of
+ * the work (8, 16 or 32), and captures the whole job -- the batched device
+ * factorizations, one persistent integration launch over all
+ * (problem, shift, tile) work items largest-d first, and the per-problem
+ * reductions -- into one HIP graph that dbg_batch_run replays.
+ * results: K x 8 doubles per problem: value, log value, err3, rel_err3,
+ * mean mantissa, err mantissa, binary exponent, rank (value NaN if the
+ * factor is rank deficient). */
+typedef struct dbg_batch dbg_batch;
+int dbg_batch_create(dbg_batch** batch, const dbg_opts* opts, int32_t K, const int32_t* dims,
+                     const double* sigmas, const double* a, const double* b, const double* shifts);
+int dbg_batch_run(dbg_batch* batch, double* results);
+/* device time of the last dbg_batch_run (HIP events around the graph) */
+int dbg_batch_last_ms(dbg_batch* batch, float* ms);
+/* chosen block width and the trial times of widths 8, 16, 32 (ms) */
+int dbg_batch_tuning(dbg_batch* batch, int32_t* width, float* ms_w8_w16_w32);
+int dbg_batch_destroy(dbg_batch* batch);
 
 #ifdef __cplusplus
 }

@@ -430,7 +430,7 @@ typedef struct {
     int64_t m;        /* points per shift and round */
     int n_reps;       /* number of random shifts */
     int gen;          /* 0 Richtmyer lattice, 1 Sobol */
-    int phi_form;     /* 0 erf form (QMCData), 1 erfc form (QMCData_Distributions) */
+    int phi_form;     /* 0 erf form (QMCData), 1 erfc form (QMCData_Distributions), 2 tail form */
     int bs_i, bs_i2, bs_j;
     int n_bits;       /* Sobol bits L */
     int threads;      /* OpenMP threads, <= 0: all */
@@ -447,6 +447,75 @@ typedef struct {
 } orc_points;
 
 static const double EPS_T = DBL_EPSILON;
+
+/* The shifted uniform of one (point, dimension, shift) before the tent map:
+ * shift mod 1 (Richtmyer, src/qmc_generators.jl:158-159) or digital shift
+ * (Sobol, :197-198). */
+static inline double point_u(const orc_desc* D, const orc_points* P, int i /*1-based*/,
+                             int k /*0-based shift*/, int64_t j /*1-based point*/)
+{
+    const int nd = D->n - 1;
+    double u;
+    if (D->gen == 0) {
+        const double shift_i = P->shifts_f[(size_t)(i - 1) + (size_t)k * nd];
+        u = shift_i + richtmyer_x(P->n_base + j, P->q[i - 1]);
+        u = u > 1.0 ? u - 1.0 : u;
+    } else {
+        const int L = D->n_bits;
+        const uint64_t mask = (L >= 64) ? ~0ull : ((1ull << L) - 1);
+        const double invscale = ldexp(1.0, -L);
+        const uint64_t shift_i = P->shifts_u[(size_t)(i - 1) + (size_t)k * nd] & mask;
+        const uint64_t x_ij = (uint64_t)sobol_x(P->V + (size_t)(i - 1) * 32, P->sobol_first + (uint64_t)j) & mask;
+        u = (double)((x_ij ^ shift_i) & mask) * invscale;
+    }
+    return u;
+}
+
+/* Lower-tail normal quantile Phi^{-1}(t) for 0 < t <= 1/2: AS 241 start and one
+ * Newton step on Phi(x) = erfc(-x / sqrt 2) / 2 (relative, no cancellation). */
+double orc_ndtri_tail(double t)
+{
+    double x = as241_ndtri(t);
+    const double phi = exp(-0.5 * x * x) * 0.39894228040143267794;
+    if (phi > 0.0 && isfinite(x)) {
+        const double f = 0.5 * erfc(-x * 0.70710678118654752440) - t;
+        x -= f / phi;
+    }
+    return x;
+}
+
+/* Tail form (phi_form 2; no reference counterpart): the interval of the
+ * dimension is kept as the normal tail masses ea = erfc(|za|)/2,
+ * eb = erfc(|zb|)/2 beyond the erf arguments za, zb (no [-9, 9] clamp):
+ *   zb <= 0:  Phi(a') = ea, dc = eb - ea        (mode 0)
+ *   za >= 0:  1 - Phi(b') = eb, dc = ea - eb    (mode 1)
+ *   else:     dc = 1 - ea - eb                  (mode 2)
+ * and the sample's tail mass comes from the nearer tail. */
+static inline double tail_sample(double w, double ea, double eb, double dc, int mode)
+{
+    const double u_lo = fma(w, dc, ea);
+    const double u_hi = fma(1.0 - w, dc, eb);
+    const int upper = (mode == 1) || (mode == 2 && u_hi < u_lo);
+    const double t = fmax(upper ? u_hi : u_lo, 1e-300);
+    const double y = orc_ndtri_tail(t);
+    return upper ? -y : y;
+}
+
+static inline void tail_masses(double za, double zb, double* ea, double* eb, double* dc, int* mode)
+{
+    *ea = 0.5 * erfc(fabs(za));
+    *eb = 0.5 * erfc(fabs(zb));
+    if (zb <= 0.0) {
+        *mode = 0;
+        *dc = *eb - *ea;
+    } else if (za >= 0.0) {
+        *mode = 1;
+        *dc = *ea - *eb;
+    } else {
+        *mode = 2;
+        *dc = (1.0 - *ea) - *eb;
+    }
+}
 
 /* rand_points!: shift mod 1 (Richtmyer) or digital shift (Sobol), tent,
  * affine map to [c, c+dc], clamp, 2u-1.  src/qmc_generators.jl:137-204 */
@@ -494,6 +563,27 @@ static inline void phi_update(int phi_form, double s_val, double a_i, double b_i
     }
     *c_out = c_val;
     *dc_out = dc_val;
+}
+
+/* y of one dimension for either form (state c/ea, dc, eb, mode). */
+static inline double draw_y(const orc_desc* D, const orc_points* P, int i, int k, int64_t j, double c,
+                            double dc, double eb, int mode)
+{
+    if (D->phi_form == 2) {
+        const double w = fabs(fma(point_u(D, P, i, k, j), 2.0, -1.0));
+        return tail_sample(w, c, eb, dc, mode);
+    }
+    return sqrt(2.0) * orc_erfinv(rand_point(D, P, i, k, j, c, dc));
+}
+
+static inline void phi_state(int phi_form, double s_val, double a_i, double b_i, double inv,
+                             double* c_out, double* dc_out, double* eb_out, int* mode_out)
+{
+    if (phi_form == 2) {
+        tail_masses((a_i - s_val) * inv, (b_i - s_val) * inv, c_out, eb_out, dc_out, mode_out);
+    } else {
+        phi_update(phi_form, s_val, a_i, b_i, inv, c_out, dc_out);
+    }
 }
 
 /* Sum of p_t * 2^{e_t} over a batch, as a scaled number. */
@@ -545,8 +635,8 @@ static scaled_t rep_merge(scaled_t rep, scaled_t sum_p, int64_t n_pts0, int64_t 
  * U: n x n column-major (only the strict upper part and the diagonal are read);
  * a, b: permuted standardized bounds; rep_m/rep_e: per-shift running means. */
 int orc_qmc_loop_dense(const orc_desc* D, const orc_points* P, const double* U, const double* a,
-                       const double* b, int64_t n_pts0, double c_1, double dc_1, double* rep_m,
-                       int* rep_e)
+                       const double* b, int64_t n_pts0, double c_1, double dc_1, double eb_1,
+                       int mode_1, double* rep_m, int* rep_e)
 {
     const int n = D->n;
     const int64_t m = D->m;
@@ -563,6 +653,8 @@ int orc_qmc_loop_dense(const orc_desc* D, const orc_points* P, const double* U, 
             double* s = calloc((size_t)bs_j * (size_t)(bs_i > 0 ? bs_i : 1), sizeof(double));
             double* cv = malloc(sizeof(double) * (size_t)bs_j);
             double* dcv = malloc(sizeof(double) * (size_t)bs_j);
+            double* ebv = malloc(sizeof(double) * (size_t)bs_j);
+            int* mdv = malloc(sizeof(int) * (size_t)bs_j);
             double* pm = malloc(sizeof(double) * (size_t)bs_j);
             int* pe = malloc(sizeof(int) * (size_t)bs_j);
 #define Y1(t, i) Y[(size_t)((t)-1) + (size_t)((i)-1) * (size_t)bs_j]
@@ -576,6 +668,8 @@ int orc_qmc_loop_dense(const orc_desc* D, const orc_points* P, const double* U, 
                 for (int t = 0; t < bs_j; ++t) {
                     cv[t] = c_1;
                     dcv[t] = dc_1;
+                    ebv[t] = eb_1;
+                    mdv[t] = mode_1;
                     pm[t] = p0.m;
                     pe[t] = p0.e;
                 }
@@ -611,8 +705,8 @@ int orc_qmc_loop_dense(const orc_desc* D, const orc_points* P, const double* U, 
                         }
                         for (int i = i1; i <= i2; ++i) {
                             for (int t = 1; t <= len; ++t) {
-                                const double yv = rand_point(D, P, i, k, j1 + t - 1, cv[t - 1], dcv[t - 1]);
-                                Y1(t, i) = c2 * orc_erfinv(yv);
+                                Y1(t, i) = draw_y(D, P, i, k, j1 + t - 1, cv[t - 1], dcv[t - 1], ebv[t - 1],
+                                                  mdv[t - 1]);
                             }
                             const int col = i - ii1 + 1;
                             const int K = i - i1 + 1;
@@ -623,10 +717,9 @@ int orc_qmc_loop_dense(const orc_desc* D, const orc_points* P, const double* U, 
                             const double a_i = a[i], b_i = b[i]; /* a[i+1], 1-based */
                             const double inv = 1.0 / (U1(i + 1, i + 1) * c2);
                             for (int t = 1; t <= len; ++t) {
-                                double cval, dcval;
-                                phi_update(D->phi_form, S1(t, col), a_i, b_i, inv, &cval, &dcval);
-                                cv[t - 1] = cval;
-                                dcv[t - 1] = dcval;
+                                phi_state(D->phi_form, S1(t, col), a_i, b_i, inv, &cv[t - 1], &dcv[t - 1],
+                                          &ebv[t - 1], &mdv[t - 1]);
+                                const double dcval = dcv[t - 1];
                                 int e;
                                 pm[t - 1] = frexp(pm[t - 1] * dcval, &e);
                                 pe[t - 1] += e;
@@ -638,7 +731,7 @@ int orc_qmc_loop_dense(const orc_desc* D, const orc_points* P, const double* U, 
             }
 #undef Y1
 #undef S1
-            free(Y); free(s); free(cv); free(dcv); free(pm); free(pe);
+            free(Y); free(s); free(cv); free(dcv); free(ebv); free(mdv); free(pm); free(pe);
         }
         scaled_t tot = {0.0, 0};
         for (int64_t bi = 0; bi < n_batches; ++bi) tot = sc_add(tot, bsum[bi]);
@@ -660,7 +753,8 @@ int orc_qmc_loop_dense(const orc_desc* D, const orc_points* P, const double* U, 
 int orc_qmc_loop_sparse(const orc_desc* D, const orc_points* P, const int64_t* colptr,
                         const int32_t* rowval, const double* nzval, const double* Ud,
                         const double* a, const double* b, int64_t n_pts0, double c_1,
-                        double dc_1, int tent, double* rep_m, int* rep_e)
+                        double dc_1, double eb_1, int mode_1, int tent, double* rep_m,
+                        int* rep_e)
 {
     const int n = D->n;
     const int64_t m = D->m;
@@ -676,6 +770,8 @@ int orc_qmc_loop_sparse(const orc_desc* D, const orc_points* P, const int64_t* c
             double* Y = malloc(sizeof(double) * (size_t)bs_j * (size_t)(nd > 0 ? nd : 1));
             double* cv = malloc(sizeof(double) * (size_t)bs_j);
             double* dcv = malloc(sizeof(double) * (size_t)bs_j);
+            double* ebv = malloc(sizeof(double) * (size_t)bs_j);
+            int* mdv = malloc(sizeof(int) * (size_t)bs_j);
             double* pm = malloc(sizeof(double) * (size_t)bs_j);
             int* pe = malloc(sizeof(int) * (size_t)bs_j);
 #pragma omp for schedule(static)
@@ -687,6 +783,8 @@ int orc_qmc_loop_sparse(const orc_desc* D, const orc_points* P, const int64_t* c
                 for (int t = 0; t < bs_j; ++t) {
                     cv[t] = c_1;
                     dcv[t] = dc_1;
+                    ebv[t] = eb_1;
+                    mdv[t] = mode_1;
                     pm[t] = p0.m;
                     pe[t] = p0.e;
                 }
@@ -694,7 +792,8 @@ int orc_qmc_loop_sparse(const orc_desc* D, const orc_points* P, const int64_t* c
                     for (int t = 0; t < len; ++t) {
                         double yv;
                         if (tent) {
-                            yv = rand_point(D, P, i, k, j1 + t, cv[t], dcv[t]);
+                            Y[(size_t)t + (size_t)(i - 1) * bs_j] = draw_y(D, P, i, k, j1 + t, cv[t], dcv[t], ebv[t], mdv[t]);
+                            continue;
                         } else {
                             const int L = D->n_bits;
                             const uint64_t mask = (L >= 64) ? ~0ull : ((1ull << L) - 1);
@@ -713,10 +812,8 @@ int orc_qmc_loop_sparse(const orc_desc* D, const orc_points* P, const int64_t* c
                         double sv = 0.0;
                         for (int64_t p = colptr[col]; p < colptr[col + 1]; ++p)
                             sv = fma(Y[(size_t)t + (size_t)rowval[p] * bs_j], nzval[p], sv);
-                        double cval, dcval;
-                        phi_update(0, sv, a_i, b_i, inv, &cval, &dcval);
-                        cv[t] = cval;
-                        dcv[t] = dcval;
+                        phi_state(D->phi_form == 2 ? 2 : 0, sv, a_i, b_i, inv, &cv[t], &dcv[t], &ebv[t], &mdv[t]);
+                        const double dcval = dcv[t];
                         int e;
                         pm[t] = frexp(pm[t] * dcval, &e);
                         pe[t] += e;
@@ -724,7 +821,7 @@ int orc_qmc_loop_sparse(const orc_desc* D, const orc_points* P, const int64_t* c
                 }
                 bsum[bi] = batch_sum(pm, pe, len);
             }
-            free(Y); free(cv); free(dcv); free(pm); free(pe);
+            free(Y); free(cv); free(dcv); free(ebv); free(mdv); free(pm); free(pe);
         }
         scaled_t tot = {0.0, 0};
         for (int64_t bi = 0; bi < n_batches; ++bi) tot = sc_add(tot, bsum[bi]);
@@ -797,8 +894,13 @@ int orc_qmc_pnorm(const orc_desc* D, int kind, const double* U, const double* a,
     const double sq2 = sqrt(2.0);
     const double arg_a = a[0] / (u11 * sq2);
     const double arg_b = b[0] / (u11 * sq2);
-    double c_1, dc_1;
-    if (D->phi_form == 1 && kind == 0) {
+    double c_1, dc_1, eb_1 = 0.0;
+    int mode_1 = 0;
+    if (D->phi_form == 2 && kind != 2) {
+        /* tail form of src/qmc.jl:800-809: unclamped arguments */
+        tail_masses(arg_a, arg_b, &c_1, &eb_1, &dc_1, &mode_1);
+        if (mode_1 == 2) dc_1 = 0.5 * (erf(arg_b) - erf(arg_a));
+    } else if (D->phi_form == 1 && kind == 0) {
         c_1 = 0.5 * erfc(-arg_a);
         dc_1 = 0.5 * (erfc(-arg_b) - erfc(-arg_a));
     } else {
@@ -844,10 +946,10 @@ int orc_qmc_pnorm(const orc_desc* D, int kind, const double* U, const double* a,
                 rep_e[k] = rep.e;
             }
         } else if (kind == 0) {
-            orc_qmc_loop_dense(D, &P, U, a, b, n_pts, c_1, dc_1, rep_m, rep_e);
+            orc_qmc_loop_dense(D, &P, U, a, b, n_pts, c_1, dc_1, eb_1, mode_1, rep_m, rep_e);
         } else {
-            orc_qmc_loop_sparse(D, &P, colptr, rowval, nzval, Ud, a, b, n_pts, c_1, dc_1,
-                                kind == 1, rep_m, rep_e);
+            orc_qmc_loop_sparse(D, &P, colptr, rowval, nzval, Ud, a, b, n_pts, c_1, dc_1, eb_1,
+                                mode_1, kind == 1, rep_m, rep_e);
         }
         rounds++;
         reps_summary(rep_m, rep_e, D->n_reps, &mean_m, &err_m, &E);

@@ -40,6 +40,8 @@ def lib():
         L.orc_erfinv.argtypes = [C.c_double]
         L.orc_as241.restype = C.c_double
         L.orc_as241.argtypes = [C.c_double]
+        L.orc_ndtri_tail.restype = C.c_double
+        L.orc_ndtri_tail.argtypes = [C.c_double]
         L.orc_sqrt_primes.argtypes = [C.c_int, dp]
         L.orc_richtmyer_block.argtypes = [C.c_int, C.c_int64, C.c_int64, dp, dp]
         L.orc_sobol_block.argtypes = [C.c_int, u32p, C.c_uint64, C.c_int64, u64p]
@@ -76,6 +78,11 @@ def erfinv(y: float) -> float:
 
 def as241(p: float) -> float:
     return lib().orc_as241(float(p))
+
+
+def ndtri_tail(t: float) -> float:
+    """Phi^{-1}(t) for 0 < t <= 1/2 (the tail form's quantile)."""
+    return lib().orc_ndtri_tail(float(t))
 
 
 def sqrt_primes(n: int) -> np.ndarray:
@@ -185,7 +192,8 @@ def qmc_pnorm(U, a, b, *, m, n_reps, gen="Richtmyer", phi_form="erf", kind="dens
     """qmc_pnorm!  src/qmc.jl:783-850 over a prepared factor.
 
     kind: "dense" (QMCData, qmc.jl:315-444; phi_form "erfc" = QMCData_Distributions
-    :485-608), "sparse" (QMCDataSparse, :618-718) or "sparseqmc" (SparseQMC module,
+    :485-608; phi_form "tail" = the complementary tail form without the [-9, 9]
+    clamp, which has no reference counterpart), "sparse" (QMCDataSparse, :618-718) or "sparseqmc" (SparseQMC module,
     src/qmc_sparse.jl:159-317, Sobol only).  U must already be truncated; a and b
     are the permuted, standardized bounds.  Block sizes are used as given (the
     QMCData constructor clamps them first, src/qmc.jl:166-174; see
@@ -200,7 +208,7 @@ def qmc_pnorm(U, a, b, *, m, n_reps, gen="Richtmyer", phi_form="erf", kind="dens
     gen_i = {"Richtmyer": 0, "Sobol": 1}[gen]
     if kind_i == 2:
         gen_i = 1
-    desc = _Desc(n, m, n_reps, gen_i, {"erf": 0, "erfc": 1}[phi_form], max(bs_i, 1), max(bs_i2, 1),
+    desc = _Desc(n, m, n_reps, gen_i, {"erf": 0, "erfc": 1, "tail": 2}[phi_form], max(bs_i, 1), max(bs_i2, 1),
                  bs_j, n_bits, threads)
     if max_pts is None:
         max_pts = m

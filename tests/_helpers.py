@@ -17,7 +17,9 @@ def oracle_same_inputs(O, D, *, kind=None, phi=None, gen_state=None, max_pts=Non
     if kind is None:
         kind = {L.KIND_DENSE: "dense", L.KIND_SPARSE: "sparse", L.KIND_SPARSEQMC: "sparseqmc"}[D._kind]
     if phi is None:
-        phi = "erfc" if D._phi == L.PHI_ERFC else "erf"
+        phi = {L.PHI_ERF: "erf", L.PHI_ERFC: "erfc", L.PHI_TAIL: "tail"}[D._phi]
+        if D.T == np.float32:
+            phi = "tail"
     gen = "Sobol" if (D.qmc_type == "Sobol" or kind == "sparseqmc") else "Richtmyer"
     V = direction_integers(max(n - 1, 1)) if gen == "Sobol" else None
     skip = D._sobol_skip if D._sobol_skip is not None else skip_count(2 * opts.m - 1)

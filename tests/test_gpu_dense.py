@@ -13,11 +13,11 @@ pytestmark = pytest.mark.gpu
 F64_TOL = 1e-12      # relative, same factor / points / shifts
 
 
-def _plan(dbg, P, T=np.float64, gen="Richtmyer", **ow):
+def _plan(dbg, P, T=np.float64, gen="Richtmyer", phi_form=None, **ow):
     kw = dict(m=P.m, n_reps=P.n_reps, max_pts=P.m, max_abs_err=0.0)
     kw.update(ow)
     opts = dbg.QMC_opts(T, **kw)
-    return dbg.QMCData(P.Sigma, P.a, P.b, opts, None, gen, shifts=P.shifts)
+    return dbg.QMCData(P.Sigma, P.a, P.b, opts, None, gen, shifts=P.shifts, phi_form=phi_form)
 
 
 def test_config0_fp64(dbg, oracle):
@@ -88,18 +88,43 @@ def test_identity_closed_form(dbg):
     assert rel(v, exact) < 1e-13 and e < 1e-15
 
 
-def test_underflow_log_domain(dbg, oracle):
-    """Lower orthant of a 500-dim factor model: P is far below 1e-308; the
-    log of the estimate matches the oracle's.  The erfc form is used: with
-    the erf form, 1 + erf(x) is exactly 0 in Float64 once x < -6, so every
-    weight of this problem becomes 0 (in the oracle as in the reference)."""
-    P = W.factor_model(500, m=1024, n_reps=3)
-    D = dbg.QMCData_Distributions(_plan(dbg, P))
+@pytest.mark.parametrize("d", [100, 500])
+def test_underflow_tail_form(dbg, oracle, d):
+    """Lower orthant of a factor model: P is far below 1e-308 and every weight
+    of the reference forms is 0 (per-dimension masses fall under the [-9, 9]
+    clamp).  The tail form keeps the weights; the log of the estimate matches
+    the oracle's tail form on the same points."""
+    P = W.factor_model(d, m=1024, n_reps=3)
+    D = _plan(dbg, P, phi_form="tail")
     dbg.qmc_pnorm(D)
     r = oracle_same_inputs(oracle, D)
     lv = D.last_result.log_value
-    print("log P", lv, r.log_value, flush=True)
+    print(f"d={d} log P {lv} oracle {r.log_value}", flush=True)
     assert np.isfinite(lv) and abs(lv - r.log_value) < 1e-9 * abs(r.log_value)
+
+
+@pytest.mark.parametrize("d,form", [(24, "tail"), (150, "erf"), (150, "tail"), (300, "erfc")])
+def test_forms_fp64(dbg, oracle, d, form):
+    """FP64 forms on both dense kernels (d >= 96 runs the matrix-core kernel)."""
+    P = W.dense_random(d, m=1500, n_reps=2, seed=d)
+    D = _plan(dbg, P, phi_form=form)
+    v, _, _ = dbg.qmc_pnorm(D)
+    r = oracle_same_inputs(oracle, D)
+    print(f"d={d} {form} rel {rel(v, r.value):.2e}", flush=True)
+    assert rel(v, r.value) < F64_TOL
+
+
+@pytest.mark.parametrize("d,gen", [(150, "Richtmyer"), (400, "Sobol")])
+def test_mixed_mfma(dbg, oracle, d, gen):
+    """FP32 mixed on the matrix-core kernel against the FP64 oracle (tail
+    form, same factor, points and shifts)."""
+    P = W.dense_random(d, m=4096, n_reps=3, seed=d, gen=gen)
+    D = _plan(dbg, P, T=np.float32, gen=gen)
+    v, _, _ = dbg.qmc_pnorm(D)
+    D64 = _plan(dbg, P, gen=gen, phi_form="tail")
+    r = oracle_same_inputs(oracle, D64)
+    print(f"mixed mfma d={d} {gen} rel err {rel(v, r.value):.3e} (se {r.err3 / 3 / r.value:.3e})", flush=True)
+    assert rel(v, r.value) < 1e-4
 
 
 @pytest.mark.parametrize("gen", ["Richtmyer", "Sobol"])

@@ -1,0 +1,98 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports
+every function include/dbgenz.h declares; option defaults, constructor
+clamps, shift drawing and the synthetic workloads."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    src = open(os.path.join(ROOT, "include", "dbgenz.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(dbg_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol(dbg):
+    from dbgenzpaper_amd import _lib
+
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    names = _declared_functions()
+    assert len(names) >= 18
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(_lib.EXPORTS) == set(names)
+
+
+def test_abi_version_and_defaults(dbg):
+    from dbgenzpaper_amd import _lib
+
+    L = _lib.lib()
+    assert L.dbg_abi_version() == 1
+    o = _lib.Opts()
+    L.dbg_opts_default(ctypes.byref(o), _lib.PREC_FP64)
+    # QMC_opts defaults, src/qmc.jl:111-125
+    assert (o.chol_block_size, o.chol_block_size2, o.m, o.block_size_i, o.block_size_j,
+            o.block_size_i2, o.n_reps, o.n_bits, o.max_pts) == (64, 512, 10000, 64, 64, 32, 12, 32, 10000)
+    assert o.max_abs_err == 1e-6 and o.sobol_skip == -1
+
+
+def test_qmc_opts_mirror(dbg):
+    o = dbg.QMC_opts()
+    assert (o.chol_block_size, o.chol_block_size2, o.m, o.block_size_i, o.block_size_i2,
+            o.block_size_j, o.n_reps, o.n_bits, o.max_pts, o.max_abs_err) == \
+           (2 ** 6, 2 ** 9, 10 ** 4, 2 ** 6, 2 ** 5, 2 ** 6, 12, 32, 10 ** 4, 1e-6)
+    assert dbg.QMC_opts(np.float32).T == np.float32
+    with pytest.raises(TypeError):
+        dbg.QMC_opts(np.int32)
+
+
+def test_constructor_clamps():
+    from dbgenzpaper_amd import _lib as L
+    from dbgenzpaper_amd.qmc import QMC_opts, _clamped_opts
+
+    o = QMC_opts(block_size_i=64, block_size_i2=32)
+    c = _clamped_opts(o, 10, L.KIND_DENSE)          # src/qmc.jl:166-174
+    assert (c.block_size_i, c.block_size_i2) == (9, 9)
+    c = _clamped_opts(o, 40, L.KIND_DENSE)
+    assert (c.block_size_i, c.block_size_i2) == (39, 32)
+    c = _clamped_opts(o, 10, L.KIND_SPARSE)         # src/qmc.jl:239-242
+    assert c.block_size_i == 10
+
+
+def test_draw_shifts_shapes(dbg):
+    s = dbg.draw_shifts(np.random.default_rng(0), 9, 5, "Richtmyer")
+    assert s.shape == (8, 5) and s.dtype == np.float64 and np.all((s >= 0) & (s < 1))
+    u = dbg.draw_shifts(np.random.default_rng(0), 9, 5, "Sobol")
+    assert u.shape == (8, 5) and u.dtype == np.uint64
+    f = dbg.draw_shifts(np.random.default_rng(0), 9, 5, "Richtmyer", np.float32)
+    assert np.all(f == f.astype(np.float32))
+
+
+def test_workloads():
+    from scipy.special import ndtr
+
+    from dbgenzpaper_amd import workloads as W
+
+    P = W.factor_model(50, 8, m=16, n_reps=3)
+    assert np.allclose(np.diag(P.Sigma), 1) and np.all(np.linalg.eigvalsh(P.Sigma) > 0)
+    u = ndtr(P.b)
+    assert np.all((u > 0.0099) & (u < 0.1001)) and np.all(np.isneginf(P.a))
+    B = W.banded(64, 4, m=16, n_reps=2)
+    U = np.linalg.cholesky(B.Sigma).T
+    assert np.allclose(np.diag(B.Sigma), 1) and np.abs(np.triu(U, 5)).max() < 1e-12
+    batch = W.batch_uneven(12, m=16)
+    assert len(batch) == 12 and all(2 <= p.d <= 256 for p in batch)
+
+
+def test_library_missing_fails_loudly(tmp_path, monkeypatch):
+    from dbgenzpaper_amd import _lib
+
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "libdbgenz.so"))
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        _lib.lib()

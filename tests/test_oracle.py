@@ -339,3 +339,41 @@ def test_sobol_estimate(oracle):
     r, _ = _run(oracle, P.Sigma, P.a, P.b, m=1024, R=4, gen="Sobol", sobol_skip=skip_count(2047))
     rr, _ = _run(oracle, P.Sigma, P.a, P.b, m=2 ** 14, R=8)
     assert abs(r.value - rr.value) < 3 * (r.err3 + rr.err3) + 1e-6
+
+
+# --- tail form (no reference counterpart) -------------------------------------------
+def test_ndtri_tail_against_mpmath(oracle):
+    with mp.workdps(400):
+        for t in [1e-300, 1e-200, 1e-100, 1e-40, 1e-17, 1e-8, 1e-3, 0.1, 0.3, 0.4999]:
+            truth = mp.sqrt(2) * mp.erfinv(2 * mp.mpf(t) - 1)
+            assert abs((oracle.ndtri_tail(t) - truth) / truth) < 4e-16
+
+
+def test_tail_form_equals_reference_form_without_underflow(oracle):
+    """Where no interval reaches the [-9, 9] clamp, the tail form is the same
+    estimator as the erf form (differences are rounding)."""
+    P = W.dense_random(32, m=2048, n_reps=3, seed=7)
+    r1, _ = _run(oracle, P.Sigma, P.a, P.b, m=P.m, R=P.n_reps, seed=1)
+    r2, _ = _run(oracle, P.Sigma, P.a, P.b, m=P.m, R=P.n_reps, seed=1, phi="tail")
+    assert abs(r1.value - r2.value) < 1e-13 * r1.value
+
+
+def test_tail_form_identity_closed_form(oracle):
+    n = 16
+    rng = np.random.default_rng(3)
+    a, b = -rng.uniform(0.2, 2, n), rng.uniform(0.2, 2, n)
+    r, _ = _run(oracle, np.eye(n), a, b, m=1024, R=2, phi="tail")
+    assert abs(r.value / np.prod(ndtr(b) - ndtr(a)) - 1) < 1e-13
+
+
+def test_tail_form_deep_lower_orthant(oracle):
+    """Factor-model lower orthant: every weight of the reference forms is 0
+    (per-dimension masses fall under the clamp); the tail form gives a finite
+    log P that agrees with itself across point sets within the QMC error."""
+    P = W.factor_model(100, m=2 ** 10, n_reps=4)
+    r_erf, _ = _run(oracle, P.Sigma, P.a, P.b, m=P.m, R=4, phi="erfc")
+    assert r_erf.value == 0.0
+    r1, _ = _run(oracle, P.Sigma, P.a, P.b, m=2 ** 10, R=4, seed=1, phi="tail")
+    r2, _ = _run(oracle, P.Sigma, P.a, P.b, m=2 ** 12, R=4, seed=2, phi="tail")
+    assert np.isfinite(r1.log_value) and np.isfinite(r2.log_value)
+    assert abs(r1.log_value - r2.log_value) < 3 * (r1.rel_err3 + r2.rel_err3) + 0.05
