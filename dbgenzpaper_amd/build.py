@@ -25,7 +25,8 @@ SOURCES = ["capi.hip", "chol.hip", "prep.hip", "qmc_dense_valu.hip", "qmc_dense_
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 CFLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
-    # packed FP32 VALU ops stay off (see the build notes)
+    # packed FP32 VALU ops stay off: hipcc -O3 would otherwise emit them from
+    # ordinary float code, and some of their forms are not trusted here
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-ffp-contract=off", "-munsafe-fp-atomics",
     "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas",
@@ -48,7 +49,10 @@ def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+    # rocBLAS supplies the plain triangular solve of the blocked Cholesky
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
+           "-L", rocm_lib, "-lrocblas", f"-Wl,-rpath,{rocm_lib}"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

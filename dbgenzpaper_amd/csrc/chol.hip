@@ -14,15 +14,18 @@
 //                       of U inside the window; inner blocks of nb pivots are
 //                       left-looking with a deferred trailing update, like
 //                       the reference's bs1 blocks (d_vec reset per block);
-//   chol_trsm_kernel    the window's rows of U beyond the window
+//   TRSM (rocBLAS)      the window's rows of U beyond the window
 //                       (ldiv!(UpperTriangular(.)', .) of :589-623);
-//   chol_syrk_kernel    the trailing Schur complement update (the mul! of
-//                       :592-595), upper triangle only.
+//   chol_syrk_kernel    the trailing Schur complement updates (the mul! of
+//                       :578-582 inside a window and :592-595 after it), upper
+//                       triangle only.
 //
 // The matrix is column-major with leading dimension lda.  Only the upper
 // triangle is kept up to date; chol_finish zeroes the strict lower triangle.
 #include <float.h>
+#include <rocblas/rocblas.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "dbg_device.h"
@@ -105,11 +108,15 @@ __device__ __forceinline__ bool better(double v, int i, double bv, int bi)
     return i < bi;
 }
 
-// Factor the window [w0, w1) with one workgroup of NT threads (window size at
-// most WMAX).  On a rank failure *C.rank is set and the function returns.
+// Factor the pivots [ib_begin, ib_end) of the window [w0, w1) (whole inner
+// blocks of nb) with one workgroup of NT threads (window size at most WMAX).
+// With inkernel_update the deferred update of the rest of the window after
+// each inner block is done here; otherwise the caller launches it.  On a rank
+// failure *C.rank is set and the function returns.
 template <int NT, int WMAX>
-__device__ __forceinline__ void chol_window_body(const CholArgs& C, int w0, int w1, int nb, int pivot,
-                                                 double eps0)
+__device__ __forceinline__ void chol_window_body(const CholArgs& C, int w0, int w1, int ib_begin,
+                                                 int ib_end, int nb, int pivot, double eps0,
+                                                 bool inkernel_update)
 {
     constexpr int kCholThreads = NT;
     constexpr int kMaxWin = WMAX;
@@ -136,7 +143,7 @@ __device__ __forceinline__ void chol_window_body(const CholArgs& C, int w0, int 
     if (tid == 0) s_stop = 0;
     __syncthreads();
 
-    for (int ib0 = w0; ib0 < w1; ib0 += nb) {
+    for (int ib0 = ib_begin; ib0 < ib_end; ib0 += nb) {
         const int ib1 = min(w1, ib0 + nb);
         for (int k = ib0 + tid; k < w1; k += kCholThreads) s_dsum[k - w0] = 0.0;
         __syncthreads();
@@ -254,7 +261,7 @@ __device__ __forceinline__ void chol_window_body(const CholArgs& C, int w0, int 
         }
         if (s_stop) break;
         // ---- deferred update of the rest of the window (src/cholesky.jl:578-582)
-        if (ib1 < w1) {
+        if (inkernel_update && ib1 < w1) {
             const int wr = w1 - ib1;
             for (int64_t idx = tid; idx < (int64_t)wr * wr; idx += kCholThreads) {
                 const int x = ib1 + (int)(idx % wr);
@@ -277,44 +284,59 @@ __device__ __forceinline__ void chol_window_body(const CholArgs& C, int w0, int 
     __syncthreads();
 }
 
-__global__ __launch_bounds__(kCholThreads) void chol_window_kernel(CholArgs C, int w0, int w1, int nb,
-                                                                   int pivot)
+// One inner block of nb pivots of the window [w0, w1).
+__global__ __launch_bounds__(kCholThreads) void chol_window_kernel(CholArgs C, int w0, int w1, int ib0,
+                                                                   int nb, int pivot)
 {
-    if (*C.rank < C.n) return;  // an earlier window found the matrix rank deficient
-    chol_window_body<kCholThreads, kMaxWin>(C, w0, w1, nb, pivot, *C.eps0);
+    if (*C.rank < C.n) return;  // an earlier pivot found the matrix rank deficient
+    chol_window_body<kCholThreads, kMaxWin>(C, w0, w1, ib0, min(w1, ib0 + nb), nb, pivot, *C.eps0, false);
 }
+
+hipError_t chol_syrk(const CholArgs& C, int k0, int k1, int r0, int r1, hipStream_t s);
 
 hipError_t chol_window(const CholArgs& C, int w0, int w1, int nb, bool pivot, hipStream_t s)
 {
-    hipLaunchKernelGGL(chol_window_kernel, dim3(1), dim3(kCholThreads), 0, s, C, w0, w1, nb,
-                       pivot ? 1 : 0);
-    return hipGetLastError();
-}
-
-// rows [w0, w1) of U beyond the window: solve U_ww^T X = M[w0:w1, c]
-__global__ __launch_bounds__(64) void chol_trsm_kernel(double* M, int64_t lda, int n, int w0, int w1,
-                                                       const int32_t* rank)
-{
-    if (*rank < n) return;
-    const int c = w1 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
-    for (int r = w0; r < w1; ++r) {
-        double acc = MM(r, c);
-        for (int l = w0; l < r; ++l) acc -= MM(l, r) * MM(l, c);
-        MM(r, c) = acc / MM(r, r);
+    for (int ib0 = w0; ib0 < w1; ib0 += nb) {
+        const int ib1 = std::min(w1, ib0 + nb);
+        hipLaunchKernelGGL(chol_window_kernel, dim3(1), dim3(kCholThreads), 0, s, C, w0, w1, ib0, nb,
+                           pivot ? 1 : 0);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        // deferred update of the rest of the window by the tiled kernel
+        if (ib1 < w1) {
+            e = chol_syrk(C, ib0, ib1, ib1, w1, s);
+            if (e != hipSuccess) return e;
+        }
     }
+    return hipSuccess;
 }
 
-// trailing update M[x, y] -= sum_l U[l, x] U[l, y] for w1 <= x <= y < n
+// One rocBLAS handle per host thread and device, created on first use and
+// kept until the process ends; its stream is set before every call.
+static rocblas_handle blas_handle(hipStream_t s)
+{
+    constexpr int kMaxDev = 64;
+    thread_local rocblas_handle handles[kMaxDev] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev) return nullptr;
+    if (!handles[dev] && rocblas_create_handle(&handles[dev]) != rocblas_status_success) {
+        handles[dev] = nullptr;
+        return nullptr;
+    }
+    if (rocblas_set_stream(handles[dev], s) != rocblas_status_success) return nullptr;
+    return handles[dev];
+}
+
+// trailing update M[x, y] -= sum_{l in [k0, k1)} U[l, x] U[l, y] for r0 <= x <= y < r1
 constexpr int kSyTile = 64;
 constexpr int kSyK = 16;
-__global__ __launch_bounds__(256) void chol_syrk_kernel(double* M, int64_t lda, int n, int w0, int w1,
-                                                        const int32_t* rank)
+__global__ __launch_bounds__(256) void chol_syrk_kernel(double* M, int64_t lda, int n, int k0, int k1,
+                                                        int r0, int r1, const int32_t* rank)
 {
     if (*rank < n) return;
     const int tx = blockIdx.x, ty = blockIdx.y;
     if (tx > ty) return;
-    const int x0 = w1 + tx * kSyTile, y0 = w1 + ty * kSyTile;
+    const int x0 = r0 + tx * kSyTile, y0 = r0 + ty * kSyTile;
     __shared__ double Ax[kSyK][kSyTile + 1], Ay[kSyK][kSyTile + 1];
     const int t = threadIdx.x;
     const int tr = t % 16, tc = t / 16;
@@ -323,12 +345,12 @@ __global__ __launch_bounds__(256) void chol_syrk_kernel(double* M, int64_t lda, 
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
-    for (int l0 = w0; l0 < w1; l0 += kSyK) {
+    for (int l0 = k0; l0 < k1; l0 += kSyK) {
         for (int e = t; e < kSyK * kSyTile; e += 256) {
             const int ll = e % kSyK, cc = e / kSyK;
             const int l = l0 + ll;
-            Ax[ll][cc] = (l < w1 && x0 + cc < n) ? MM(l, x0 + cc) : 0.0;
-            Ay[ll][cc] = (l < w1 && y0 + cc < n) ? MM(l, y0 + cc) : 0.0;
+            Ax[ll][cc] = (l < k1 && x0 + cc < r1) ? MM(l, x0 + cc) : 0.0;
+            Ay[ll][cc] = (l < k1 && y0 + cc < r1) ? MM(l, y0 + cc) : 0.0;
         }
         __syncthreads();
 #pragma unroll 4
@@ -350,22 +372,39 @@ __global__ __launch_bounds__(256) void chol_syrk_kernel(double* M, int64_t lda, 
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int x = x0 + tr * 4 + i, y = y0 + tc * 4 + j;
-            if (x <= y && y < n) MM(x, y) -= acc[i][j];
+            if (x <= y && y < r1) MM(x, y) -= acc[i][j];
         }
 }
 
+hipError_t chol_syrk(const CholArgs& C, int k0, int k1, int r0, int r1, hipStream_t s)
+{
+    const int rest = r1 - r0;
+    if (rest <= 0 || k1 <= k0) return hipSuccess;
+    const int nt = (rest + kSyTile - 1) / kSyTile;
+    hipLaunchKernelGGL(chol_syrk_kernel, dim3(nt, nt), dim3(256), 0, s, C.M, C.lda, C.n, k0, k1, r0, r1,
+                       C.rank);
+    return hipGetLastError();
+}
+
+// Rows [w0, w1) of U beyond the window, U_ww^T X = M[w0:w1, w1:n] (a plain
+// triangular solve, left / upper / transposed / non-unit), then the trailing
+// update of [w1, n).  A window that found the matrix rank deficient leaves
+// values that are never read: the caller reports the rank and discards M.
 hipError_t chol_trsm_syrk(const CholArgs& C, int w0, int w1, hipStream_t s)
 {
     const int rest = C.n - w1;
     if (rest <= 0) return hipSuccess;
-    hipLaunchKernelGGL(chol_trsm_kernel, dim3((rest + 63) / 64), dim3(64), 0, s, C.M, C.lda, C.n, w0,
-                       w1, C.rank);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    const int nt = (rest + kSyTile - 1) / kSyTile;
-    hipLaunchKernelGGL(chol_syrk_kernel, dim3(nt, nt), dim3(256), 0, s, C.M, C.lda, C.n, w0, w1,
-                       C.rank);
-    return hipGetLastError();
+    rocblas_handle h = blas_handle(s);
+    if (!h) return hipErrorNotInitialized;
+    const double one = 1.0;
+    double* M = C.M;
+    const int64_t lda = C.lda;
+    const rocblas_status st =
+        rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
+                      rocblas_diagonal_non_unit, w1 - w0, rest, &one, &MM(w0, w0), (rocblas_int)lda,
+                      &MM(w0, w1), (rocblas_int)lda);
+    if (st != rocblas_status_success) return hipErrorLaunchFailure;
+    return chol_syrk(C, w0, w1, w1, C.n, s);
 }
 
 __global__ void chol_zero_lower_kernel(double* M, int64_t lda, int n)
@@ -448,7 +487,7 @@ __global__ __launch_bounds__(256) void batch_factor_kernel(BatchArgs B,
     }
     __syncthreads();
     CholArgs C{M, lda, d, a, b, scale, perm, sv, nullptr, B.rank + p};
-    chol_window_body<256, kBatchMaxDim>(C, 0, d, max(1, min(B.bs1, 64)), 1, eps0);
+    chol_window_body<256, kBatchMaxDim>(C, 0, d, 0, d, max(1, min(B.bs1, 64)), 1, eps0, true);
     const bool ok = B.rank[p] == d;
     // zero the strict lower triangle, truncate_matrixU! (src/qmc.jl:133-142)
     const double min_val = MIXED ? (double)(FLT_EPSILON / (float)(d * 9)) : DBL_EPSILON / (double)(d * 9);
