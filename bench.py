@@ -35,6 +35,12 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 matrix
 PEAK_FP64_TFLOPS = 78.6
 
+# HBM bytes per launch of the mixed dense integration kernel, keyed by
+# (d, N, shifts): rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (factors from
+# a plain 1 GiB device copy), profiles/dense_d2500_mixed_counters.md.  Other
+# shapes report null.
+TRAFFIC_BYTES = {(2500, 2 ** 18, 10): 2.868e11}
+
 
 def _log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -52,12 +58,17 @@ def main() -> int:
                     help="points of shift 1 in the CPU oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES bytes three times (the reference "
+                         "kernel for scaling the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the path would shard "
-              "by (shift, point range) with one reduction of the shift sums; see DESIGN.md)",
+              "by (shift, point range) with one reduction of the shift sums)",
               file=sys.stderr)
         return 2
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
 
     import numpy as np
 
@@ -116,7 +127,8 @@ def main() -> int:
     flops = float(m) * R * d * (d - 1)
     achieved = flops / (k_mixed * 1e-3) / 1e12
     out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                       "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                       "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                       "traffic": TRAFFIC_BYTES.get((d, m, R)),
                        "kernel": "dense integration (mixed)", "flops_per_launch": flops}
     Dm.close()
 
@@ -137,6 +149,24 @@ def main() -> int:
         out["cpu_baseline"] = cpu_leg(P, d, args.cpu_points, out)
 
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def copy_calibration(nbytes):
+    """A plain device-to-device copy of nbytes (torch hands it to the HIP
+    runtime's blit kernel), run three times: under rocprofv3 --pmc the summed
+    FETCH_SIZE and WRITE_SIZE of its dispatches give the counters' scale
+    against known bytes."""
+    import torch
+
+    n = nbytes // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "check": float(y[-1].item())}), flush=True)
     return 0
 
 
