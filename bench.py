@@ -54,8 +54,9 @@ def main() -> int:
     ap.add_argument("--d", type=int, default=2500)
     ap.add_argument("--m", type=int, default=2 ** 18)
     ap.add_argument("--reps", type=int, default=10)
-    ap.add_argument("--cpu-points", type=int, default=2 ** 14,
-                    help="points of shift 1 in the CPU oracle sample")
+    ap.add_argument("--cpu-points", type=int, default=2 ** 18,
+                    help="points of shift 1 in the CPU oracle sample (2^18: the whole first shift, "
+                         "several seconds on 16 host threads)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",

@@ -66,6 +66,8 @@ __global__ __launch_bounds__(kTileB) void batch_integrate_kernel(BatchArgs B, co
         pts.q = B.q;
         pts.shifts_f = B.shifts + P.shift_off;
         pts.n_base = 0;
+        pts.q64 = B.q64;
+        pts.shift32 = B.shift32 + P.shift_off;
 
         typename Path<MIXED, kPhiErf>::State st;
         Weight<T> w;
@@ -82,7 +84,7 @@ __global__ __launch_bounds__(kTileB) void batch_integrate_kernel(BatchArgs B, co
             }
 #pragma unroll
             for (int c = 0; c < W; ++c) s_x[tid * SXS + c] = s[c];
-            walk_block<MIXED, kGenRichtmyer, kPhiErf, kTileB, SXS>(pts, g0, min(W, nd - g0), r, j, tid, st,
+            walk_block<MIXED, kGenRichtmyer, kPhiErf, kTileB, SXS, W>(pts, g0, min(W, nd - g0), r, j, tid, st,
                                                                      w, s_x, s_y, Ys, Ucol, d, ak, bk, invk);
         }
         const double pm = valid ? (double)w.pm : 0.0;

@@ -75,6 +75,22 @@ std::vector<double> sqrt_prime_fracs(int nd)
     return qh;
 }
 
+// q_g in 64-bit fixed point, floor(q_g 2^64) (q_g in (0, 1))
+std::vector<uint64_t> fixed_q64(const std::vector<double>& q)
+{
+    std::vector<uint64_t> r(q.size());
+    for (size_t i = 0; i < q.size(); ++i) r[i] = (uint64_t)ldexp(q[i] - floor(q[i]), 64);
+    return r;
+}
+
+// shifts in 32-bit fixed point, floor(frac(shift) 2^32)
+std::vector<uint32_t> fixed_shift32(const double* sh, size_t cnt)
+{
+    std::vector<uint32_t> r(cnt);
+    for (size_t i = 0; i < cnt; ++i) r[i] = (uint32_t)(uint64_t)ldexp(sh[i] - floor(sh[i]), 32);
+    return r;
+}
+
 struct CholBuffers {
     double* M = nullptr;  // n x n, lda = n
     double* a = nullptr;
@@ -204,6 +220,8 @@ struct dbg_plan {
     // points
     double* q = nullptr;
     double* shifts_f = nullptr;
+    uint64_t* q64 = nullptr;     // mixed path: q in 64-bit fixed point
+    uint32_t* shift32 = nullptr; // mixed path: Richtmyer shifts in 32-bit fixed point
     uint64_t* shifts_u = nullptr;
     uint32_t* V = nullptr;
     bool shifts_set = false, dirs_set = false;
@@ -226,7 +244,7 @@ struct dbg_plan {
     {
         F.release();
         void* ptrs[] = {sig, a0, b0, Uk, ucol32, ak, bk, invk, first, colcnt, colptr, rowval, nzval, q, shifts_f,
-                        shifts_u, V, Y, part_m, part_e, rep_m, rep_e, out};
+                        q64, shift32, shifts_u, V, Y, part_m, part_e, rep_m, rep_e, out};
         for (void* p : ptrs) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -316,7 +334,11 @@ int plan_build(dbg_plan* P, const double* Sigma, int64_t lda, const double* a, c
     const std::vector<double> qh = sqrt_prime_fracs(nd);
     HIPCHK(dalloc(&P->q, qh.size() * sizeof(double)));
     HIPCHK(hipMemcpyAsync(P->q, qh.data(), qh.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    const std::vector<uint64_t> q64h = fixed_q64(qh);
+    HIPCHK(dalloc(&P->q64, q64h.size() * sizeof(uint64_t)));
+    HIPCHK(hipMemcpyAsync(P->q64, q64h.data(), q64h.size() * sizeof(uint64_t), hipMemcpyHostToDevice, s));
     HIPCHK(dalloc(&P->shifts_f, (size_t)std::max(nd, 1) * o.n_reps * sizeof(double)));
+    HIPCHK(dalloc(&P->shift32, (size_t)std::max(nd, 1) * o.n_reps * sizeof(uint32_t)));
     HIPCHK(dalloc(&P->shifts_u, (size_t)std::max(nd, 1) * o.n_reps * sizeof(uint64_t)));
     HIPCHK(dalloc(&P->V, (size_t)std::max(nd, 1) * 32 * sizeof(uint32_t)));
 
@@ -358,6 +380,9 @@ PointArgs point_args(const dbg_plan* P, int64_t block)
     const int L = P->o.n_bits;
     pa.mask = (L >= 64) ? ~0ull : ((1ull << L) - 1ull);
     pa.invscale = ldexp(1.0, -L);
+    pa.sobol_bits = L;
+    pa.q64 = P->q64;
+    pa.shift32 = P->shift32;
     return pa;
 }
 
@@ -558,8 +583,11 @@ int dbg_plan_set_shifts(dbg_plan* P, const void* shifts)
         const bool sob = P->o.generator == DBG_GEN_SOBOL || P->o.kind == DBG_KIND_SPARSEQMC;
         if (sob)
             HIPCHK(hipMemcpyAsync(P->shifts_u, shifts, cnt * sizeof(uint64_t), hipMemcpyHostToDevice, P->stream));
-        else
+        else {
+            const std::vector<uint32_t> s32 = fixed_shift32((const double*)shifts, cnt);
             HIPCHK(hipMemcpyAsync(P->shifts_f, shifts, cnt * sizeof(double), hipMemcpyHostToDevice, P->stream));
+            HIPCHK(hipMemcpyAsync(P->shift32, s32.data(), cnt * sizeof(uint32_t), hipMemcpyHostToDevice, P->stream));
+        }
         HIPCHK(hipStreamSynchronize(P->stream));
     }
     P->shifts_set = true;
@@ -807,7 +835,13 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
     BALLOC(Bt->sample_items, std::max<size_t>(sample.size(), 1) * sizeof(int2));
     BALLOC(B.counter, sizeof(unsigned long long));
     const std::vector<double> qh = sqrt_prime_fracs(dmax - 1);
+    const std::vector<uint64_t> q64h = fixed_q64(qh);
+    const std::vector<uint32_t> s32h = fixed_shift32(shifts, (size_t)sh);
     BALLOC(q_d, qh.size() * sizeof(double));
+    uint64_t* q64_d;
+    uint32_t* s32_d;
+    BALLOC(q64_d, q64h.size() * sizeof(uint64_t));
+    BALLOC(s32_d, std::max<int64_t>(sh, 1) * sizeof(uint32_t));
     int dev = 0, ncu = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -829,6 +863,8 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
     if (!sample.empty())
         HIPCHK(hipMemcpyAsync(Bt->sample_items, sample.data(), sample.size() * sizeof(int2), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(q_d, qh.data(), qh.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(q64_d, q64h.data(), q64h.size() * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    if (sh > 0) HIPCHK(hipMemcpyAsync(s32_d, s32h.data(), sh * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     B.K = K;
     B.n_reps = R;
     B.m = o.m;
@@ -841,6 +877,8 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
     B.n_items = (int64_t)items.size();
     B.q = q_d;
     B.shifts = sh_d;
+    B.q64 = q64_d;
+    B.shift32 = s32_d;
     Bt->n_sample = (int64_t)sample.size();
 
     // ---- autotune the dimension-block width on a sample of the work items

@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include "dbg_internal.h"
+#include "tailq.h"
 
 namespace dbg {
 
@@ -32,6 +33,7 @@ __device__ __attribute__((noinline)) double dbg_erfc(double x) { return erfc(x);
 __device__ __attribute__((noinline)) double dbg_erfcinv(double x) { return erfcinv(x); }
 __device__ __attribute__((noinline)) float dbg_erfcf(float x) { return erfcf(x); }
 __device__ __attribute__((noinline)) float dbg_erfcinvf(float x) { return erfcinvf(x); }
+__device__ __attribute__((noinline)) float dbg_tail_quantile(float t, int k) { return tail_quantile_f32(t, k); }
 
 // ---------------------------------------------------------------------------
 // Point generators (src/qmc_generators.jl)
@@ -74,6 +76,46 @@ __device__ __forceinline__ double point_u(const PointArgs& P, int g, int k, int6
         const uint64_t x = (uint64_t)sobol_x(P.V + (size_t)g * 32, P.sobol_first + (uint64_t)j) & mask;
         return __dmul_rn((double)((x ^ shift) & mask), P.invscale);
     }
+}
+
+// The same uniform as a 32-bit fixed-point fraction u32 * 2^-32 (mixed path).
+// Richtmyer: frac((n_base + j) q) from q in 64-bit fixed point (exact modular
+// product, within 2^-32 of the FP64 value for n_base + j < 2^21), plus the
+// shift in 32-bit fixed point, wrapping mod 1.  Sobol: the masked XOR of
+// the FP64 path, aligned to the top of the word.
+template <int GEN>
+__device__ __forceinline__ uint32_t point_u32(const PointArgs& P, int g, int k, int64_t j)
+{
+    if constexpr (GEN == kGenRichtmyer) {
+        const uint64_t Q = P.q64[g];
+        const uint64_t nj = (uint64_t)(P.n_base + j);
+        const uint32_t nlo = (uint32_t)nj, nhi = (uint32_t)(nj >> 32);
+        const uint32_t qlo = (uint32_t)Q, qhi = (uint32_t)(Q >> 32);
+        const uint32_t x = __umulhi(nlo, qlo) + nlo * qhi + nhi * qlo;
+        return x + P.shift32[(size_t)g + (size_t)k * P.nd];
+    } else {
+        const uint64_t v = ((uint64_t)sobol_x(P.V + (size_t)g * 32, P.sobol_first + (uint64_t)j) ^
+                            P.shifts_u[(size_t)g + (size_t)k * P.nd]) & P.mask;
+        return (uint32_t)(P.sobol_bits <= 32 ? (v << (32 - P.sobol_bits)) : (v >> (P.sobol_bits - 32)));
+    }
+}
+
+// w = |2u - 1| (the tent map of src/qmc_generators.jl:160) and 1 - w in FP32
+// from the fixed-point u; u = 0 gives w = 1.
+__device__ __forceinline__ void tent_w(uint32_t u, float& w, float& wc)
+{
+    const uint32_t tw = (u & 0x80000000u) ? (u << 1) : (0u - (u << 1));
+    constexpr float kS = 2.3283064365386963e-10f;  // 2^-32
+    w = (u == 0u) ? 1.0f : (float)tw * kS;
+    wc = (u == 0u) ? 0.0f : (tw == 0u ? 1.0f : (float)(0u - tw) * kS);
+}
+
+// w = u and 1 - u without the tent map (the SparseQMC module)
+__device__ __forceinline__ void plain_w(uint32_t u, float& w, float& wc)
+{
+    constexpr float kS = 2.3283064365386963e-10f;
+    w = (float)u * kS;
+    wc = (u == 0u) ? 1.0f : (float)(0u - u) * kS;
 }
 
 // ---------------------------------------------------------------------------
@@ -157,11 +199,14 @@ __device__ __forceinline__ void tail_set64(double ea, double eb, double dc, int 
     st.k = k;
 }
 
+// erfc(|z|)/2 is 0 exactly for an infinite bound: skip the call
 __device__ __forceinline__ void tail_masses64(double za, double zb, double& ea, double& eb, double& dc,
                                               int& mode)
 {
-    ea = 0.5 * dbg_erfc(fabs(za));
-    eb = 0.5 * dbg_erfc(fabs(zb));
+    ea = 0.0;
+    eb = 0.0;
+    if (!isinf(za)) ea = 0.5 * dbg_erfc(fabs(za));
+    if (!isinf(zb)) eb = 0.5 * dbg_erfc(fabs(zb));
     if (zb <= 0.0) {
         mode = 0;
         dc = eb - ea;
@@ -184,10 +229,13 @@ __device__ __forceinline__ void tail_from_z(StateTail<double>& st, double za, do
 
 __device__ __forceinline__ void tail_from_z(StateTail<float>& st, float za, float zb)
 {
-    if (fabsf(za) <= 9.0f && fabsf(zb) <= 9.0f) {
+    const bool ia = isinf(za), ib = isinf(zb);
+    if ((ia || fabsf(za) <= 9.0f) && (ib || fabsf(zb) <= 9.0f)) {
         st.k = 0;
-        st.ea = 0.5f * dbg_erfcf(fabsf(za));
-        st.eb = 0.5f * dbg_erfcf(fabsf(zb));
+        st.ea = 0.0f;
+        st.eb = 0.0f;
+        if (!ia) st.ea = 0.5f * dbg_erfcf(fabsf(za));
+        if (!ib) st.eb = 0.5f * dbg_erfcf(fabsf(zb));
         if (zb <= 0.0f) {
             st.mode = 0;
             st.dc = st.eb - st.ea;
@@ -213,6 +261,18 @@ __device__ __forceinline__ double tail_sample_w(const StateTail<double>& st, dou
     const bool upper = (st.mode == 1) || (st.mode == 2 && u_hi < u_lo);
     const double t = fmax(upper ? u_hi : u_lo, 1e-300);
     const double y = -kSqrt2 * dbg_erfcinv(2.0 * t);
+    return upper ? -y : y;
+}
+
+// FP32 sample from the fixed-point point: w and 1 - w as FP32 values, the
+// quantile of the nearer tail mass t * 2^-k in FP32 (log-domain, any k)
+__device__ __forceinline__ float tail_sample_f(const StateTail<float>& st, float w, float wc)
+{
+    const float u_lo = fmaf(w, st.dc, st.ea);
+    const float u_hi = fmaf(wc, st.dc, st.eb);
+    const bool upper = (st.mode == 1) || (st.mode == 2 && u_hi < u_lo);
+    const float t = fmaxf(upper ? u_hi : u_lo, 1.17549435e-38f);
+    const float y = dbg_tail_quantile(t, st.k);
     return upper ? -y : y;
 }
 
@@ -321,20 +381,38 @@ __device__ __forceinline__ typename Path<MIXED, FORM>::T path_sample(const typen
     return Path<MIXED, FORM>::sample_w(st, fabs(fma(u, 2.0, -1.0)));
 }
 
+// y of dimension g for the lane's sample: point, tent map, inverse CDF.  The
+// FP64 forms follow the reference's FP64 arithmetic; the mixed form takes
+// the point in fixed point and the quantile in FP32.
+template <bool MIXED, int GEN, int FORM>
+__device__ __forceinline__ typename Path<MIXED, FORM>::T path_draw(const PointArgs& pts,
+                                                                   const typename Path<MIXED, FORM>::State& st,
+                                                                   int g, int k, int64_t j)
+{
+    if constexpr (MIXED) {
+        float w, wc;
+        tent_w(point_u32<GEN>(pts, g, k, j), w, wc);
+        return tail_sample_f(st, w, wc);
+    } else {
+        return path_sample<MIXED, FORM>(st, point_u<GEN>(pts, g, k, j));
+    }
+}
+
 // ---------------------------------------------------------------------------
-// Walk of one block of up to 16 sampled dimensions [gb, gb + cnt) for the
+// Walk of one block of up to SB sampled dimensions [gb, gb + cnt) for the
 // lane's sample (lane t of a tile of TS samples).
 //
 // On entry s_x[t * SXS + c] holds the conditional mean of target gb + 1 + c
 // accumulated from every dimension before gb.  For each dimension g the lane
 // draws the point, forms y_g, stores it (LDS row s_y[g - gb], global
 // workspace row Ys[g]), completes the mean of target g + 1 with the block's
-// own y values in ascending order (the in-block dot of src/qmc.jl:390-403),
-// and updates its interval and weight (:405-429).  Ucol is the factor in
-// column-major order (Ucol[j * ldc + l] = U[l, j]), read through the scalar
-// cache since every lane needs the same column.
+// own y values in ascending order (the in-block dot of src/qmc.jl:390-403,
+// y held in registers: the loop is unrolled), and updates its interval and
+// weight (:405-429).  Ucol is the factor in column-major order
+// (Ucol[j * ldc + l] = U[l, j]), read through the scalar cache since every
+// lane needs the same column.
 // ---------------------------------------------------------------------------
-template <bool MIXED, int GEN, int FORM, int TS, int SXS, typename T, typename State>
+template <bool MIXED, int GEN, int FORM, int TS, int SXS, int SB, typename T, typename State>
 __device__ __forceinline__ void walk_block(const PointArgs& pts, int gb, int cnt, int k, int64_t j,
                                            int t, State& st, Weight<T>& w, const T* __restrict__ s_x,
                                            T* __restrict__ s_y, T* __restrict__ Ys,
@@ -343,17 +421,22 @@ __device__ __forceinline__ void walk_block(const PointArgs& pts, int gb, int cnt
                                            const T* __restrict__ invk)
 {
     using PT = Path<MIXED, FORM>;
-    for (int cc = 0; cc < cnt; ++cc) {
-        const int g = gb + cc;
-        const double u = point_u<GEN>(pts, g, k, j);
-        const T y = path_sample<MIXED, FORM>(st, u);
-        s_y[cc * TS + t] = y;
-        Ys[(size_t)g * TS + t] = y;
-        T acc = s_x[t * SXS + cc];
-        const T* __restrict__ ucol = Ucol + (size_t)(g + 1) * ldc + gb;
-        for (int l = 0; l <= cc; ++l) acc = fma(s_y[l * TS + t], ucol[l], acc);
-        PT::phi(st, acc, ak[g + 1], bk[g + 1], invk[g + 1]);
-        PT::weigh(w, st);
+    T yv[SB];
+#pragma unroll
+    for (int cc = 0; cc < SB; ++cc) {
+        if (cc < cnt) {  // uniform
+            const int g = gb + cc;
+            const T y = path_draw<MIXED, GEN, FORM>(pts, st, g, k, j);
+            yv[cc] = y;
+            s_y[cc * TS + t] = y;
+            Ys[(size_t)g * TS + t] = y;
+            T acc = s_x[t * SXS + cc];
+            const T* __restrict__ ucol = Ucol + (size_t)(g + 1) * ldc + gb;
+#pragma unroll
+            for (int l = 0; l <= cc; ++l) acc = fma(yv[l], ucol[l], acc);
+            PT::phi(st, acc, ak[g + 1], bk[g + 1], invk[g + 1]);
+            PT::weigh(w, st);
+        }
     }
 }
 

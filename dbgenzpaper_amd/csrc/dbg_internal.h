@@ -20,6 +20,9 @@ struct PointArgs {
     uint64_t sobol_first;      // Sobol: point j uses sequence index sobol_first + j
     uint64_t mask;             // Sobol: 2^L - 1
     double invscale;           // Sobol: 2^-L
+    int sobol_bits;            // Sobol: L
+    const uint64_t* q64;       // mixed Richtmyer: floor(q_g 2^64)
+    const uint32_t* shift32;   // mixed Richtmyer: floor(shift 2^32), nd x n_reps
 };
 
 // Arguments of the dense integration kernels.
@@ -120,6 +123,8 @@ struct BatchArgs {
     unsigned long long* counter;   // work-queue head
     const double* q;
     const double* shifts;
+    const uint64_t* q64;           // q in 64-bit fixed point (mixed)
+    const uint32_t* shift32;       // shifts in 32-bit fixed point (mixed)
     int64_t y_slot;
     double* rep_m;                 // n_reps per problem
     int* rep_e;

@@ -67,6 +67,20 @@ struct MfmaCfg<false> {  // FP64: 128 samples x 128 targets, 64 accumulators per
     static constexpr int TS = 128, PW = 128;
 };
 
+constexpr int kPF = 4;  // depth of the operand ring of the panel contraction
+
+// one K-step of the panel contraction: Y rows k0 + lq of the lane's sample
+// blocks, U rows k0 + lq of the panel's target blocks
+template <typename T, int NSB, int NTB, int TS>
+__device__ __forceinline__ void load_step(T (&av)[NSB], T (&bv)[NTB], const T* __restrict__ ya,
+                                          const T* __restrict__ ub, int64_t ldu, int k0)
+{
+#pragma unroll
+    for (int sb = 0; sb < NSB; ++sb) av[sb] = ya[(size_t)k0 * TS + 16 * sb];
+#pragma unroll
+    for (int tb = 0; tb < NTB; ++tb) bv[tb] = ub[(size_t)k0 * ldu + 16 * tb];
+}
+
 int dense_mfma_tile(bool mixed) { return mixed ? MfmaCfg<true>::TS : MfmaCfg<false>::TS; }
 int dense_mfma_panel(bool mixed) { return mixed ? MfmaCfg<true>::PW : MfmaCfg<false>::PW; }
 
@@ -118,18 +132,32 @@ __global__ __launch_bounds__(256) void dense_mfma_kernel(DenseArgs A, const T* _
 #pragma unroll
                 for (int tb = 0; tb < NTB; ++tb) acc[sb][tb] = V{0, 0, 0, 0};
             {
+                // operands of K-step k0 (4 dimensions) are loaded kPF - 1 steps
+                // ahead into a ring of register buffers, so the loads of
+                // later steps are in flight while the matrix cores run
                 const T* __restrict__ ya = Ys + (size_t)lq * TS + wv * WS + lr;
                 const T* __restrict__ ub = Uk + (size_t)lq * ldu + g0 + 1 + lr;
-                for (int k0 = 0; k0 < g0; k0 += 4) {
-                    T av[NSB], bv[NTB];
+                // g0 is a multiple of PW, hence of 4 kPF: no partial ring turn.
+                // Prefetches past the panel start re-read the last K-step
+                // (unconditional loads keep the wait counts exact).
+                static_assert(PW % (4 * kPF) == 0, "panel width must be a multiple of the ring");
+                T av[kPF][NSB], bv[kPF][NTB];
+                if (g0 > 0) {
 #pragma unroll
-                    for (int sb = 0; sb < NSB; ++sb) av[sb] = ya[(size_t)k0 * TS + 16 * sb];
+                    for (int q = 0; q < kPF - 1; ++q) load_step<T, NSB, NTB, TS>(av[q], bv[q], ya, ub, ldu, 4 * q);
+                }
+                for (int k0 = 0; k0 < g0; k0 += 4 * kPF) {
 #pragma unroll
-                    for (int tb = 0; tb < NTB; ++tb) bv[tb] = ub[(size_t)k0 * ldu + 16 * tb];
+                    for (int q = 0; q < kPF; ++q) {
+                        const int kn = min(k0 + 4 * (q + kPF - 1), g0 - 4);
+                        load_step<T, NSB, NTB, TS>(av[(q + kPF - 1) % kPF], bv[(q + kPF - 1) % kPF], ya, ub, ldu,
+                                                   kn);
 #pragma unroll
-                    for (int sb = 0; sb < NSB; ++sb)
+                        for (int sb = 0; sb < NSB; ++sb)
 #pragma unroll
-                        for (int tb = 0; tb < NTB; ++tb) acc[sb][tb] = Mfma<T>::mac(av[sb], bv[tb], acc[sb][tb]);
+                            for (int tb = 0; tb < NTB; ++tb)
+                                acc[sb][tb] = Mfma<T>::mac(av[q][sb], bv[q][tb], acc[sb][tb]);
+                    }
                 }
             }
             // ---- walk the panel in sub-blocks of 16 dimensions
@@ -150,7 +178,7 @@ __global__ __launch_bounds__(256) void dense_mfma_kernel(DenseArgs A, const T* _
                 __syncthreads();
                 const int cnt = min(kSB, nd - gb);
                 if (walker)
-                    walk_block<MIXED, GEN, FORM, TS, kSXS>(A.pts, gb, cnt, k, j, tid, st, w, s_x, s_y,
+                    walk_block<MIXED, GEN, FORM, TS, kSXS, kSB>(A.pts, gb, cnt, k, j, tid, st, w, s_x, s_y,
                                                              Ys, Ucol, A.ldc, ak, bk, invk);
                 __syncthreads();
                 // later targets of the panel += Y_sub * U[sub, later]

@@ -41,9 +41,18 @@ __global__ __launch_bounds__(kTileS) void sparse_kernel(SparseArgs A)
         Weight<T> w;
         PT::init(st, w, A.first);
         for (int g = 0; g < nd; ++g) {
-            const double u = point_u<GEN>(A.pts, g, k, j);
             // the SparseQMC module uses the shifted point without the tent map
-            const T y = A.tent ? path_sample<MIXED, FORM>(st, u) : PT::sample_w(st, u);
+            T y;
+            if constexpr (MIXED) {
+                const uint32_t u = point_u32<GEN>(A.pts, g, k, j);
+                float wf, wc;
+                if (A.tent) tent_w(u, wf, wc);
+                else plain_w(u, wf, wc);
+                y = tail_sample_f(st, wf, wc);
+            } else {
+                const double u = point_u<GEN>(A.pts, g, k, j);
+                y = A.tent ? path_sample<MIXED, FORM>(st, u) : PT::sample_w(st, u);
+            }
             Ys[(size_t)g * kTileS + tid] = y;
             T sv = T(0);
             const int64_t p1 = colptr[g + 2];

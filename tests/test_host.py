@@ -96,3 +96,55 @@ def test_library_missing_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "libdbgenz.so"))
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         _lib.lib()
+
+
+def _host_build(tmp_path, src, name):
+    """Compile a small host program against the device headers (plain C++)."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    csrc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "dbgenzpaper_amd", "csrc")
+    cpp = tmp_path / f"{name}.cpp"
+    cpp.write_text(src)
+    so = tmp_path / f"lib{name}.so"
+    subprocess.run([gxx, "-O2", "-ffp-contract=off", "-shared", "-fPIC", "-I", csrc, str(cpp), "-o", str(so)],
+                   check=True)
+    return ctypes.CDLL(str(so))
+
+
+def test_fp32_tail_quantile_against_mpmath(tmp_path):
+    """The mixed path's FP32 inverse CDF (AS 241 in FP32, -log p formed as
+    -log t + k log 2) against a 40-digit quantile, from p = 0.5 down to
+    p = 2^-3000 given as scaled masses t 2^-k.  Bound: 4 FP32 ulps of
+    max(|y|, 1)."""
+    mpmath = pytest.importorskip("mpmath")
+    lib = _host_build(tmp_path, '#include "tailq.h"\n'
+                      'extern "C" float tq(float t, int k) { return dbg::tail_quantile_f32(t, k); }\n', "tq")
+    lib.tq.restype = ctypes.c_float
+    lib.tq.argtypes = [ctypes.c_float, ctypes.c_int]
+    mpmath.mp.dps = 40
+
+    def truth(p):
+        y = mpmath.mpf(-0.5) if p > 0.1 else -mpmath.sqrt(-2 * mpmath.log(p))
+        for _ in range(200):
+            f = mpmath.log(mpmath.ncdf(y)) - mpmath.log(p)
+            yn = y - f * mpmath.ncdf(y) / mpmath.npdf(y)
+            if abs(yn - y) < mpmath.mpf(10) ** -30:
+                return yn
+            y = yn
+        return y
+
+    worst = 0.0
+    log2p = np.concatenate([np.linspace(-1.0, -40.0, 157), np.linspace(-40.0, -3000.0, 97)])
+    for lp in log2p:
+        k = max(0, int(-lp) - 100) if lp < -110 else 0
+        t = float(np.float32(2.0 ** (lp + k)))
+        p = mpmath.mpf(t) * mpmath.mpf(2) ** (-k)
+        tr = truth(p)
+        y = lib.tq(t, k)
+        worst = max(worst, float(abs(y - tr) / max(abs(tr), 1)))
+    # p = 1/2 exactly and the central/tail switch at |p - 1/2| = 0.425
+    assert lib.tq(0.5, 0) == 0.0
+    assert worst < 4 * np.finfo(np.float32).eps, worst
