@@ -5,7 +5,7 @@
  * The entry points replace the hot path of JakobPeterlin/DBGenzPaper, a pure
  * Julia package.  Each declaration names the reference routine it stands in
  * for (paths relative to the reference repository root).  A Julia caller binds
- * them with ccall; the binding is spelled out in INTEGRATION.md.
+ * them with one ccall per entry point.
  *
  * Conventions
  *   - Matrices are column-major with an explicit leading dimension, like a
@@ -37,7 +37,7 @@ extern "C" {
 /* precision of the integration (the Julia type parameter T) */
 #define DBG_PREC_FP64 0   /* T = Float64: the reference arithmetic in FP64 */
 #define DBG_PREC_MIXED 1  /* T = Float32: FP32 contraction and special functions,
-                             FP64 factor, points and reductions */
+                             32-bit fixed-point points, FP64 factor and reductions */
 /* point set (qmc_type) */
 #define DBG_GEN_RICHTMYER 0
 #define DBG_GEN_SOBOL 1
@@ -130,7 +130,8 @@ int dbg_plan_refactor(dbg_plan* plan);
 
 /* Random shifts: rand(rng, T, n-1, n_reps) (src/qmc_generators.jl:52) as
  * double, or rand(rng, UInt64, n-1, n_reps) (:63) for Sobol; column-major
- * (n-1) x n_reps.  For DBG_PREC_MIXED the double shifts are used as given. */
+ * (n-1) x n_reps.  For DBG_PREC_MIXED the Richtmyer shifts enter the point
+ * map as 32-bit fixed-point fractions, floor(frac(shift) 2^32). */
 int dbg_plan_set_shifts(dbg_plan* plan, const void* shifts);
 /* Sobol direction integers, (n-1) x 32 row-major: V[d*32+k] = m_{k+1} << (31-k) */
 int dbg_plan_set_sobol_directions(dbg_plan* plan, const uint32_t* V);
