@@ -78,6 +78,28 @@ __device__ __forceinline__ double point_u(const PointArgs& P, int g, int k, int6
     }
 }
 
+// The point-set tables enter each kernel as separate __restrict__ arguments:
+// with that guarantee the compiler reads their per-dimension entries through
+// the scalar cache (uniform addresses, no aliasing store) instead of as vector
+// loads.  with_tables() puts them back into the PointArgs the helpers take.
+#define DBG_POINT_PARAMS                                                                          \
+    const double* __restrict__ pt_q, const double* __restrict__ pt_sf,                            \
+        const uint64_t* __restrict__ pt_q64, const uint32_t* __restrict__ pt_s32,                 \
+        const uint32_t* __restrict__ pt_V, const uint64_t* __restrict__ pt_su
+#define DBG_POINT_ARGS(P) (P).q, (P).shifts_f, (P).q64, (P).shift32, (P).V, (P).shifts_u
+#define DBG_WITH_TABLES(P) with_tables((P), pt_q, pt_sf, pt_q64, pt_s32, pt_V, pt_su)
+
+__device__ __forceinline__ PointArgs with_tables(PointArgs P, DBG_POINT_PARAMS)
+{
+    P.q = pt_q;
+    P.shifts_f = pt_sf;
+    P.q64 = pt_q64;
+    P.shift32 = pt_s32;
+    P.V = pt_V;
+    P.shifts_u = pt_su;
+    return P;
+}
+
 // The same uniform as a 32-bit fixed-point fraction u32 * 2^-32 (mixed path).
 // Richtmyer: frac((n_base + j) q) from q in 64-bit fixed point (exact modular
 // product, within 2^-32 of the FP64 value for n_base + j < 2^21), plus the

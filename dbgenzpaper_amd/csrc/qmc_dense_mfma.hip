@@ -58,16 +58,16 @@ constexpr int kSXS = kSB + 4;  // LDS row stride of the accumulator hand-off
 
 template <bool MIXED>
 struct MfmaCfg;
+// TS samples x PW targets per workgroup; PF: depth of the operand ring of the
+// panel contraction; OCC: waves per SIMD the register budget is cut for.
 template <>
 struct MfmaCfg<true> {   // FP32: 256 samples x 128 targets, 128 accumulators per lane
-    static constexpr int TS = 256, PW = 128;
+    static constexpr int TS = 256, PW = 128, PF = 2, OCC = 2;
 };
 template <>
-struct MfmaCfg<false> {  // FP64: 128 samples x 128 targets, 64 accumulators per lane
-    static constexpr int TS = 128, PW = 128;
+struct MfmaCfg<false> {  // FP64: 128 samples x 128 targets, 128 accumulator registers per lane
+    static constexpr int TS = 128, PW = 128, PF = 4, OCC = 1;
 };
-
-constexpr int kPF = 4;  // depth of the operand ring of the panel contraction
 
 // one K-step of the panel contraction: Y rows k0 + lq of the lane's sample
 // blocks, U rows k0 + lq of the panel's target blocks
@@ -85,18 +85,19 @@ int dense_mfma_tile(bool mixed) { return mixed ? MfmaCfg<true>::TS : MfmaCfg<fal
 int dense_mfma_panel(bool mixed) { return mixed ? MfmaCfg<true>::PW : MfmaCfg<false>::PW; }
 
 template <bool MIXED, int GEN, int FORM, typename T = typename std::conditional<MIXED, float, double>::type>
-__global__ __launch_bounds__(256) void dense_mfma_kernel(DenseArgs A, const T* __restrict__ Uk,
+__global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(DenseArgs A, const T* __restrict__ Uk,
                                                          const T* __restrict__ Ucol,
                                                          const T* __restrict__ ak,
                                                          const T* __restrict__ bk,
                                                          const T* __restrict__ invk,
                                                          T* __restrict__ Yall,
                                                          double* __restrict__ part_m,
-                                                         int* __restrict__ part_e)
+                                                         int* __restrict__ part_e, DBG_POINT_PARAMS)
 {
     using V = typename Mfma<T>::V;
     constexpr int TS = MfmaCfg<MIXED>::TS;
     constexpr int PW = MfmaCfg<MIXED>::PW;
+    constexpr int kPF = MfmaCfg<MIXED>::PF;
     constexpr int WS = TS / 4;         // samples per wavefront
     constexpr int NSB = WS / 16;       // 16-sample blocks per wavefront
     constexpr int NTB = PW / 16;       // 16-target blocks per panel
@@ -113,6 +114,7 @@ __global__ __launch_bounds__(256) void dense_mfma_kernel(DenseArgs A, const T* _
     const int64_t ldu = A.ldu;
     T* __restrict__ Ys = Yall + (size_t)blockIdx.x * (size_t)A.y_slot;
     const bool walker = tid < TS;
+    const PointArgs pts = DBG_WITH_TABLES(A.pts);
 
     for (int64_t tile = blockIdx.x; tile < A.n_tiles; tile += gridDim.x) {
         const int k = (int)(tile / A.tiles_per_rep);
@@ -178,7 +180,7 @@ __global__ __launch_bounds__(256) void dense_mfma_kernel(DenseArgs A, const T* _
                 __syncthreads();
                 const int cnt = min(kSB, nd - gb);
                 if (walker)
-                    walk_block<MIXED, GEN, FORM, TS, kSXS, kSB>(A.pts, gb, cnt, k, j, tid, st, w, s_x, s_y,
+                    walk_block<MIXED, GEN, FORM, TS, kSXS, kSB>(pts, gb, cnt, k, j, tid, st, w, s_x, s_y,
                                                              Ys, Ucol, A.ldc, ak, bk, invk);
                 __syncthreads();
                 // later targets of the panel += Y_sub * U[sub, later]
@@ -220,7 +222,7 @@ static void launch_m(const DenseArgs& A, int grid, hipStream_t s)
     using T = typename std::conditional<MIXED, float, double>::type;
     hipLaunchKernelGGL((dense_mfma_kernel<MIXED, GEN, FORM>), dim3(grid), dim3(256), 0, s, A,
                        (const T*)A.Uk, (const T*)A.Ucol, (const T*)A.ak, (const T*)A.bk,
-                       (const T*)A.invk, (T*)A.Y, A.part_m, A.part_e);
+                       (const T*)A.invk, (T*)A.Y, A.part_m, A.part_e, DBG_POINT_ARGS(A.pts));
 }
 
 hipError_t launch_dense_mfma(const DenseArgs& A, bool mixed, int gen, int form, int grid,

@@ -33,8 +33,9 @@ __global__ __launch_bounds__(kTileV) void dense_valu_kernel(DenseArgs A, const T
                                                             const T* __restrict__ invk,
                                                             T* __restrict__ Yall,
                                                             double* __restrict__ part_m,
-                                                            int* __restrict__ part_e)
+                                                            int* __restrict__ part_e, DBG_POINT_PARAMS)
 {
+    const PointArgs pts = DBG_WITH_TABLES(A.pts);
     __shared__ T s_x[kTileV * kSXV];
     __shared__ T s_y[kWV * kTileV];
     const int tid = threadIdx.x;
@@ -64,7 +65,7 @@ __global__ __launch_bounds__(kTileV) void dense_valu_kernel(DenseArgs A, const T
             }
 #pragma unroll
             for (int c = 0; c < kWV; ++c) s_x[tid * kSXV + c] = s[c];
-            walk_block<MIXED, GEN, FORM, kTileV, kSXV, kWV>(A.pts, g0, min(kWV, nd - g0), k, j, tid, st, w,
+            walk_block<MIXED, GEN, FORM, kTileV, kSXV, kWV>(pts, g0, min(kWV, nd - g0), k, j, tid, st, w,
                                                           s_x, s_y, Ys, Ucol, A.ldc, ak, bk, invk);
         }
         const double pm = valid ? (double)w.pm : 0.0;
@@ -84,7 +85,7 @@ static void launch_v(const DenseArgs& A, int grid, hipStream_t s)
     using T = typename std::conditional<MIXED, float, double>::type;
     hipLaunchKernelGGL((dense_valu_kernel<MIXED, GEN, FORM>), dim3(grid), dim3(kTileV), 0, s, A,
                        (const T*)A.Uk, (const T*)A.Ucol, (const T*)A.ak, (const T*)A.bk,
-                       (const T*)A.invk, (T*)A.Y, A.part_m, A.part_e);
+                       (const T*)A.invk, (T*)A.Y, A.part_m, A.part_e, DBG_POINT_ARGS(A.pts));
 }
 
 hipError_t launch_dense_valu(const DenseArgs& A, bool mixed, int gen, int form, int grid,

@@ -18,8 +18,9 @@ constexpr int kTileS = 256;
 int sparse_tile() { return kTileS; }
 
 template <bool MIXED, int GEN, int FORM>
-__global__ __launch_bounds__(kTileS) void sparse_kernel(SparseArgs A)
+__global__ __launch_bounds__(kTileS) void sparse_kernel(SparseArgs A, DBG_POINT_PARAMS)
 {
+    const PointArgs pts = DBG_WITH_TABLES(A.pts);
     using T = typename std::conditional<MIXED, float, double>::type;
     using PT = Path<MIXED, FORM>;
     const int tid = threadIdx.x;
@@ -44,13 +45,13 @@ __global__ __launch_bounds__(kTileS) void sparse_kernel(SparseArgs A)
             // the SparseQMC module uses the shifted point without the tent map
             T y;
             if constexpr (MIXED) {
-                const uint32_t u = point_u32<GEN>(A.pts, g, k, j);
+                const uint32_t u = point_u32<GEN>(pts, g, k, j);
                 float wf, wc;
                 if (A.tent) tent_w(u, wf, wc);
                 else plain_w(u, wf, wc);
                 y = tail_sample_f(st, wf, wc);
             } else {
-                const double u = point_u<GEN>(A.pts, g, k, j);
+                const double u = point_u<GEN>(pts, g, k, j);
                 y = A.tent ? path_sample<MIXED, FORM>(st, u) : PT::sample_w(st, u);
             }
             Ys[(size_t)g * kTileS + tid] = y;
@@ -77,19 +78,19 @@ hipError_t launch_sparse(const SparseArgs& A, bool mixed, int gen, int grid, hip
     dim3 g(grid), b(kTileS);
     if (mixed) {
         if (gen == kGenRichtmyer)
-            hipLaunchKernelGGL((sparse_kernel<true, kGenRichtmyer, kPhiTail>), g, b, 0, s, A);
+            hipLaunchKernelGGL((sparse_kernel<true, kGenRichtmyer, kPhiTail>), g, b, 0, s, A, DBG_POINT_ARGS(A.pts));
         else
-            hipLaunchKernelGGL((sparse_kernel<true, kGenSobol, kPhiTail>), g, b, 0, s, A);
+            hipLaunchKernelGGL((sparse_kernel<true, kGenSobol, kPhiTail>), g, b, 0, s, A, DBG_POINT_ARGS(A.pts));
     } else if (A.form == kPhiTail) {
         if (gen == kGenRichtmyer)
-            hipLaunchKernelGGL((sparse_kernel<false, kGenRichtmyer, kPhiTail>), g, b, 0, s, A);
+            hipLaunchKernelGGL((sparse_kernel<false, kGenRichtmyer, kPhiTail>), g, b, 0, s, A, DBG_POINT_ARGS(A.pts));
         else
-            hipLaunchKernelGGL((sparse_kernel<false, kGenSobol, kPhiTail>), g, b, 0, s, A);
+            hipLaunchKernelGGL((sparse_kernel<false, kGenSobol, kPhiTail>), g, b, 0, s, A, DBG_POINT_ARGS(A.pts));
     } else {
         if (gen == kGenRichtmyer)
-            hipLaunchKernelGGL((sparse_kernel<false, kGenRichtmyer, kPhiErf>), g, b, 0, s, A);
+            hipLaunchKernelGGL((sparse_kernel<false, kGenRichtmyer, kPhiErf>), g, b, 0, s, A, DBG_POINT_ARGS(A.pts));
         else
-            hipLaunchKernelGGL((sparse_kernel<false, kGenSobol, kPhiErf>), g, b, 0, s, A);
+            hipLaunchKernelGGL((sparse_kernel<false, kGenSobol, kPhiErf>), g, b, 0, s, A, DBG_POINT_ARGS(A.pts));
     }
     return hipGetLastError();
 }

@@ -969,6 +969,185 @@ int orc_qmc_pnorm(const orc_desc* D, int kind, const double* U, const double* a,
     return 0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Adjoint of the tail-form estimate (no reference counterpart)              */
+/* ------------------------------------------------------------------------ */
+/* For one sample with points w_i, the integrand is p = prod_i dc_i with
+ *   alpha_i = (a_i - s_i) / U_ii, beta_i = (b_i - s_i) / U_ii, s_0 = 0,
+ *   c_i = Phi(alpha_i), dc_i = Phi(beta_i) - Phi(alpha_i),
+ *   y_i = Phi^{-1}(c_i + w_i dc_i)             (i <= d - 2),
+ *   s_t = sum_{l < t} U[l, t] y_l.
+ * log P = log mean(p), so d log P = sum_s p_s dL_s / sum_s p_s with
+ * L = sum_i log dc_i.  The reverse sweep, i = d-1 .. 0:
+ *   dcbar_i = 1/dc_i + ybar_i w_i / phi(y_i),  cbar_i = ybar_i / phi(y_i)
+ *   albar_i = (cbar_i - dcbar_i) phi(alpha_i), bebar_i = dcbar_i phi(beta_i)
+ *   abar_i += albar_i / U_ii, bbar_i += bebar_i / U_ii,
+ *   sbar_i = -(albar_i + bebar_i) / U_ii,
+ *   Ubar_ii += -(albar_i alpha_i + bebar_i beta_i) / U_ii,
+ *   ybar_l += U[l, i] sbar_i, Ubar[l, i] += sbar_i y_l   (l < i).
+ * Phi^{-1} is differentiated as a function (dy/du = 1/phi(y)); the tail
+ * form only changes how u is evaluated, not the function.  The weights p_s
+ * are carried relative to a running maximum of L (rescaled when it grows). */
+static inline double npdf(double x) { return isinf(x) ? 0.0 : exp(-0.5 * x * x) * 0.39894228040143267794; }
+
+typedef struct {
+    double* ga;
+    double* gb;
+    double* gU;
+    double W;     /* sum of weights exp(L - M) */
+    double M;     /* running maximum of L */
+} grad_acc;
+
+static void grad_rescale(grad_acc* G, int n, double M_new)
+{
+    const double f = exp(G->M - M_new);
+    for (int i = 0; i < n; ++i) {
+        G->ga[i] *= f;
+        G->gb[i] *= f;
+    }
+    for (size_t e = 0; e < (size_t)n * n; ++e) G->gU[e] *= f;
+    G->W *= f;
+    G->M = M_new;
+}
+
+int orc_qmc_grad_dense(const orc_desc* D, const double* U, const double* a, const double* b,
+                       const double* q, const double* shifts_f, const uint32_t* V,
+                       const uint64_t* shifts_u, uint64_t sobol_skip, double* ga, double* gb,
+                       double* gU, double* logP)
+{
+    const int n = D->n, nd = n - 1;
+    const double sq2 = sqrt(2.0);
+    orc_points P;
+    memset(&P, 0, sizeof(P));
+    P.q = q;
+    P.shifts_f = shifts_f;
+    P.V = V;
+    P.shifts_u = shifts_u;
+    P.sobol_first = sobol_skip;
+    const int64_t total = D->m * (int64_t)D->n_reps;
+    int nt = 1;
+#ifdef _OPENMP
+    nt = D->threads > 0 ? D->threads : omp_get_max_threads();
+#endif
+    grad_acc* acc = calloc((size_t)nt, sizeof(grad_acc));
+    for (int t = 0; t < nt; ++t) {
+        acc[t].ga = calloc((size_t)n, sizeof(double));
+        acc[t].gb = calloc((size_t)n, sizeof(double));
+        acc[t].gU = calloc((size_t)n * n, sizeof(double));
+        acc[t].M = -INFINITY;
+    }
+#pragma omp parallel num_threads(nt)
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        grad_acc* G = &acc[tid];
+        double* al = malloc(sizeof(double) * (size_t)n);
+        double* be = malloc(sizeof(double) * (size_t)n);
+        double* dc = malloc(sizeof(double) * (size_t)n);
+        double* w = malloc(sizeof(double) * (size_t)n);
+        double* y = malloc(sizeof(double) * (size_t)n);
+        double* yb = malloc(sizeof(double) * (size_t)n);
+        double* lga = malloc(sizeof(double) * (size_t)n);
+        double* lgb = malloc(sizeof(double) * (size_t)n);
+        double* lgd = malloc(sizeof(double) * (size_t)n);   /* diagonal of Ubar */
+        double* sb = malloc(sizeof(double) * (size_t)n);
+#pragma omp for schedule(static)
+        for (int64_t smp = 0; smp < total; ++smp) {
+            const int k = (int)(smp / D->m);
+            const int64_t j = smp % D->m + 1;
+            /* forward */
+            double L = 0.0;
+            double s = 0.0;
+            for (int i = 0; i < n; ++i) {
+                if (i > 0) {
+                    s = 0.0;
+                    for (int l = 0; l < i; ++l) s += U[(size_t)l + (size_t)i * n] * y[l];
+                }
+                const double uii = U[(size_t)i + (size_t)i * n];
+                al[i] = (a[i] - s) / uii;
+                be[i] = (b[i] - s) / uii;
+                double ea, eb, dci;
+                int mode;
+                tail_masses(al[i] / sq2, be[i] / sq2, &ea, &eb, &dci, &mode);
+                dc[i] = dci;
+                L += log(dci);
+                if (i < nd) {
+                    w[i] = fabs(fma(point_u(D, &P, i + 1, k, j), 2.0, -1.0));
+                    y[i] = tail_sample(w[i], ea, eb, dci, mode);
+                }
+            }
+            if (!(L > -INFINITY)) continue;  /* zero weight */
+            /* reverse */
+            for (int l = 0; l < n; ++l) {
+                yb[l] = 0.0;
+                sb[l] = 0.0;
+            }
+            for (int i = n - 1; i >= 0; --i) {
+                double dcb = 1.0 / dc[i], cb = 0.0;
+                if (i < nd) {
+                    const double iphy = 1.0 / npdf(y[i]);
+                    dcb += yb[i] * w[i] * iphy;
+                    cb = yb[i] * iphy;
+                }
+                const double pa = npdf(al[i]), pb = npdf(be[i]);
+                const double alb = (cb - dcb) * pa, beb = dcb * pb;
+                const double uii = U[(size_t)i + (size_t)i * n];
+                lga[i] = alb / uii;
+                lgb[i] = beb / uii;
+                const double ta = isinf(al[i]) ? 0.0 : alb * al[i];
+                const double tb = isinf(be[i]) ? 0.0 : beb * be[i];
+                lgd[i] = -(ta + tb) / uii;
+                sb[i] = -(alb + beb) / uii;
+                if (i > 0)
+                    for (int l = 0; l < i; ++l) yb[l] += U[(size_t)l + (size_t)i * n] * sb[i];
+            }
+            /* accumulate with weight exp(L - M) */
+            if (L > G->M) grad_rescale(G, n, L);
+            const double wt = exp(L - G->M);
+            G->W += wt;
+            for (int i = 0; i < n; ++i) {
+                G->ga[i] += wt * lga[i];
+                G->gb[i] += wt * lgb[i];
+                G->gU[(size_t)i + (size_t)i * n] += wt * lgd[i];
+                for (int l = 0; l < i; ++l) G->gU[(size_t)l + (size_t)i * n] += wt * sb[i] * y[l];
+            }
+        }
+        free(al); free(be); free(dc); free(w); free(y); free(yb); free(lga); free(lgb); free(lgd); free(sb);
+    }
+    /* merge the threads */
+    double M = -INFINITY;
+    for (int t = 0; t < nt; ++t) M = fmax(M, acc[t].M);
+    double W = 0.0;
+    for (int i = 0; i < n; ++i) ga[i] = gb[i] = 0.0;
+    for (size_t e = 0; e < (size_t)n * n; ++e) gU[e] = 0.0;
+    for (int t = 0; t < nt; ++t) {
+        if (acc[t].M == -INFINITY) continue;
+        grad_rescale(&acc[t], n, M);
+        W += acc[t].W;
+        for (int i = 0; i < n; ++i) {
+            ga[i] += acc[t].ga[i];
+            gb[i] += acc[t].gb[i];
+        }
+        for (size_t e = 0; e < (size_t)n * n; ++e) gU[e] += acc[t].gU[e];
+    }
+    for (int i = 0; i < n; ++i) {
+        ga[i] /= W;
+        gb[i] /= W;
+    }
+    for (size_t e = 0; e < (size_t)n * n; ++e) gU[e] /= W;
+    /* log P = M + log(W / total) */
+    *logP = M + log(W / (double)total);
+    for (int t = 0; t < nt; ++t) {
+        free(acc[t].ga);
+        free(acc[t].gb);
+        free(acc[t].gU);
+    }
+    free(acc);
+    return 0;
+}
+
 /* The batch loop uses orc_erfinv; expose erf-form helpers for the tests. */
 double orc_phi_c(double x) { return 0.5 * (1.0 + erf(x)); }
 

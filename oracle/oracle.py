@@ -52,6 +52,8 @@ def lib():
         L.orc_qmc_pnorm.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, i64p, C.POINTER(C.c_int32), dp,
                                     dp, dp, u32p, u64p, C.c_uint64, C.c_int64, C.c_double, i64p, dp,
                                     ip, C.c_void_p]
+        L.orc_qmc_grad_dense.argtypes = [C.c_void_p, dp, dp, dp, dp, dp, u32p, u64p, C.c_uint64, dp, dp,
+                                         dp, dp]
         L.orc_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -244,6 +246,48 @@ def qmc_pnorm(U, a, b, *, m, n_reps, gen="Richtmyer", phi_form="erf", kind="dens
     assert rc == 0
     return PnormResult(res.value, res.log_value, res.err3, res.rel_err3, res.n_pts, res.rounds,
                        np.ldexp(rep_m, rep_e), rep_m, rep_e.astype(np.int64), state.value)
+
+
+@dataclass
+class GradResult:
+    ga: np.ndarray        # d log P / d a (standardized, permuted bounds)
+    gb: np.ndarray        # d log P / d b
+    gU: np.ndarray        # d log P / d U (upper triangle incl. diagonal, column-major n x n)
+    log_value: float      # log P of the same estimate
+
+
+def qmc_grad(U, a, b, *, m, n_reps, gen="Richtmyer", shifts=None, q=None, V=None, sobol_skip=None,
+             n_bits=32, threads=0) -> GradResult:
+    """Gradient of log P (tail form, dense loop, one block of m points per
+    shift) with respect to a, b and U, in FP64.  No reference counterpart:
+    the reverse sweep of the forward loop, documented in genz_oracle.c."""
+    U = np.asfortranarray(U, np.float64)
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    n = U.shape[0]
+    nd = max(n - 1, 1)
+    gen_i = {"Richtmyer": 0, "Sobol": 1}[gen]
+    desc = _Desc(n, m, n_reps, gen_i, 2, 1, 1, 1, n_bits, threads)
+    q = np.ascontiguousarray(sqrt_primes(nd) if q is None else q, np.float64)
+    if gen_i == 0:
+        shifts_f = np.asfortranarray(shifts, np.float64)
+        shifts_u = np.zeros(1, np.uint64)
+        V = np.zeros((1, 32), np.uint32)
+    else:
+        shifts_u = np.asfortranarray(shifts, np.uint64)
+        shifts_f = np.zeros(1, np.float64)
+        V = np.ascontiguousarray(V, np.uint32)
+    if sobol_skip is None:
+        sobol_skip = sobol_skip_count(2 * m - 1)
+    ga, gb = np.zeros(n), np.zeros(n)
+    gU = np.zeros((n, n), order="F")
+    lp = C.c_double(0.0)
+    rc = lib().orc_qmc_grad_dense(C.byref(desc), _ptr(U, C.c_double), _ptr(a, C.c_double), _ptr(b, C.c_double),
+                                  _ptr(q, C.c_double), _ptr(shifts_f, C.c_double), _ptr(V, C.c_uint32),
+                                  _ptr(shifts_u, C.c_uint64), sobol_skip, _ptr(ga, C.c_double),
+                                  _ptr(gb, C.c_double), _ptr(gU, C.c_double), C.byref(lp))
+    assert rc == 0
+    return GradResult(ga, gb, gU, lp.value)
 
 
 def sobol_skip_count(n: int) -> int:

@@ -46,3 +46,52 @@ def random_spd(n, seed, jitter=100.0):
 def standardized(C):
     d = np.sqrt(np.diag(C))
     return C / np.outer(d, d)
+
+
+def chain_to_sigma(U, perm, Sigma, ga_std, gb_std, gU, a_std, b_std):
+    """Gradient of log P with respect to the original a, b and Sigma from the
+    gradient with respect to the standardized, permuted bounds and the factor
+    U (C_p = U^T U, permutation held fixed): the Cholesky adjoint
+    C_bar = U^{-1} Phi(U U_bar^T) U^{-T} (Phi: lower triangle, halved
+    diagonal; then symmetrized), the standardization C = D^{-1/2} Sigma D^{-1/2}
+    and a' = a / sqrt(Sigma_ii).  Returns (ga, gb, G) with d log P =
+    sum_ij G_ij E_ij for a symmetric perturbation E of Sigma."""
+    n = U.shape[0]
+    perm = np.asarray(perm)
+    s = np.sqrt(np.diag(Sigma))
+    sp = s[perm]
+    Ub = np.triu(gU)
+    X = U @ Ub.T
+    P = np.tril(X)
+    P[np.diag_indices(n)] *= 0.5
+    Z = np.linalg.solve(U, P)
+    Cb = np.linalg.solve(U, Z.T).T
+    Cb = 0.5 * (Cb + Cb.T)
+    Cp = U.T @ U
+    G = np.zeros((n, n))
+    G[np.ix_(perm, perm)] = Cb / np.outer(sp, sp)
+    for i in range(n):
+        k = perm[i]
+        G[k, k] -= np.sum(Cb[i, :] * Cp[i, :]) / Sigma[k, k]
+        ta = 0.0 if np.isinf(a_std[i]) else ga_std[i] * a_std[i]
+        tb = 0.0 if np.isinf(b_std[i]) else gb_std[i] * b_std[i]
+        G[k, k] -= (ta + tb) / (2 * Sigma[k, k])
+    ga = np.zeros(n)
+    gb = np.zeros(n)
+    ga[perm] = ga_std / sp
+    gb[perm] = gb_std / sp
+    return ga, gb, G
+
+
+def logp_fixed_perm(O, Sigma, a, b, perm, *, m, n_reps, shifts):
+    """log P (tail form) with the factor of the permuted, standardized Sigma
+    taken by a plain Cholesky in a fixed order: the function whose
+    derivative the adjoint returns."""
+    perm = np.asarray(perm)
+    s = np.sqrt(np.diag(Sigma))
+    Cp = (Sigma / np.outer(s, s))[np.ix_(perm, perm)]
+    U = np.linalg.cholesky(Cp).T
+    ap, bp = a[perm] / s[perm], b[perm] / s[perm]
+    n = Sigma.shape[0]
+    r = O.qmc_pnorm(U, ap, bp, m=m, n_reps=n_reps, shifts=shifts, phi_form="tail", bs_i=n - 1, bs_i2=n - 1)
+    return r.log_value, U, ap, bp

@@ -377,3 +377,80 @@ def test_tail_form_deep_lower_orthant(oracle):
     r2, _ = _run(oracle, P.Sigma, P.a, P.b, m=2 ** 12, R=4, seed=2, phi="tail")
     assert np.isfinite(r1.log_value) and np.isfinite(r2.log_value)
     assert abs(r1.log_value - r2.log_value) < 3 * (r1.rel_err3 + r2.rel_err3) + 0.05
+
+
+# --- adjoint (no reference counterpart) ------------------------------------------
+def test_grad_matches_central_differences(oracle):
+    """d log P / d(a, b, U) of the oracle's reverse sweep against central
+    differences of its own forward on the same points (h = 1e-5)."""
+    rng = np.random.default_rng(21)
+    n = 10
+    Gm = rng.normal(size=(n, n))
+    S = Gm @ Gm.T + n * np.eye(n)
+    a, b = -rng.uniform(0.3, 2, n), rng.uniform(0.3, 2, n)
+    a[3] = -np.inf
+    F = oracle.prepare_dense(S, a, b)
+    sh = rng.random((n - 1, 4))
+    g = oracle.qmc_grad(F.U, F.a, F.b, m=1024, n_reps=4, shifts=sh)
+
+    def lp(U, a_, b_):
+        return oracle.qmc_pnorm(U, a_, b_, m=1024, n_reps=4, shifts=sh, phi_form="tail",
+                                bs_i=n - 1, bs_i2=n - 1).log_value
+
+    assert abs(g.log_value - lp(F.U, F.a, F.b)) < 1e-12 * abs(g.log_value)
+    h = 1e-5
+    for i in range(n):
+        for which, vec, gv in (("b", F.b, g.gb), ("a", F.a, g.ga)):
+            if np.isinf(vec[i]):
+                assert gv[i] == 0.0
+                continue
+            vp, vm = vec.copy(), vec.copy()
+            vp[i] += h
+            vm[i] -= h
+            fd = (lp(F.U, *((F.a, vp) if which == "b" else (vp, F.b))) -
+                  lp(F.U, *((F.a, vm) if which == "b" else (vm, F.b)))) / (2 * h)
+            assert abs(fd - gv[i]) < 1e-6 * max(1.0, abs(fd)), (which, i, fd, gv[i])
+    for (l, t) in [(0, 1), (0, 9), (2, 5), (4, 4), (8, 9), (9, 9), (0, 0)]:
+        Up, Um = F.U.copy(), F.U.copy()
+        Up[l, t] += h
+        Um[l, t] -= h
+        fd = (lp(Up, F.a, F.b) - lp(Um, F.a, F.b)) / (2 * h)
+        assert abs(fd - g.gU[l, t]) < 1e-6 * max(1.0, abs(fd)), (l, t, fd, g.gU[l, t])
+
+
+def test_grad_sigma_chain_rule(oracle):
+    """The Cholesky / standardization / bound chain rule used by the tests
+    (tests._helpers.chain_to_sigma) against central differences in Sigma,
+    a and b with the pivot order held fixed."""
+    from tests._helpers import chain_to_sigma, logp_fixed_perm
+
+    rng = np.random.default_rng(22)
+    n = 8
+    Gm = rng.normal(size=(n, n))
+    S = Gm @ Gm.T + n * np.eye(n)
+    a, b = -rng.uniform(0.5, 2, n), rng.uniform(0.5, 2, n)
+    a[2] = -np.inf
+    perm = oracle.prepare_dense(S, a, b).perm
+    sh = rng.random((n - 1, 4))
+    kw = dict(m=2048, n_reps=4, shifts=sh)
+    lp0, U, ap, bp = logp_fixed_perm(oracle, S, a, b, perm, **kw)
+    g = oracle.qmc_grad(U, ap, bp, m=2048, n_reps=4, shifts=sh)
+    ga, gb, G = chain_to_sigma(U, perm, S, g.ga, g.gb, g.gU, ap, bp)
+    h = 1e-5
+    for (i, j) in [(0, 0), (1, 3), (2, 5), (7, 7), (4, 6)]:
+        E = np.zeros((n, n))
+        E[i, j] += h
+        if i != j:
+            E[j, i] += h
+        fd = (logp_fixed_perm(oracle, S + E, a, b, perm, **kw)[0] -
+              logp_fixed_perm(oracle, S - E, a, b, perm, **kw)[0]) / (2 * h)
+        an = G[i, j] * (1 if i == j else 2)
+        assert abs(fd - an) < 1e-6 * max(1.0, abs(fd)), (i, j, fd, an)
+    for i in (0, 3, 6):
+        e = np.zeros(n)
+        e[i] = h
+        fdb = (logp_fixed_perm(oracle, S, a, b + e, perm, **kw)[0] -
+               logp_fixed_perm(oracle, S, a, b - e, perm, **kw)[0]) / (2 * h)
+        fda = (logp_fixed_perm(oracle, S, a + e, b, perm, **kw)[0] -
+               logp_fixed_perm(oracle, S, a - e, b, perm, **kw)[0]) / (2 * h)
+        assert abs(fdb - gb[i]) < 1e-6 and abs(fda - ga[i]) < 1e-6
