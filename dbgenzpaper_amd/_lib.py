@@ -30,7 +30,7 @@ EXPORTS = (
     "dbg_plan_set_sobol_directions", "dbg_qmc_pnorm", "dbg_qmc_loop", "dbg_plan_sync",
     "dbg_plan_last_kernel_ms", "dbg_plan_summary", "dbg_plan_get_factor", "dbg_plan_get_reps",
     "dbg_plan_get_state", "dbg_plan_set_state", "dbg_plan_refactor", "dbg_batch_create",
-    "dbg_batch_run", "dbg_batch_last_ms", "dbg_batch_tuning", "dbg_batch_destroy",
+    "dbg_batch_run", "dbg_batch_last_ms", "dbg_batch_tuning", "dbg_batch_destroy", "dbg_qmc_grad",
 )
 
 

@@ -1,0 +1,203 @@
+// Helper kernels of the adjoint (gradient of log P; no reference
+// counterpart).  The sweep itself runs in the matrix-core integration kernel
+// (record and reverse modes); these kernels prepare its inputs and turn its
+// outputs into the gradient with respect to the original a, b and Sigma:
+//
+//   pack_flip      the flipped factor Uf[g1, g2] = U[n-1-g2, n-1-g1] in the
+//                  packed-row and column-major layouts of the dense kernel,
+//                  so the reverse recursion ybar_l = sum_{r>l} U[l, r] sbar_r
+//                  is the forward contraction in flipped index order;
+//   grad_norm      E = max exponent of the tile sums, 1 / sum(p 2^-E) and
+//                  log P of the recorded forward;
+//   acc_upper      Ubar[l, g+1] += C[l, g] (l <= g) from the chunked
+//                  Y^T Sbar products;
+//   tril_half      Phi(X): lower triangle, halved diagonal (Cholesky adjoint);
+//   sigma_map      C_bar -> G (symmetrized, unpermuted, unstandardized) and
+//                  the gradients of the original bounds;
+//   sigma_diag     the diagonal terms from the standardization and the bounds.
+#include <algorithm>
+
+#include "dbg_device.h"
+
+namespace dbg {
+
+template <typename T>
+__global__ void pack_flip_kernel(const double* __restrict__ U, int n, T* __restrict__ Ufk, int64_t ldu,
+                                 T* __restrict__ Ufc)
+{
+    const int g1 = blockIdx.y;  // 0 .. n-1
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < ldu; g += (int64_t)gridDim.x * blockDim.x) {
+        const bool in = g < n;
+        const double v = in ? U[(size_t)(n - 1 - g) + (size_t)(n - 1 - g1) * n] : 0.0;
+        if (g1 < n - 1) Ufk[(size_t)g1 * ldu + g] = (T)((g > g1 && in) ? v : 0.0);
+        if (in) Ufc[(size_t)g * n + g1] = (T)(g1 <= g ? v : 0.0);
+    }
+}
+
+hipError_t pack_flip(const double* U, int n, bool mixed, void* Ufk, int64_t ldu, void* Ufc, hipStream_t s)
+{
+    dim3 g((unsigned)std::min<int64_t>((ldu + 255) / 256, 64), n), b(256);
+    if (mixed)
+        hipLaunchKernelGGL(pack_flip_kernel<float>, g, b, 0, s, U, n, (float*)Ufk, ldu, (float*)Ufc);
+    else
+        hipLaunchKernelGGL(pack_flip_kernel<double>, g, b, 0, s, U, n, (double*)Ufk, ldu, (double*)Ufc);
+    return hipGetLastError();
+}
+
+// norm[0] = E, norm[1] = 1 / S with S = sum_t part_m[t] 2^(part_e[t] - E);
+// norm[2] = log P = log(S / total) + E log 2.  One block.
+__global__ __launch_bounds__(256) void grad_norm_kernel(const double* __restrict__ pm, const int* __restrict__ pe,
+                                                        int64_t n_tiles, double total, double* norm)
+{
+    __shared__ int s_e[4];
+    __shared__ double s_v[256], s_c[256];
+    const int tid = threadIdx.x;
+    int e = INT32_MIN;
+    for (int64_t t = tid; t < n_tiles; t += 256)
+        if (pm[t] != 0.0) e = max(e, pe[t]);
+    e = wave_max_int(e);
+    if ((tid & 63) == 0) s_e[tid >> 6] = e;
+    __syncthreads();
+    int E = max(max(s_e[0], s_e[1]), max(s_e[2], s_e[3]));
+    if (E == INT32_MIN) E = 0;
+    double sum = 0.0, comp = 0.0;
+    for (int64_t t = tid; t < n_tiles; t += 256) {
+        const double v = pm[t] != 0.0 ? ldexp(pm[t], pe[t] - E) : 0.0;
+        const double tt = sum + v;
+        comp += (fabs(sum) >= fabs(v)) ? (sum - tt) + v : (v - tt) + sum;
+        sum = tt;
+    }
+    s_v[tid] = sum;
+    s_c[tid] = comp;
+    __syncthreads();
+    if (tid == 0) {
+        double S = 0.0, C = 0.0;
+        for (int q = 0; q < 256; ++q) {
+            const double v = s_v[q];
+            const double tt = S + v;
+            C += (fabs(S) >= fabs(v)) ? (S - tt) + v : (v - tt) + S;
+            S = tt;
+            C += s_c[q];
+        }
+        S += C;
+        norm[0] = (double)E;
+        norm[1] = S > 0.0 ? 1.0 / S : 0.0;
+        norm[2] = S > 0.0 ? log(S / total) + E * 0.69314718055994530942 : -INFINITY;
+    }
+}
+
+hipError_t grad_norm(const double* pm, const int* pe, int64_t n_tiles, double total, double* norm, hipStream_t s)
+{
+    hipLaunchKernelGGL(grad_norm_kernel, dim3(1), dim3(256), 0, s, pm, pe, n_tiles, total, norm);
+    return hipGetLastError();
+}
+
+// Ubar (n x n, column-major) += C (nd x nd) shifted: Ubar[l, g + 1] += C[l, g] for l <= g
+template <typename T>
+__global__ void acc_upper_kernel(const T* __restrict__ C, int nd, double* __restrict__ Ub, int n)
+{
+    const int g = blockIdx.y;
+    for (int l = blockIdx.x * blockDim.x + threadIdx.x; l <= g; l += gridDim.x * blockDim.x)
+        Ub[(size_t)l + (size_t)(g + 1) * n] += (double)C[(size_t)l + (size_t)g * nd];
+}
+
+hipError_t acc_upper(const void* C, bool mixed, int nd, double* Ub, int n, hipStream_t s)
+{
+    dim3 g((nd + 255) / 256, nd), b(256);
+    if (mixed)
+        hipLaunchKernelGGL(acc_upper_kernel<float>, g, b, 0, s, (const float*)C, nd, Ub, n);
+    else
+        hipLaunchKernelGGL(acc_upper_kernel<double>, g, b, 0, s, (const double*)C, nd, Ub, n);
+    return hipGetLastError();
+}
+
+// diagonal of Ubar from the reverse sweep's accumulator
+__global__ void set_diag_kernel(double* Ub, int n, const double* __restrict__ d)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) Ub[(size_t)i + (size_t)i * n] += d[i];
+}
+
+// X <- Phi(X): strict upper part zeroed, diagonal halved
+__global__ void tril_half_kernel(double* X, int n)
+{
+    const int j = blockIdx.y;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double& v = X[(size_t)i + (size_t)j * n];
+        if (i < j) v = 0.0;
+        else if (i == j) v *= 0.5;
+    }
+}
+
+// G[p_i, p_j] = (Cb[i, j] + Cb[j, i]) / 2 / (s_{p_i} s_{p_j})
+__global__ void sigma_map_kernel(const double* __restrict__ Cb, int n, const int32_t* __restrict__ perm,
+                                 const double* __restrict__ scale, double* __restrict__ G)
+{
+    const int j = blockIdx.y;
+    const int pj = perm[j];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int pi = perm[i];
+        const double c = 0.5 * (Cb[(size_t)i + (size_t)j * n] + Cb[(size_t)j + (size_t)i * n]);
+        G[(size_t)pi + (size_t)pj * n] = c / (scale[pi] * scale[pj]);
+    }
+}
+
+// One block per pivot position i (k = perm[i]):
+//   G[k, k] -= sum_j Cs[i, j] C_p[i, j] / Sigma_kk + (ga'_i a'_i + gb'_i b'_i) / (2 Sigma_kk)
+// with Cs the symmetrized C_bar and C_p[i, j] = Sigma[p_i, p_j] / (s_{p_i} s_{p_j});
+// ga[k] = ga'_i / s_k, gb[k] = gb'_i / s_k.
+__global__ __launch_bounds__(256) void sigma_diag_kernel(const double* __restrict__ Cb, int n,
+                                                         const int32_t* __restrict__ perm,
+                                                         const double* __restrict__ scale,
+                                                         const double* __restrict__ Sig,
+                                                         const double* __restrict__ a_std,
+                                                         const double* __restrict__ b_std,
+                                                         const double* __restrict__ ga_std,
+                                                         const double* __restrict__ gb_std,
+                                                         double* __restrict__ G, double* __restrict__ ga,
+                                                         double* __restrict__ gb)
+{
+    __shared__ double s_r[4];
+    const int i = blockIdx.x;
+    const int k = perm[i];
+    const int tid = threadIdx.x;
+    double acc = 0.0;
+    for (int j = tid; j < n; j += 256) {
+        const int pj = perm[j];
+        const double c = 0.5 * (Cb[(size_t)i + (size_t)j * n] + Cb[(size_t)j + (size_t)i * n]);
+        const double cp = Sig[(size_t)k + (size_t)pj * n] / (scale[k] * scale[pj]);
+        acc += c * cp;
+    }
+    acc = wave_sum_f64(acc);
+    if ((tid & 63) == 0) s_r[tid >> 6] = acc;
+    __syncthreads();
+    if (tid == 0) {
+        const double row = s_r[0] + s_r[1] + s_r[2] + s_r[3];
+        const double skk = Sig[(size_t)k + (size_t)k * n];
+        const double ta = isinf(a_std[i]) ? 0.0 : ga_std[i] * a_std[i];
+        const double tb = isinf(b_std[i]) ? 0.0 : gb_std[i] * b_std[i];
+        G[(size_t)k + (size_t)k * n] -= row / skk + (ta + tb) / (2.0 * skk);
+        ga[k] = ga_std[i] / scale[k];
+        gb[k] = gb_std[i] / scale[k];
+    }
+}
+
+hipError_t sigma_chain(double* Ub, int n, const double* gd, double* X, hipStream_t s, int step,
+                       const double* Cb, const int32_t* perm, const double* scale, const double* Sig,
+                       const double* a_std, const double* b_std, const double* ga_std, const double* gb_std,
+                       double* G, double* ga, double* gb)
+{
+    dim3 g2((n + 255) / 256, n), b(256);
+    if (step == 0) {
+        hipLaunchKernelGGL(set_diag_kernel, dim3((n + 255) / 256), b, 0, s, Ub, n, gd);
+    } else if (step == 1) {
+        hipLaunchKernelGGL(tril_half_kernel, g2, b, 0, s, X, n);
+    } else {
+        hipLaunchKernelGGL(sigma_map_kernel, g2, b, 0, s, Cb, n, perm, scale, G);
+        hipLaunchKernelGGL(sigma_diag_kernel, dim3(n), b, 0, s, Cb, n, perm, scale, Sig, a_std, b_std, ga_std,
+                           gb_std, G, ga, gb);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace dbg

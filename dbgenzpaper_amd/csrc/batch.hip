@@ -85,7 +85,7 @@ __global__ __launch_bounds__(kTileB) void batch_integrate_kernel(BatchArgs B, co
 #pragma unroll
             for (int c = 0; c < W; ++c) s_x[tid * SXS + c] = s[c];
             walk_block<MIXED, kGenRichtmyer, kPhiErf, kTileB, SXS, W>(pts, g0, min(W, nd - g0), r, j, tid, st,
-                                                                     w, s_x, s_y, Ys, Ucol, d, ak, bk, invk);
+                                                                     w, s_x, s_y, Ys, kTileB, Ucol, d, ak, bk, invk);
         }
         const double pm = valid ? (double)w.pm : 0.0;
         double S;

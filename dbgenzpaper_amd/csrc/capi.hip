@@ -8,6 +8,8 @@
 #include <algorithm>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+
 #include "../../include/dbgenz.h"
 #include "dbg_internal.h"
 
@@ -239,9 +241,36 @@ struct dbg_plan {
     double* rep_m = nullptr;
     int* rep_e = nullptr;
     double* out = nullptr;
+    // adjoint buffers (allocated on the first dbg_qmc_grad)
+    struct Grad {
+        bool ready = false;
+        int tile = 0, grid = 0;
+        int64_t tiles_per_rep = 0, n_tiles = 0, rs = 0;
+        void* yrec = nullptr;   // nd x rs (T): y of every sample
+        void* srec = nullptr;   // nd x rs (T): s_{g+1}, then sbar_{g+1}
+        void* wm = nullptr;     // rs (T)
+        int* we = nullptr;      // rs
+        void* ywork = nullptr;  // grid x nd x tile (T): reverse workspace
+        double* part_m = nullptr;
+        int* part_e = nullptr;
+        double* norm = nullptr; // E, 1/S, log P
+        double* acc = nullptr;  // 3 n: d log P / d a', b', U_rr
+        void* ufk = nullptr;    // flipped factor, packed rows (T)
+        void* ufc = nullptr;    // flipped factor, column-major (T)
+        void* ctmp = nullptr;   // nd x nd (T)
+        double* ub = nullptr;   // n x n: d log P / d U
+        double* x = nullptr;    // n x n work
+        double* g = nullptr;    // n x n: d log P / d Sigma
+        double* gab = nullptr;  // 2 n: d log P / d a, b
+        std::vector<void*> all() const
+        {
+            return {yrec, srec, wm, we, ywork, part_m, part_e, norm, acc, ufk, ufc, ctmp, ub, x, g, gab};
+        }
+    } G;
 
     void release()
     {
+        for (void* p : G.all()) (void)hipFree(p);
         F.release();
         void* ptrs[] = {sig, a0, b0, Uk, ucol32, ak, bk, invk, first, colcnt, colptr, rowval, nzval, q, shifts_f,
                         q64, shift32, shifts_u, V, Y, part_m, part_e, rep_m, rep_e, out};
@@ -448,6 +477,141 @@ int run_loop(dbg_plan* P, int64_t block, int64_t n_pts0)
     return DBG_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint (no reference counterpart): forward with records, weights, reverse
+// sweep, Ubar = Y^T Sbar by rocBLAS in sample chunks with FP64 accumulation,
+// the Cholesky adjoint C_bar = U^{-1} Phi(U Ubar^T) U^{-T} (rocBLAS dgemm,
+// dtrsm) and the map to the original a, b, Sigma.
+// ---------------------------------------------------------------------------
+int grad_alloc(dbg_plan* P)
+{
+    auto& G = P->G;
+    if (G.ready) return DBG_OK;
+    const int n = P->n, nd = P->nd;
+    const size_t T = P->tsz;
+    int dev = 0, ncu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    G.tile = dense_mfma_tile(P->mixed);
+    G.tiles_per_rep = (P->o.m + G.tile - 1) / G.tile;
+    G.n_tiles = G.tiles_per_rep * P->o.n_reps;
+    G.rs = G.n_tiles * G.tile;
+    G.grid = (int)std::min<int64_t>(G.n_tiles, (int64_t)ncu * 2);
+    const size_t rec = (size_t)nd * G.rs * T;
+    HIPCHK(dalloc(&G.yrec, rec));
+    HIPCHK(dalloc(&G.srec, rec));
+    HIPCHK(dalloc(&G.wm, (size_t)G.rs * T));
+    HIPCHK(dalloc(&G.we, (size_t)G.rs * sizeof(int)));
+    HIPCHK(dalloc(&G.ywork, (size_t)G.grid * nd * G.tile * T));
+    HIPCHK(dalloc(&G.part_m, (size_t)G.n_tiles * sizeof(double)));
+    HIPCHK(dalloc(&G.part_e, (size_t)G.n_tiles * sizeof(int)));
+    HIPCHK(dalloc(&G.norm, 4 * sizeof(double)));
+    HIPCHK(dalloc(&G.acc, (size_t)3 * n * sizeof(double)));
+    HIPCHK(dalloc(&G.ufk, (size_t)nd * P->ldu * T));
+    HIPCHK(dalloc(&G.ufc, (size_t)n * n * T + 256 * T));
+    HIPCHK(dalloc(&G.ctmp, (size_t)nd * nd * T));
+    HIPCHK(dalloc(&G.ub, (size_t)n * n * sizeof(double)));
+    HIPCHK(dalloc(&G.x, (size_t)n * n * sizeof(double)));
+    HIPCHK(dalloc(&G.g, (size_t)n * n * sizeof(double)));
+    HIPCHK(dalloc(&G.gab, (size_t)2 * n * sizeof(double)));
+    G.ready = true;
+    return DBG_OK;
+}
+
+#define BLASCHK(expr)                                                                          \
+    do {                                                                                       \
+        rocblas_status st_ = (expr);                                                           \
+        if (st_ != rocblas_status_success)                                                     \
+            return fail(DBG_ERR_DEVICE, std::string("rocBLAS: ") + rocblas_status_to_string(st_)); \
+    } while (0)
+
+int run_grad(dbg_plan* P)
+{
+    auto& G = P->G;
+    const dbg_opts& o = P->o;
+    const int n = P->n, nd = P->nd;
+    hipStream_t s = P->stream;
+    const int gen = o.generator;
+    HIPCHK(hipMemsetAsync(G.acc, 0, (size_t)3 * n * sizeof(double), s));
+    HIPCHK(hipMemsetAsync(G.ub, 0, (size_t)n * n * sizeof(double), s));
+    HIPCHK(pack_flip(P->F.M, n, P->mixed, G.ufk, P->ldu, G.ufc, s));
+
+    DenseArgs A{};
+    A.n = n;
+    A.m = o.m;
+    A.n_reps = o.n_reps;
+    A.tiles_per_rep = G.tiles_per_rep;
+    A.n_tiles = G.n_tiles;
+    A.ldu = P->ldu;
+    A.ldc = n;
+    A.ak = P->ak;
+    A.bk = P->bk;
+    A.invk = P->invk;
+    for (int q = 0; q < 6; ++q) A.first[q] = P->first_h[q];
+    A.pts = point_args(P, P->state);
+    A.y_slot = (int64_t)nd * G.tile;
+    A.part_m = G.part_m;
+    A.part_e = G.part_e;
+    A.rs = G.rs;
+    A.srec = G.srec;
+    A.yrec = G.yrec;
+    A.wm = G.wm;
+    A.we = G.we;
+    A.norm = G.norm;
+    A.ga = G.acc;
+    A.gb = G.acc + n;
+    A.gd = G.acc + 2 * n;
+    // forward with records (tail form)
+    A.mode = kModeRecord;
+    A.Uk = P->Uk;
+    A.Ucol = P->mixed ? (const void*)P->ucol32 : (const void*)P->F.M;
+    A.Y = G.yrec;
+    HIPCHK(launch_dense_adjoint(A, P->mixed, gen, G.grid, s));
+    HIPCHK(grad_norm(G.part_m, G.part_e, G.n_tiles, (double)o.m * o.n_reps, G.norm, s));
+    // reverse sweep in flipped index order
+    A.mode = kModeReverse;
+    A.Uk = G.ufk;
+    A.Ucol = G.ufc;
+    A.Y = G.ywork;
+    HIPCHK(launch_dense_adjoint(A, P->mixed, gen, G.grid, s));
+    // Ubar[l, g + 1] = sum_samples y_l sbar_{g+1}: chunks of samples, FP64 accumulation
+    rocblas_handle h = (rocblas_handle)blas_handle_void(s);
+    if (!h) return fail(DBG_ERR_DEVICE, "rocBLAS handle");
+    const int64_t chunk = (int64_t)G.tile * 256;
+    for (int64_t c0 = 0; c0 < G.rs; c0 += chunk) {
+        const int kk = (int)std::min<int64_t>(chunk, G.rs - c0);
+        if (P->mixed) {
+            const float one = 1.0f, zero = 0.0f;
+            BLASCHK(rocblas_sgemm(h, rocblas_operation_transpose, rocblas_operation_none, nd, nd, kk, &one,
+                                  (const float*)G.yrec + c0, (rocblas_int)G.rs, (const float*)G.srec + c0,
+                                  (rocblas_int)G.rs, &zero, (float*)G.ctmp, nd));
+        } else {
+            const double one = 1.0, zero = 0.0;
+            BLASCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nd, nd, kk, &one,
+                                  (const double*)G.yrec + c0, (rocblas_int)G.rs, (const double*)G.srec + c0,
+                                  (rocblas_int)G.rs, &zero, (double*)G.ctmp, nd));
+        }
+        HIPCHK(acc_upper(G.ctmp, P->mixed, nd, G.ub, n, s));
+    }
+    HIPCHK(sigma_chain(G.ub, n, G.acc + 2 * n, nullptr, s, 0, nullptr, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
+    // Cholesky adjoint: X = U Ubar^T, Phi(X), U^{-1} X, then X U^{-T}
+    {
+        const double one = 1.0, zero = 0.0;
+        BLASCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, n, n, n, &one, P->F.M, n,
+                              G.ub, n, &zero, G.x, n));
+        HIPCHK(sigma_chain(nullptr, n, nullptr, G.x, s, 1, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                           nullptr, nullptr, nullptr, nullptr, nullptr));
+        BLASCHK(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                              rocblas_diagonal_non_unit, n, n, &one, P->F.M, n, G.x, n));
+        BLASCHK(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_transpose,
+                              rocblas_diagonal_non_unit, n, n, &one, P->F.M, n, G.x, n));
+    }
+    HIPCHK(sigma_chain(nullptr, n, nullptr, nullptr, s, 2, G.x, P->F.perm, P->F.scale, P->sig, P->F.a, P->F.b,
+                       G.acc, G.acc + n, G.g, G.gab, G.gab + n));
+    return DBG_OK;
+}
+
 int read_summary(dbg_plan* P, dbg_result* r)
 {
     double h[8];
@@ -626,6 +790,38 @@ int dbg_plan_last_kernel_ms(dbg_plan* P, float* ms)
     if (!P || !ms) return fail(DBG_ERR_ARG, "null argument");
     HIPCHK(hipEventSynchronize(P->ev1));
     HIPCHK(hipEventElapsedTime(ms, P->ev0, P->ev1));
+    return DBG_OK;
+}
+
+int dbg_qmc_grad(dbg_plan* P, double* log_value, double* ga, double* gb, double* gSigma, double* ga_std,
+                 double* gb_std, double* gU)
+{
+    g_err.clear();
+    if (!P) return fail(DBG_ERR_ARG, "null plan");
+    const dbg_opts& o = P->o;
+    const int n = P->n;
+    if (o.kind != DBG_KIND_DENSE) return fail(DBG_ERR_ARG, "the gradient is defined for the dense (QMCData) plan");
+    if (n < 2) return fail(DBG_ERR_ARG, "the gradient needs n >= 2");
+    if (P->nd > 0 && !P->shifts_set) return fail(DBG_ERR_STATE, "shifts not set");
+    if (o.generator == DBG_GEN_SOBOL && !P->dirs_set) return fail(DBG_ERR_STATE, "Sobol direction integers not set");
+    int rc = grad_alloc(P);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(P->ev0, P->stream));
+    rc = run_grad(P);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(P->ev1, P->stream));
+    auto& G = P->G;
+    double norm[4];
+    HIPCHK(hipMemcpyAsync(norm, G.norm, 3 * sizeof(double), hipMemcpyDeviceToHost, P->stream));
+    const size_t nn = (size_t)n * n * sizeof(double);
+    if (ga) HIPCHK(hipMemcpyAsync(ga, G.gab, n * sizeof(double), hipMemcpyDeviceToHost, P->stream));
+    if (gb) HIPCHK(hipMemcpyAsync(gb, G.gab + n, n * sizeof(double), hipMemcpyDeviceToHost, P->stream));
+    if (gSigma) HIPCHK(hipMemcpyAsync(gSigma, G.g, nn, hipMemcpyDeviceToHost, P->stream));
+    if (ga_std) HIPCHK(hipMemcpyAsync(ga_std, G.acc, n * sizeof(double), hipMemcpyDeviceToHost, P->stream));
+    if (gb_std) HIPCHK(hipMemcpyAsync(gb_std, G.acc + n, n * sizeof(double), hipMemcpyDeviceToHost, P->stream));
+    if (gU) HIPCHK(hipMemcpyAsync(gU, G.ub, nn, hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    if (log_value) *log_value = norm[2];
     return DBG_OK;
 }
 

@@ -293,6 +293,8 @@ __global__ __launch_bounds__(kCholThreads) void chol_window_kernel(CholArgs C, i
 }
 
 hipError_t chol_syrk(const CholArgs& C, int k0, int k1, int r0, int r1, hipStream_t s);
+rocblas_handle blas_handle(hipStream_t s);
+void* blas_handle_void(hipStream_t s) { return (void*)blas_handle(s); }
 
 hipError_t chol_window(const CholArgs& C, int w0, int w1, int nb, bool pivot, hipStream_t s)
 {
@@ -313,7 +315,8 @@ hipError_t chol_window(const CholArgs& C, int w0, int w1, int nb, bool pivot, hi
 
 // One rocBLAS handle per host thread and device, created on first use and
 // kept until the process ends; its stream is set before every call.
-static rocblas_handle blas_handle(hipStream_t s)
+// (Declared for the other translation units as blas_handle_void.)
+rocblas_handle blas_handle(hipStream_t s)
 {
     constexpr int kMaxDev = 64;
     thread_local rocblas_handle handles[kMaxDev] = {};

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "dbg_internal.h"
 #include "tailq.h"
 
@@ -434,13 +436,15 @@ __device__ __forceinline__ typename Path<MIXED, FORM>::T path_draw(const PointAr
 // (Ucol[j * ldc + l] = U[l, j]), read through the scalar cache since every
 // lane needs the same column.
 // ---------------------------------------------------------------------------
+// ys: row stride of the y workspace Ys; srec (optional): the conditional
+// means s_{g+1} are also recorded there (same stride), for the adjoint.
 template <bool MIXED, int GEN, int FORM, int TS, int SXS, int SB, typename T, typename State>
 __device__ __forceinline__ void walk_block(const PointArgs& pts, int gb, int cnt, int k, int64_t j,
                                            int t, State& st, Weight<T>& w, const T* __restrict__ s_x,
-                                           T* __restrict__ s_y, T* __restrict__ Ys,
+                                           T* __restrict__ s_y, T* __restrict__ Ys, int64_t ys,
                                            const T* __restrict__ Ucol, int64_t ldc,
                                            const T* __restrict__ ak, const T* __restrict__ bk,
-                                           const T* __restrict__ invk)
+                                           const T* __restrict__ invk, T* __restrict__ srec = nullptr)
 {
     using PT = Path<MIXED, FORM>;
     T yv[SB];
@@ -451,13 +455,131 @@ __device__ __forceinline__ void walk_block(const PointArgs& pts, int gb, int cnt
             const T y = path_draw<MIXED, GEN, FORM>(pts, st, g, k, j);
             yv[cc] = y;
             s_y[cc * TS + t] = y;
-            Ys[(size_t)g * TS + t] = y;
+            Ys[(size_t)g * ys + t] = y;
             T acc = s_x[t * SXS + cc];
             const T* __restrict__ ucol = Ucol + (size_t)(g + 1) * ldc + gb;
 #pragma unroll
             for (int l = 0; l <= cc; ++l) acc = fma(yv[l], ucol[l], acc);
+            if (srec) srec[(size_t)g * ys + t] = acc;
             PT::phi(st, acc, ak[g + 1], bk[g + 1], invk[g + 1]);
             PT::weigh(w, st);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reverse sweep (adjoint of the tail-form estimate; no reference counterpart).
+//
+// For one sample, with alpha_r = (a_r - s_r)/U_rr, beta_r = (b_r - s_r)/U_rr,
+// dc_r = Phi(beta_r) - Phi(alpha_r), y_r = Phi^{-1}(Phi(alpha_r) + w_r dc_r)
+// and s_r = sum_{l<r} U[l, r] y_l, the reverse step of index r (given the
+// complete ybar_r = sum_{r'>r} U[r, r'] sbar_r') is
+//   dcbar = wt / dc_r + ybar_r w_r / phi(y_r),   cbar = ybar_r / phi(y_r)
+//   albar = (cbar - dcbar) phi(alpha_r),         bebar = dcbar phi(beta_r)
+//   abar_r = albar / U_rr, bbar_r = bebar / U_rr,
+//   Ubar_rr = -(albar alpha_r + bebar beta_r) / U_rr,
+//   sbar_r = -(albar + bebar) / U_rr,
+// with wt the sample's normalised weight p / sum(p), so that the sums over
+// samples are d log P directly.  The masses and densities carry the state's
+// binary scale 2^k (FP32), which cancels in every product above.  The terms
+// of y_r are absent for r = d - 1, and s_0 = 0.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum_f64(double v);
+
+template <typename T>
+__device__ __forceinline__ T scaled_pdf(T x, int k)
+{
+    if (isinf(x)) return T(0);
+    if constexpr (std::is_same<T, float>::value)
+        return expf(fmaf(-0.5f * x, x, (float)k * 0.693147180559945309f)) * 0.398942280401432678f;
+    else
+        return exp(-0.5 * x * x + k * 0.69314718055994530942) * 0.39894228040143267794;
+}
+
+struct RevOut {
+    double ga, gb, gd;
+};
+
+template <bool MIXED, int GEN, typename T = typename std::conditional<MIXED, float, double>::type>
+__device__ __forceinline__ T rev_step(const PointArgs& pts, int n, int r, int k, int64_t j, T ybar, T wt,
+                                      T s_r, T y_r, T a_r, T b_r, T inv_r, RevOut& o)
+{
+    const T za = (a_r - s_r) * inv_r, zb = (b_r - s_r) * inv_r;
+    typename Path<MIXED, kPhiTail>::State st;
+    tail_from_z(st, za, zb);
+    const T sq2 = T(1.4142135623730951);
+    const T al = za * sq2, be = zb * sq2;
+    T dcb = (wt > T(0) && st.dc > T(0)) ? wt / st.dc : T(0);
+    T cb = T(0);
+    if (r <= n - 2) {
+        T w;
+        if constexpr (MIXED) {
+            float wf, wc;
+            tent_w(point_u32<GEN>(pts, r, k, j), wf, wc);
+            w = wf;
+        } else {
+            w = fabs(fma(point_u<GEN>(pts, r, k, j), 2.0, -1.0));
+        }
+        const T ph = scaled_pdf(y_r, st.k);
+        const T iy = ph > T(0) ? ybar / ph : T(0);
+        cb = iy;
+        dcb += iy * w;
+    }
+    const T albar = (cb - dcb) * scaled_pdf(al, st.k);
+    const T bebar = dcb * scaled_pdf(be, st.k);
+    const T invU = inv_r * sq2;
+    o.ga = (double)(albar * invU);
+    o.gb = (double)(bebar * invU);
+    const T ta = isinf(al) ? T(0) : albar * al;
+    const T tb = isinf(be) ? T(0) : bebar * be;
+    o.gd = (double)(-(ta + tb) * invU);
+    return -(albar + bebar) * invU;
+}
+
+// Reverse walk of flipped indices [gb, gb + cnt) (flipped index g is r =
+// n - 1 - g).  On entry s_x[t * SXS + c] holds the part of ybar for flipped
+// index gb + c accumulated from every flipped index before gb; Uf is the
+// flipped factor Uf[g1, g2] = U[n-1-g2, n-1-g1] (column-major, ldc), so the
+// in-block part is the same ascending dot as in the forward walk.  sbar goes
+// to the flipped workspace (s_y, Ys) for the contraction and to the record
+// srec[(r - 1) * rs + t] in place of s_r for the gradient of U.  Per index,
+// the three sums over the workgroup's samples are left in red[wave][cc][3].
+template <bool MIXED, int GEN, int TS, int SXS, int SB, typename T>
+__device__ __forceinline__ void walk_block_rev(const PointArgs& pts, int n, int gb, int cnt, int k, int64_t j,
+                                               int t, T& ybar, T wt, const T* __restrict__ s_x,
+                                               T* __restrict__ s_y, T* __restrict__ Ys,
+                                               const T* __restrict__ Uf, int64_t ldc,
+                                               const T* __restrict__ ak, const T* __restrict__ bk,
+                                               const T* __restrict__ invk, T* __restrict__ srec,
+                                               const T* __restrict__ yrec, int64_t rs,
+                                               double* __restrict__ red)
+{
+    T sv[SB];
+#pragma unroll
+    for (int cc = 0; cc < SB; ++cc) {
+        if (cc < cnt) {  // uniform
+            const int g = gb + cc;
+            const int r = n - 1 - g;
+            const T s_r = srec[(size_t)(r - 1) * rs + t];   // r >= 1 here
+            const T y_r = (r <= n - 2) ? yrec[(size_t)r * rs + t] : T(0);
+            RevOut o;
+            const T sbar = rev_step<MIXED, GEN>(pts, n, r, k, j, ybar, wt, s_r, y_r, ak[r], bk[r], invk[r], o);
+            sv[cc] = sbar;
+            s_y[cc * TS + t] = sbar;
+            Ys[(size_t)g * TS + t] = sbar;
+            srec[(size_t)(r - 1) * rs + t] = sbar;
+            const double a0 = wave_sum_f64(o.ga), b0 = wave_sum_f64(o.gb), d0 = wave_sum_f64(o.gd);
+            if ((t & 63) == 0) {
+                double* rr = red + ((t >> 6) * SB + cc) * 3;
+                rr[0] = a0;
+                rr[1] = b0;
+                rr[2] = d0;
+            }
+            T acc = s_x[t * SXS + cc];
+            const T* __restrict__ ucol = Uf + (size_t)(g + 1) * ldc + gb;
+#pragma unroll
+            for (int l = 0; l <= cc; ++l) acc = fma(sv[l], ucol[l], acc);
+            ybar = acc;
         }
     }
 }

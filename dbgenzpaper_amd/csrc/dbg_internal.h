@@ -45,7 +45,19 @@ struct DenseArgs {
     int64_t y_slot;            // elements per slot
     double* part_m;            // per-tile partial sums (scaled)
     int* part_e;
+    // adjoint support (matrix-core kernel only)
+    int mode;                  // kModeIntegrate, kModeRecord or kModeReverse
+    int64_t rs;                // row stride of the records: n_tiles * tile samples
+    void* srec;                // record: s_{g+1} per (g, sample); reverse: replaced by sbar_{g+1}
+    void* yrec;                // record: y_g per (g, sample) (the workspace itself in record mode)
+    void* wm;                  // record: per-sample weight mantissa (T)
+    int* we;                   // record: per-sample weight exponent
+    const double* norm;        // reverse: {E, 1 / sum(p 2^-E)}
+    double* ga;                // reverse: d log P / d a', b', U_rr accumulators (n each)
+    double* gb;
+    double* gd;
 };
+enum { kModeIntegrate = 0, kModeRecord = 1, kModeReverse = 2 };
 
 // Arguments of the sparse (CSC) integration kernel.
 struct SparseArgs {
@@ -76,6 +88,18 @@ int dense_valu_tile();
 int dense_valu_width();
 hipError_t launch_dense_mfma(const DenseArgs& A, bool mixed, int gen, int form, int grid,
                              hipStream_t s);
+// record (forward keeping y, s and the weights) / reverse sweep, tail form
+hipError_t launch_dense_adjoint(const DenseArgs& A, bool mixed, int gen, int grid, hipStream_t s);
+hipError_t pack_flip(const double* U, int n, bool mixed, void* Ufk, int64_t ldu, void* Ufc, hipStream_t s);
+hipError_t grad_norm(const double* pm, const int* pe, int64_t n_tiles, double total, double* norm, hipStream_t s);
+hipError_t acc_upper(const void* C, bool mixed, int nd, double* Ub, int n, hipStream_t s);
+// step 0: Ubar diagonal; 1: Phi(X); 2: C_bar -> G, ga, gb
+hipError_t sigma_chain(double* Ub, int n, const double* gd, double* X, hipStream_t s, int step,
+                       const double* Cb, const int32_t* perm, const double* scale, const double* Sig,
+                       const double* a_std, const double* b_std, const double* ga_std, const double* gb_std,
+                       double* G, double* ga, double* gb);
+// the rocBLAS handle of this host thread and device (rocblas_handle), stream set to s
+void* blas_handle_void(hipStream_t s);
 int dense_mfma_tile(bool mixed);
 int dense_mfma_panel(bool mixed);
 // dimension from which the dense path uses the matrix-core kernel

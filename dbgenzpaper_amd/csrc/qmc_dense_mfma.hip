@@ -71,12 +71,12 @@ struct MfmaCfg<false> {  // FP64: 128 samples x 128 targets, 128 accumulator reg
 
 // one K-step of the panel contraction: Y rows k0 + lq of the lane's sample
 // blocks, U rows k0 + lq of the panel's target blocks
-template <typename T, int NSB, int NTB, int TS>
-__device__ __forceinline__ void load_step(T (&av)[NSB], T (&bv)[NTB], const T* __restrict__ ya,
+template <typename T, int NSB, int NTB>
+__device__ __forceinline__ void load_step(T (&av)[NSB], T (&bv)[NTB], const T* __restrict__ ya, int64_t ys,
                                           const T* __restrict__ ub, int64_t ldu, int k0)
 {
 #pragma unroll
-    for (int sb = 0; sb < NSB; ++sb) av[sb] = ya[(size_t)k0 * TS + 16 * sb];
+    for (int sb = 0; sb < NSB; ++sb) av[sb] = ya[(size_t)k0 * ys + 16 * sb];
 #pragma unroll
     for (int tb = 0; tb < NTB; ++tb) bv[tb] = ub[(size_t)k0 * ldu + 16 * tb];
 }
@@ -84,7 +84,13 @@ __device__ __forceinline__ void load_step(T (&av)[NSB], T (&bv)[NTB], const T* _
 int dense_mfma_tile(bool mixed) { return mixed ? MfmaCfg<true>::TS : MfmaCfg<false>::TS; }
 int dense_mfma_panel(bool mixed) { return mixed ? MfmaCfg<true>::PW : MfmaCfg<false>::PW; }
 
-template <bool MIXED, int GEN, int FORM, typename T = typename std::conditional<MIXED, float, double>::type>
+// MODE kModeIntegrate: the estimate.  kModeRecord: the same, keeping y and
+// the conditional means of every sample (rows of A.rs) and the per-sample
+// weights, for the adjoint.  kModeReverse: the reverse sweep in flipped index
+// order (Uk / Ucol hold the flipped factor, the "y" of a step is sbar); see
+// walk_block_rev.
+template <bool MIXED, int GEN, int FORM, int MODE,
+          typename T = typename std::conditional<MIXED, float, double>::type>
 __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(DenseArgs A, const T* __restrict__ Uk,
                                                          const T* __restrict__ Ucol,
                                                          const T* __restrict__ ak,
@@ -92,7 +98,10 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
                                                          const T* __restrict__ invk,
                                                          T* __restrict__ Yall,
                                                          double* __restrict__ part_m,
-                                                         int* __restrict__ part_e, DBG_POINT_PARAMS)
+                                                         int* __restrict__ part_e,
+                                                         T* __restrict__ srec_all,
+                                                         const T* __restrict__ yrec_all,
+                                                         DBG_POINT_PARAMS)
 {
     using V = typename Mfma<T>::V;
     constexpr int TS = MfmaCfg<MIXED>::TS;
@@ -101,18 +110,22 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
     constexpr int WS = TS / 4;         // samples per wavefront
     constexpr int NSB = WS / 16;       // 16-sample blocks per wavefront
     constexpr int NTB = PW / 16;       // 16-target blocks per panel
+    constexpr bool REV = MODE == kModeReverse;
+    constexpr bool REC = MODE == kModeRecord;
 
     __shared__ T s_x[TS * kSXS];       // accumulator hand-off, [sample][16]
     __shared__ T s_y[kSB * TS];        // y of the current sub-block, [k][sample]
+    __shared__ double s_red[REV ? 4 * kSB * 3 : 1];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wv = tid >> 6;
     const int lr = lane & 15;          // row/column inside a 16x16 block
     const int lq = lane >> 4;          // k (operands) or row quad (result)
-    const int nd = A.n - 1;
+    const int n = A.n;
+    const int nd = n - 1;
     const int64_t ldu = A.ldu;
-    T* __restrict__ Ys = Yall + (size_t)blockIdx.x * (size_t)A.y_slot;
+    const int64_t ys = REC ? A.rs : TS;  // row stride of the y workspace
     const bool walker = tid < TS;
     const PointArgs pts = DBG_WITH_TABLES(A.pts);
 
@@ -121,10 +134,22 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
         const int64_t jt = tile - (int64_t)k * A.tiles_per_rep;
         const int64_t j = jt * TS + tid + 1;  // 1-based point index of the walker's sample
         const bool valid = walker && j <= A.m;
+        T* __restrict__ Ys = REC ? Yall + (size_t)tile * TS : Yall + (size_t)blockIdx.x * (size_t)A.y_slot;
+        T* __restrict__ srec = (REC || REV) ? srec_all + (size_t)tile * TS : nullptr;
+        const T* __restrict__ yrec = REV ? yrec_all + (size_t)tile * TS : nullptr;
 
         typename Path<MIXED, FORM>::State st;
         Weight<T> w;
-        Path<MIXED, FORM>::init(st, w, A.first);
+        T ybar = T(0), wt = T(0);
+        if constexpr (REV) {
+            if (valid) {
+                const size_t idx = (size_t)tile * TS + tid;
+                const T pmv = ((const T*)A.wm)[idx];
+                wt = pmv != T(0) ? (T)(ldexp((double)pmv, A.we[idx] - (int)A.norm[0]) * A.norm[1]) : T(0);
+            }
+        } else {
+            Path<MIXED, FORM>::init(st, w, A.first);
+        }
 
         for (int g0 = 0; g0 < nd; g0 += PW) {
             // ---- S = Y[:, 0:g0] * U[0:g0, g0+1 : g0+PW+1]
@@ -137,7 +162,7 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
                 // operands of K-step k0 (4 dimensions) are loaded kPF - 1 steps
                 // ahead into a ring of register buffers, so the loads of
                 // later steps are in flight while the matrix cores run
-                const T* __restrict__ ya = Ys + (size_t)lq * TS + wv * WS + lr;
+                const T* __restrict__ ya = Ys + (size_t)lq * ys + wv * WS + lr;
                 const T* __restrict__ ub = Uk + (size_t)lq * ldu + g0 + 1 + lr;
                 // g0 is a multiple of PW, hence of 4 kPF: no partial ring turn.
                 // Prefetches past the panel start re-read the last K-step
@@ -146,14 +171,14 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
                 T av[kPF][NSB], bv[kPF][NTB];
                 if (g0 > 0) {
 #pragma unroll
-                    for (int q = 0; q < kPF - 1; ++q) load_step<T, NSB, NTB, TS>(av[q], bv[q], ya, ub, ldu, 4 * q);
+                    for (int q = 0; q < kPF - 1; ++q) load_step<T, NSB, NTB>(av[q], bv[q], ya, ys, ub, ldu, 4 * q);
                 }
                 for (int k0 = 0; k0 < g0; k0 += 4 * kPF) {
 #pragma unroll
                     for (int q = 0; q < kPF; ++q) {
                         const int kn = min(k0 + 4 * (q + kPF - 1), g0 - 4);
-                        load_step<T, NSB, NTB, TS>(av[(q + kPF - 1) % kPF], bv[(q + kPF - 1) % kPF], ya, ub, ldu,
-                                                   kn);
+                        load_step<T, NSB, NTB>(av[(q + kPF - 1) % kPF], bv[(q + kPF - 1) % kPF], ya, ys, ub, ldu,
+                                               kn);
 #pragma unroll
                         for (int sb = 0; sb < NSB; ++sb)
 #pragma unroll
@@ -179,10 +204,28 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
                 }
                 __syncthreads();
                 const int cnt = min(kSB, nd - gb);
-                if (walker)
-                    walk_block<MIXED, GEN, FORM, TS, kSXS, kSB>(pts, gb, cnt, k, j, tid, st, w, s_x, s_y,
-                                                             Ys, Ucol, A.ldc, ak, bk, invk);
+                if constexpr (REV) {
+                    if (walker)
+                        walk_block_rev<MIXED, GEN, TS, kSXS, kSB>(pts, n, gb, cnt, k, j, tid, ybar, wt, s_x, s_y, Ys,
+                                                                  Ucol, A.ldc, ak, bk, invk, srec, yrec, A.rs,
+                                                                  s_red);
+                } else {
+                    if (walker)
+                        walk_block<MIXED, GEN, FORM, TS, kSXS, kSB>(pts, gb, cnt, k, j, tid, st, w, s_x, s_y, Ys, ys,
+                                                                     Ucol, A.ldc, ak, bk, invk, srec);
+                }
                 __syncthreads();
+                if constexpr (REV) {
+                    // per-index sums of the four waves -> the global accumulators
+                    if (tid < 3 * cnt) {
+                        const int cc = tid / 3, q = tid - 3 * cc;
+                        double v = 0.0;
+#pragma unroll
+                        for (int ww = 0; ww < TS / 64; ++ww) v += s_red[(ww * kSB + cc) * 3 + q];  // walker waves
+                        double* dst = q == 0 ? A.ga : (q == 1 ? A.gb : A.gd);
+                        atomicAdd(dst + (n - 1 - (gb + cc)), v);
+                    }
+                }
                 // later targets of the panel += Y_sub * U[sub, later]
                 if (b + 1 < NTB && gb + kSB < nd) {
 #pragma unroll
@@ -205,24 +248,46 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
                 __syncthreads();
             }
         }
-        const double pm = valid ? (double)w.pm : 0.0;
-        double S;
-        int E;
-        block_scaled_sum<256>(pm, w.pe, S, E);
-        if (tid == 0) {
-            part_m[tile] = S;
-            part_e[tile] = E;
+        if constexpr (REV) {
+            // index 0: s_0 = 0, no sbar to pass on
+            RevOut o{0.0, 0.0, 0.0};
+            if (walker)
+                (void)rev_step<MIXED, GEN>(pts, n, 0, k, j, ybar, wt, T(0), n >= 2 ? yrec[tid] : T(0), ak[0], bk[0],
+                                           invk[0], o);
+            const double a0 = wave_sum_f64(o.ga), b0 = wave_sum_f64(o.gb), d0 = wave_sum_f64(o.gd);
+            if (lane == 0 && walker) {
+                atomicAdd(A.ga, a0);
+                atomicAdd(A.gb, b0);
+                atomicAdd(A.gd, d0);
+            }
+        } else {
+            const double pm = valid ? (double)w.pm : 0.0;
+            if constexpr (REC) {
+                if (walker) {
+                    ((T*)A.wm)[(size_t)tile * TS + tid] = valid ? w.pm : T(0);
+                    A.we[(size_t)tile * TS + tid] = w.pe;
+                }
+            }
+            double S;
+            int E;
+            block_scaled_sum<256>(pm, w.pe, S, E);
+            if (tid == 0) {
+                part_m[tile] = S;
+                part_e[tile] = E;
+            }
         }
     }
 }
 
-template <bool MIXED, int GEN, int FORM>
+template <bool MIXED, int GEN, int FORM, int MODE = kModeIntegrate>
 static void launch_m(const DenseArgs& A, int grid, hipStream_t s)
 {
     using T = typename std::conditional<MIXED, float, double>::type;
-    hipLaunchKernelGGL((dense_mfma_kernel<MIXED, GEN, FORM>), dim3(grid), dim3(256), 0, s, A,
+    T* Y = (MODE == kModeRecord) ? (T*)A.yrec : (T*)A.Y;
+    hipLaunchKernelGGL((dense_mfma_kernel<MIXED, GEN, FORM, MODE>), dim3(grid), dim3(256), 0, s, A,
                        (const T*)A.Uk, (const T*)A.Ucol, (const T*)A.ak, (const T*)A.bk,
-                       (const T*)A.invk, (T*)A.Y, A.part_m, A.part_e, DBG_POINT_ARGS(A.pts));
+                       (const T*)A.invk, Y, A.part_m, A.part_e, (T*)A.srec, (const T*)A.yrec,
+                       DBG_POINT_ARGS(A.pts));
 }
 
 hipError_t launch_dense_mfma(const DenseArgs& A, bool mixed, int gen, int form, int grid,
@@ -247,6 +312,29 @@ hipError_t launch_dense_mfma(const DenseArgs& A, bool mixed, int gen, int form, 
             launch_m<false, kGenSobol, kPhiErfc>(A, grid, s);
         else
             launch_m<false, kGenSobol, kPhiTail>(A, grid, s);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_dense_adjoint(const DenseArgs& A, bool mixed, int gen, int grid, hipStream_t s)
+{
+    const bool rev = A.mode == kModeReverse;
+    if (mixed) {
+        if (gen == kGenRichtmyer) {
+            if (rev) launch_m<true, kGenRichtmyer, kPhiTail, kModeReverse>(A, grid, s);
+            else launch_m<true, kGenRichtmyer, kPhiTail, kModeRecord>(A, grid, s);
+        } else {
+            if (rev) launch_m<true, kGenSobol, kPhiTail, kModeReverse>(A, grid, s);
+            else launch_m<true, kGenSobol, kPhiTail, kModeRecord>(A, grid, s);
+        }
+    } else {
+        if (gen == kGenRichtmyer) {
+            if (rev) launch_m<false, kGenRichtmyer, kPhiTail, kModeReverse>(A, grid, s);
+            else launch_m<false, kGenRichtmyer, kPhiTail, kModeRecord>(A, grid, s);
+        } else {
+            if (rev) launch_m<false, kGenSobol, kPhiTail, kModeReverse>(A, grid, s);
+            else launch_m<false, kGenSobol, kPhiTail, kModeRecord>(A, grid, s);
+        }
     }
     return hipGetLastError();
 }

@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kTileV) void dense_valu_kernel(DenseArgs A, const T
 #pragma unroll
             for (int c = 0; c < kWV; ++c) s_x[tid * kSXV + c] = s[c];
             walk_block<MIXED, GEN, FORM, kTileV, kSXV, kWV>(pts, g0, min(kWV, nd - g0), k, j, tid, st, w,
-                                                          s_x, s_y, Ys, Ucol, A.ldc, ak, bk, invk);
+                                                          s_x, s_y, Ys, kTileV, Ucol, A.ldc, ak, bk, invk);
         }
         const double pm = valid ? (double)w.pm : 0.0;
         double S;

@@ -134,6 +134,18 @@ def cholesky_classic(M, a, b, block_size=2 ** 7, block_size2=2 ** 9, block_size3
 # ---------------------------------------------------------------------------
 # QMC data objects  (QMCData, QMCDataSparse, QMCData_Distributions)
 # ---------------------------------------------------------------------------
+@dataclass
+class QMCGrad:
+    """d log P of the tail-form estimate (see _QMCBase.grad)."""
+    log_value: float
+    a: np.ndarray          # d log P / d a (original order)
+    b: np.ndarray          # d log P / d b
+    Sigma: np.ndarray      # d log P / d Sigma, symmetric
+    a_std: np.ndarray      # d log P / d a' (standardized, permuted)
+    b_std: np.ndarray      # d log P / d b'
+    U: np.ndarray          # d log P / d U (upper triangle)
+
+
 class _QMCBase:
     """Owns one device plan (factor, packed layouts, points, workspace)."""
 
@@ -253,6 +265,25 @@ class _QMCBase:
         e = np.zeros(R, np.int32)
         L.check(L.lib().dbg_plan_get_reps(self._plan, L.ptr(m, C.c_double), L.ptr(e, C.c_int32)))
         return m, e.astype(np.int64)
+
+    # -- adjoint -------------------------------------------------------------
+    def grad(self) -> "QMCGrad":
+        """Gradient of log P of one round of the tail-form estimate (the
+        plan's current block of m points and its shifts), computed on the
+        device: with respect to the constructor's a, b and Sigma (symmetric,
+        pivot order fixed; d log P = sum_ij G_ij E_ij for a symmetric
+        perturbation E) and to the standardized, permuted a', b' and the
+        factor U.  No reference counterpart."""
+        n = self.n
+        lv = C.c_double()
+        ga, gb = np.zeros(n), np.zeros(n)
+        gs = np.zeros((n, n), order="F")
+        ga_s, gb_s = np.zeros(n), np.zeros(n)
+        gU = np.zeros((n, n), order="F")
+        L.check(L.lib().dbg_qmc_grad(self._plan, C.byref(lv), L.ptr(ga, C.c_double), L.ptr(gb, C.c_double),
+                                     L.ptr(gs, C.c_double), L.ptr(ga_s, C.c_double), L.ptr(gb_s, C.c_double),
+                                     L.ptr(gU, C.c_double)))
+        return QMCGrad(float(lv.value), ga, gb, gs, ga_s, gb_s, gU)
 
     # -- hot path ------------------------------------------------------------
     def loop(self, n_pts0: int = 0) -> None:

@@ -153,6 +153,24 @@ int dbg_plan_last_kernel_ms(dbg_plan* plan, float* ms);
 /* summary of the current per-shift means (after dbg_qmc_loop) */
 int dbg_plan_summary(dbg_plan* plan, dbg_result* out);
 
+/* Gradient of log P (no reference counterpart).  The estimate differentiated
+ * is one round of the tail-form estimate over the plan's current block of m
+ * points and its shifts (the forward is re-run with the per-sample y, the
+ * conditional means and the weights recorded).  Outputs (any may be NULL):
+ *   log_value  log P of that estimate;
+ *   ga, gb     d log P / d a, d log P / d b of the inputs of dbg_plan_create
+ *              (length n, original order);
+ *   gSigma     d log P / d Sigma (n x n, column-major, symmetric): for a
+ *              symmetric perturbation E, d log P = sum_ij gSigma_ij E_ij;
+ *              the pivot order of the factorization is held fixed;
+ *   ga_std, gb_std, gU  the same with respect to the plan's standardized,
+ *              permuted bounds and its factor U (upper triangle, n x n).
+ * Dense plans only.  The records take 2 (n-1) m n_reps values of T on the
+ * device; dbg_plan_last_kernel_ms then reports the device time of the whole
+ * gradient. */
+int dbg_qmc_grad(dbg_plan* plan, double* log_value, double* ga, double* gb, double* gSigma,
+                 double* ga_std, double* gb_std, double* gU);
+
 /* The factor and the shift means held by the plan. */
 int dbg_plan_get_factor(dbg_plan* plan, double* U, int32_t* perm, double* a, double* b,
                         double* scale, int32_t* rank);
