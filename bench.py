@@ -59,6 +59,7 @@ def main() -> int:
                          "several seconds on 16 host threads)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-grad", action="store_true", help="skip the gradient leg (d = 1000 factor model)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes three times (the reference "
                          "kernel for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -146,11 +147,51 @@ def main() -> int:
         out["rel_err_mixed_vs_fp64_full"] = abs(math.expm1(s_mixed.log_value - s64.log_value))
         D64.close()
 
+    if not args.no_grad:
+        out["adjoint"] = grad_leg(args.steps)
+
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_leg(P, d, args.cpu_points, out)
 
     print(json.dumps(out), flush=True)
     return 0
+
+
+def grad_leg(steps):
+    """Gradient of log P on its own workload: factor model d = 1000,
+    B 1000 x 16, lower orthant b = Phi^{-1}(u), u ~ U(0.02, 0.20), Richtmyer
+    N = 2^18 x 8, FP32-mixed.  Device time of one forward integration kernel
+    against the device time of the whole gradient (forward with records,
+    reverse sweep, Y^T Sbar, Cholesky / standardization chain to Sigma)."""
+    import numpy as np
+
+    import dbgenzpaper_amd as dbg
+    from dbgenzpaper_amd import workloads as W
+
+    d, m, R = 1000, 2 ** 18, 8
+    P = W.factor_model(d, 16, m=m, n_reps=R, seed=4, u_range=(0.02, 0.20))
+    opts = dbg.QMC_opts(np.float32, m=m, n_reps=R, max_pts=m, max_abs_err=0.0)
+    D = dbg.QMCData(P.Sigma, P.a, P.b, opts, None, "Richtmyer", shifts=P.shifts, phi_form="tail")
+    D.loop(0)
+    D.sync()
+    fw = []
+    for _ in range(max(1, steps)):
+        D.loop(0)
+        D.sync()
+        fw.append(D.last_kernel_ms())
+    g = D.grad()  # first call allocates the records
+    gm = []
+    for _ in range(max(1, steps)):
+        g = D.grad()
+        gm.append(D.last_kernel_ms())
+    fwd = sum(fw) / len(fw)
+    grd = sum(gm) / len(gm)
+    _log(f"adjoint d={d}: forward kernel {fwd:.1f} ms, gradient {grd:.1f} ms, log P {g.log_value:.4f}")
+    D.close()
+    return {"workload": f"factor_model d={d} k=16 lower orthant u~U(0.02,0.20), Richtmyer N={m} x {R} shifts, "
+                        "FP32-mixed, gradient w.r.t. a, b, Sigma",
+            "forward_kernel_ms": fwd, "gradient_ms": grd, "ratio": grd / fwd, "log_p": g.log_value,
+            "grad_b_norm": float(np.linalg.norm(g.b)), "grad_sigma_norm": float(np.linalg.norm(g.Sigma))}
 
 
 def copy_calibration(nbytes):

@@ -9,8 +9,8 @@
 //                  is the forward contraction in flipped index order;
 //   grad_norm      E = max exponent of the tile sums, 1 / sum(p 2^-E) and
 //                  log P of the recorded forward;
-//   acc_upper      Ubar[l, g+1] += C[l, g] (l <= g) from the chunked
-//                  Y^T Sbar products;
+//   ubar_product   Ubar[l, g+1] = sum_s y_l sbar_{g+1} (l <= g), split-K on
+//                  the matrix cores with FP64 sums of the chunk partials;
 //   tril_half      Phi(X): lower triangle, halved diagonal (Cholesky adjoint);
 //   sigma_map      C_bar -> G (symmetrized, unpermuted, unstandardized) and
 //                  the gradients of the original bounds;
@@ -92,24 +92,134 @@ hipError_t grad_norm(const double* pm, const int* pe, int64_t n_tiles, double to
     return hipGetLastError();
 }
 
-// Ubar (n x n, column-major) += C (nd x nd) shifted: Ubar[l, g + 1] += C[l, g] for l <= g
+// ---------------------------------------------------------------------------
+// Ubar[l, g + 1] = sum_s y_l(s) sbar_{g+1}(s): a product of two row-major
+// (row, sample) arrays over a very long sample axis and a short output
+// (nd x nd, upper triangle only).  Split-K on the matrix cores: workgroup
+// (tile pair, chunk) computes a 128 x 128 output tile over one chunk of
+// samples (each wave 64 x 64, 4 x 4 blocks of 16 x 16); lane (r, q) loads
+// four consecutive samples of one row with one vector load and feeds them to
+// four K = 4 instructions (sample s + 4q + c is the k index q of the c-th),
+// so both operands stream straight from memory in 64-byte row segments.
+// The chunk partials go to `part` and are summed in FP64 by ubar_reduce.
+// ---------------------------------------------------------------------------
 template <typename T>
-__global__ void acc_upper_kernel(const T* __restrict__ C, int nd, double* __restrict__ Ub, int n)
+struct Vec4;
+template <>
+struct Vec4<float> {
+    typedef float V __attribute__((ext_vector_type(4)));
+};
+template <>
+struct Vec4<double> {
+    typedef double V __attribute__((ext_vector_type(4)));
+};
+
+constexpr int kUbTile = 128;
+
+template <typename T>
+__global__ __launch_bounds__(256) void ubar_gemm_kernel(const T* __restrict__ Y, const T* __restrict__ S,
+                                                        int64_t rs, int nd, const int2* __restrict__ pairs,
+                                                        int n_pairs, int64_t kchunk, int64_t kmax,
+                                                        T* __restrict__ part)
 {
-    const int g = blockIdx.y;
-    for (int l = blockIdx.x * blockDim.x + threadIdx.x; l <= g; l += gridDim.x * blockDim.x)
-        Ub[(size_t)l + (size_t)(g + 1) * n] += (double)C[(size_t)l + (size_t)g * nd];
+    using V = typename Mfma<T>::V;
+    using L4 = typename Vec4<T>::V;
+    const int pair = blockIdx.x, kc = blockIdx.y;
+    const int2 tp = pairs[pair];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    const int wr = wv & 1, wc = wv >> 1;
+    const int l0 = tp.x * kUbTile + wr * 64, g0 = tp.y * kUbTile + wc * 64;
+    const int64_t k0 = (int64_t)kc * kchunk;
+    const int64_t k1 = min(kmax, k0 + kchunk);
+    const T* __restrict__ pa[4];
+    const T* __restrict__ pb[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int l = min(l0 + 16 * q + lr, nd - 1);  // rows past nd read row nd-1; discarded
+        const int g = min(g0 + 16 * q + lr, nd - 1);
+        pa[q] = Y + (size_t)l * rs + 4 * lq;
+        pb[q] = S + (size_t)g * rs + 4 * lq;
+    }
+    V acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = V{0, 0, 0, 0};
+    L4 a[2][4], b[2][4];
+    if (k0 < k1) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            a[0][q] = *(const L4*)(pa[q] + k0);
+            b[0][q] = *(const L4*)(pb[q] + k0);
+        }
+    }
+    for (int64_t s = k0; s < k1; s += 32) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int64_t sn = min(s + 16 * (h + 1), k1 - 16);  // next step (re-reads the last at the end)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                a[h ^ 1][q] = *(const L4*)(pa[q] + sn);
+                b[h ^ 1][q] = *(const L4*)(pb[q] + sn);
+            }
+            if (s + 16 * h < k1) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::mac(a[h][i][c], b[h][j][c], acc[i][j]);
+            }
+        }
+    }
+    T* __restrict__ out = part + ((size_t)kc * n_pairs + pair) * (kUbTile * kUbTile);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                out[(size_t)(wr * 64 + 16 * i + Mfma<T>::row(lq, r)) * kUbTile + wc * 64 + 16 * j + lr] = acc[i][j][r];
 }
 
-hipError_t acc_upper(const void* C, bool mixed, int nd, double* Ub, int n, hipStream_t s)
+// Ub[l, g + 1] += sum over chunks of the partial tiles (l <= g < nd), FP64
+template <typename T>
+__global__ __launch_bounds__(256) void ubar_reduce_kernel(const T* __restrict__ part, const int2* __restrict__ pairs,
+                                                          int n_pairs, int n_chunks, int nd, double* __restrict__ Ub,
+                                                          int n)
 {
-    dim3 g((nd + 255) / 256, nd), b(256);
-    if (mixed)
-        hipLaunchKernelGGL(acc_upper_kernel<float>, g, b, 0, s, (const float*)C, nd, Ub, n);
-    else
-        hipLaunchKernelGGL(acc_upper_kernel<double>, g, b, 0, s, (const double*)C, nd, Ub, n);
+    const int pair = blockIdx.x;
+    const int2 tp = pairs[pair];
+    for (int e = blockIdx.y * 256 + threadIdx.x; e < kUbTile * kUbTile; e += gridDim.y * 256) {
+        const int i = e / kUbTile, j = e - i * kUbTile;
+        const int l = tp.x * kUbTile + i, g = tp.y * kUbTile + j;
+        if (l >= nd || g >= nd || l > g) continue;
+        double sum = 0.0;
+        for (int kc = 0; kc < n_chunks; ++kc) sum += (double)part[((size_t)kc * n_pairs + pair) * (kUbTile * kUbTile) + e];
+        Ub[(size_t)l + (size_t)(g + 1) * n] += sum;
+    }
+}
+
+hipError_t ubar_product(const void* Y, const void* S, int64_t rs, int nd, bool mixed, const int2* pairs,
+                        int n_pairs, int n_chunks, int64_t kchunk, void* part, double* Ub, int n, hipStream_t s)
+{
+    dim3 g1(n_pairs, n_chunks), b(256);
+    dim3 g2(n_pairs, 16);
+    if (mixed) {
+        hipLaunchKernelGGL(ubar_gemm_kernel<float>, g1, b, 0, s, (const float*)Y, (const float*)S, rs, nd, pairs,
+                           n_pairs, kchunk, rs, (float*)part);
+        hipLaunchKernelGGL(ubar_reduce_kernel<float>, g2, b, 0, s, (const float*)part, pairs, n_pairs, n_chunks,
+                           nd, Ub, n);
+    } else {
+        hipLaunchKernelGGL(ubar_gemm_kernel<double>, g1, b, 0, s, (const double*)Y, (const double*)S, rs, nd,
+                           pairs, n_pairs, kchunk, rs, (double*)part);
+        hipLaunchKernelGGL(ubar_reduce_kernel<double>, g2, b, 0, s, (const double*)part, pairs, n_pairs, n_chunks,
+                           nd, Ub, n);
+    }
     return hipGetLastError();
 }
+int ubar_tile() { return kUbTile; }
 
 // diagonal of Ubar from the reverse sweep's accumulator
 __global__ void set_diag_kernel(double* Ub, int n, const double* __restrict__ d)

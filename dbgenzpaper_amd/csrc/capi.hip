@@ -257,14 +257,17 @@ struct dbg_plan {
         double* acc = nullptr;  // 3 n: d log P / d a', b', U_rr
         void* ufk = nullptr;    // flipped factor, packed rows (T)
         void* ufc = nullptr;    // flipped factor, column-major (T)
-        void* ctmp = nullptr;   // nd x nd (T)
+        void* part = nullptr;   // split-K partial tiles of Y^T Sbar (T)
+        int2* pairs = nullptr;  // upper-triangle tile pairs
+        int n_pairs = 0, n_chunks = 0;
+        int64_t kchunk = 0;
         double* ub = nullptr;   // n x n: d log P / d U
         double* x = nullptr;    // n x n work
         double* g = nullptr;    // n x n: d log P / d Sigma
         double* gab = nullptr;  // 2 n: d log P / d a, b
         std::vector<void*> all() const
         {
-            return {yrec, srec, wm, we, ywork, part_m, part_e, norm, acc, ufk, ufc, ctmp, ub, x, g, gab};
+            return {yrec, srec, wm, we, ywork, part_m, part_e, norm, acc, ufk, ufc, part, pairs, ub, x, g, gab};
         }
     } G;
 
@@ -509,7 +512,22 @@ int grad_alloc(dbg_plan* P)
     HIPCHK(dalloc(&G.acc, (size_t)3 * n * sizeof(double)));
     HIPCHK(dalloc(&G.ufk, (size_t)nd * P->ldu * T));
     HIPCHK(dalloc(&G.ufc, (size_t)n * n * T + 256 * T));
-    HIPCHK(dalloc(&G.ctmp, (size_t)nd * nd * T));
+    {
+        // split-K layout of Ubar = Y^T Sbar: upper-triangle pairs of 128-row tiles,
+        // enough sample chunks for about four workgroups per CU
+        const int tb = ubar_tile();
+        const int nt = (nd + tb - 1) / tb;
+        std::vector<int2> pr;
+        for (int ti = 0; ti < nt; ++ti)
+            for (int tj = ti; tj < nt; ++tj) pr.push_back(make_int2(ti, tj));
+        G.n_pairs = (int)pr.size();
+        const int want = std::max(1, (4 * ncu + G.n_pairs - 1) / G.n_pairs);
+        G.kchunk = std::max<int64_t>(32, ((G.rs + want - 1) / want + 31) / 32 * 32);
+        G.n_chunks = (int)((G.rs + G.kchunk - 1) / G.kchunk);
+        HIPCHK(dalloc(&G.pairs, pr.size() * sizeof(int2)));
+        HIPCHK(hipMemcpy(G.pairs, pr.data(), pr.size() * sizeof(int2), hipMemcpyHostToDevice));
+        HIPCHK(dalloc(&G.part, (size_t)G.n_chunks * G.n_pairs * tb * tb * T));
+    }
     HIPCHK(dalloc(&G.ub, (size_t)n * n * sizeof(double)));
     HIPCHK(dalloc(&G.x, (size_t)n * n * sizeof(double)));
     HIPCHK(dalloc(&G.g, (size_t)n * n * sizeof(double)));
@@ -574,25 +592,11 @@ int run_grad(dbg_plan* P)
     A.Ucol = G.ufc;
     A.Y = G.ywork;
     HIPCHK(launch_dense_adjoint(A, P->mixed, gen, G.grid, s));
-    // Ubar[l, g + 1] = sum_samples y_l sbar_{g+1}: chunks of samples, FP64 accumulation
+    // Ubar[l, g + 1] = sum_samples y_l sbar_{g+1}: split-K on the matrix cores, FP64 chunk sums
+    HIPCHK(ubar_product(G.yrec, G.srec, G.rs, nd, P->mixed, G.pairs, G.n_pairs, G.n_chunks, G.kchunk, G.part,
+                        G.ub, n, s));
     rocblas_handle h = (rocblas_handle)blas_handle_void(s);
     if (!h) return fail(DBG_ERR_DEVICE, "rocBLAS handle");
-    const int64_t chunk = (int64_t)G.tile * 256;
-    for (int64_t c0 = 0; c0 < G.rs; c0 += chunk) {
-        const int kk = (int)std::min<int64_t>(chunk, G.rs - c0);
-        if (P->mixed) {
-            const float one = 1.0f, zero = 0.0f;
-            BLASCHK(rocblas_sgemm(h, rocblas_operation_transpose, rocblas_operation_none, nd, nd, kk, &one,
-                                  (const float*)G.yrec + c0, (rocblas_int)G.rs, (const float*)G.srec + c0,
-                                  (rocblas_int)G.rs, &zero, (float*)G.ctmp, nd));
-        } else {
-            const double one = 1.0, zero = 0.0;
-            BLASCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nd, nd, kk, &one,
-                                  (const double*)G.yrec + c0, (rocblas_int)G.rs, (const double*)G.srec + c0,
-                                  (rocblas_int)G.rs, &zero, (double*)G.ctmp, nd));
-        }
-        HIPCHK(acc_upper(G.ctmp, P->mixed, nd, G.ub, n, s));
-    }
     HIPCHK(sigma_chain(G.ub, n, G.acc + 2 * n, nullptr, s, 0, nullptr, nullptr, nullptr, nullptr, nullptr,
                        nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
     // Cholesky adjoint: X = U Ubar^T, Phi(X), U^{-1} X, then X U^{-T}

@@ -26,33 +26,6 @@
 
 namespace dbg {
 
-typedef float floatx4 __attribute__((ext_vector_type(4)));
-typedef double doublex4 __attribute__((ext_vector_type(4)));
-
-template <typename T>
-struct Mfma;
-// Result layout of the 16x16x4 instructions (checked on the device): lane l
-// holds column l % 16; element r of its accumulator is row 4 * (l / 16) + r
-// for f32 and row (l / 16) + 4 * r for f64.
-template <>
-struct Mfma<float> {
-    using V = floatx4;
-    static __device__ __forceinline__ int row(int lq, int r) { return 4 * lq + r; }
-    static __device__ __forceinline__ V mac(float a, float b, V c)
-    {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-};
-template <>
-struct Mfma<double> {
-    using V = doublex4;
-    static __device__ __forceinline__ int row(int lq, int r) { return lq + 4 * r; }
-    static __device__ __forceinline__ V mac(double a, double b, V c)
-    {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-};
-
 constexpr int kSB = 16;      // walk sub-block (dimensions)
 constexpr int kSXS = kSB + 4;  // LDS row stride of the accumulator hand-off
 

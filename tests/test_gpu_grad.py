@@ -85,3 +85,64 @@ def test_grad_repeatable_and_state(dbg):
     assert _relnorm(g1.Sigma, g2.Sigma) < 1e-13
     v1, _, _ = dbg.qmc_pnorm(D)
     assert v0 == v1
+
+
+def test_grad_c4_central_differences(dbg):
+    """The gradient workload (factor model d = 1000, B 1000 x 16, lower
+    orthant b = Phi^{-1}(u), u ~ U(0.02, 0.20); points cut to 2^13 x 8 for
+    the test): the mixed and the FP64 gradients against FP64 central
+    differences (h = 1e-5) of the device estimate on the same points, for 32
+    random coordinates of b and 32 of Sigma (pivot order held: a coordinate
+    whose perturbation changes it is skipped)."""
+    d, m, R, h = 1000, 2 ** 13, 8, 1e-5
+    P = W.factor_model(d, 16, m=m, n_reps=R, seed=4, u_range=(0.02, 0.20))
+    print("grad c4 fd", flush=True)
+
+    def plan(S, b, T=np.float64):
+        opts = dbg.QMC_opts(T, m=m, n_reps=R, max_pts=m, max_abs_err=0.0)
+        return dbg.QMCData(S, P.a, b, opts, None, "Richtmyer", shifts=P.shifts, phi_form="tail")
+
+    def logp(S, b):
+        D = plan(S, b)
+        dbg.qmc_pnorm(D)
+        lp = D.last_result.log_value
+        perm = D.C.perm.copy()
+        D.close()
+        return lp, perm
+
+    D64 = plan(P.Sigma, P.b)
+    perm0 = D64.C.perm.copy()
+    g64 = D64.grad()
+    g32 = plan(P.Sigma, P.b, np.float32).grad()
+    rng = np.random.default_rng(7)
+    checked = 0
+    worst64, worst32 = 0.0, 0.0
+    for i in rng.choice(d, 32, replace=False):
+        e = np.zeros(d)
+        e[i] = h
+        lp_p, pp = logp(P.Sigma, P.b + e)
+        lp_m, pm = logp(P.Sigma, P.b - e)
+        if not (np.array_equal(pp, perm0) and np.array_equal(pm, perm0)):
+            continue
+        fd = (lp_p - lp_m) / (2 * h)
+        scale = max(abs(fd), 1.0)
+        worst64 = max(worst64, abs(g64.b[i] - fd) / scale)
+        worst32 = max(worst32, abs(g32.b[i] - fd) / scale)
+        checked += 1
+    for _ in range(32):
+        i, j = rng.choice(d, 2, replace=False)
+        E = np.zeros((d, d))
+        E[i, j] = E[j, i] = h
+        lp_p, pp = logp(P.Sigma + E, P.b)
+        lp_m, pm = logp(P.Sigma - E, P.b)
+        if not (np.array_equal(pp, perm0) and np.array_equal(pm, perm0)):
+            continue
+        fd = (lp_p - lp_m) / (2 * h)
+        scale = max(abs(fd), 1.0)
+        worst64 = max(worst64, abs(2 * g64.Sigma[i, j] - fd) / scale)
+        worst32 = max(worst32, abs(2 * g32.Sigma[i, j] - fd) / scale)
+        checked += 1
+    print(f"c4 fd: {checked} coordinates, worst fp64 {worst64:.2e}, mixed {worst32:.2e}", flush=True)
+    assert checked >= 48
+    assert worst64 < 1e-4
+    assert worst32 < 2e-2
