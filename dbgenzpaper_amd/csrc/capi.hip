@@ -233,6 +233,7 @@ struct dbg_plan {
     void* Y = nullptr;
     int64_t y_slot = 0;
     int grid = 0;
+    int ncu = 0;
     int tile = 256;
     bool use_mfma = false;
     int64_t tiles_per_rep = 0, n_tiles = 0;
@@ -378,6 +379,7 @@ int plan_build(dbg_plan* P, const double* Sigma, int64_t lda, const double* a, c
     int dev = 0, ncu = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    P->ncu = ncu;
     P->use_mfma = (o.kind == DBG_KIND_DENSE) && n >= kMfmaMinDim;
     P->tile = (o.kind != DBG_KIND_DENSE) ? sparse_tile()
                                           : (P->use_mfma ? dense_mfma_tile(P->mixed) : dense_valu_tile());
