@@ -60,6 +60,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-grad", action="store_true", help="skip the gradient leg (d = 1000 factor model)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other configurations (d = 16, d = 500, Sobol at d = 2500, banded d = 4000)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes three times (the reference "
                          "kernel for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -150,11 +152,95 @@ def main() -> int:
     if not args.no_grad:
         out["adjoint"] = grad_leg(args.steps)
 
+    if not args.no_configs:
+        out["other_configs"] = config_legs(args.steps)
+
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_leg(P, d, args.cpu_points, out)
 
     print(json.dumps(out), flush=True)
     return 0
+
+
+def _time_loop(D, steps):
+    """Device time of the integration kernel (HIP events) and wall time of a
+    whole step (refactor + loop), averaged over steps after one warm-up."""
+    D.refactor()
+    D.loop(0)
+    D.sync()
+    ks, t0 = [], time.perf_counter()
+    for _ in range(max(1, steps)):
+        D.refactor()
+        D.loop(0)
+        ks.append(D.last_kernel_ms())
+    D.sync()
+    return (time.perf_counter() - t0) / max(1, steps) * 1e3, sum(ks) / len(ks), D.summary()
+
+
+def config_legs(steps):
+    """The other configurations of the benchmark set, each on its own shape:
+    d = 16 random correlation (N = 2^14 x 8, FP64, with the CPU oracle on the
+    whole job), the factor model at d = 500 (mixed and FP64), the Sobol
+    variant at d = 2500 (mixed and FP64), and the banded d = 4000 sparse path
+    (bandwidth 16, unpivoted, N = 2^18 x 10; FP64 erf form and mixed)."""
+    import numpy as np
+
+    import dbgenzpaper_amd as dbg
+    from dbgenzpaper_amd import workloads as W
+
+    res = {}
+
+    def dense_pair(P, gen, label):
+        out = {}
+        for name, T in (("mixed", np.float32), ("fp64", np.float64)):
+            opts = dbg.QMC_opts(T, m=P.m, n_reps=P.n_reps, max_pts=P.m, max_abs_err=0.0)
+            D = dbg.QMCData(P.Sigma, P.a, P.b, opts, None, gen, shifts=P.shifts, phi_form="tail")
+            step, kms, sm = _time_loop(D, steps)
+            out[name] = {"step_ms": step, "kernel_ms": kms, "log_p": sm.log_value,
+                         "sample_dims_per_s": P.m * P.n_reps * P.Sigma.shape[0] / (step * 1e-3)}
+            D.close()
+        out["rel_err_mixed_vs_fp64"] = abs(math.expm1(out["mixed"]["log_p"] - out["fp64"]["log_p"]))
+        _log(f"{label}: mixed kernel {out['mixed']['kernel_ms']:.2f} ms, fp64 {out['fp64']['kernel_ms']:.2f} ms, "
+             f"rel err {out['rel_err_mixed_vs_fp64']:.2e}")
+        return out
+
+    # d = 16 random correlation, b ~ U(0, 2), N = 2^14 x 8 (the CPU-runnable case)
+    P0 = W.dense_random(16)
+    opts = dbg.QMC_opts(np.float64, m=P0.m, n_reps=P0.n_reps, max_pts=P0.m, max_abs_err=0.0)
+    D0 = dbg.QMCData(P0.Sigma, P0.a, P0.b, opts, None, "Richtmyer", shifts=P0.shifts)
+    t0 = time.perf_counter()
+    for _ in range(max(1, steps)):
+        v0, e0, _ = dbg.qmc_pnorm(D0)
+    t_gpu = (time.perf_counter() - t0) / max(1, steps)
+    from oracle import oracle as O
+    from tests._helpers import oracle_same_inputs
+    t0 = time.perf_counter()
+    r0 = oracle_same_inputs(O, D0)
+    t_cpu = time.perf_counter() - t0
+    res["d16_random_correlation"] = {"value": v0, "err3": e0, "gpu_ms": t_gpu * 1e3, "cpu_oracle_ms": t_cpu * 1e3,
+                                     "cpu_threads": O.num_threads(), "rel_err_vs_oracle": abs(v0 / r0.value - 1)}
+    D0.close()
+    _log(f"d=16: P {v0:.6f} +- {e0:.1e}, gpu {t_gpu * 1e3:.2f} ms/call, oracle {t_cpu * 1e3:.1f} ms")
+
+    res["factor_model_d500"] = dense_pair(W.factor_model(500, 8, m=2 ** 18, n_reps=10, seed=1), "Richtmyer", "d=500")
+    res["factor_model_d2500_sobol"] = dense_pair(W.factor_model(2500, 8, m=2 ** 18, n_reps=10, seed=1, gen="Sobol"),
+                                                 "Sobol", "d=2500 Sobol")
+
+    # banded d = 4000, bandwidth 16, unpivoted sparse path
+    P2 = W.banded(4000, 16, m=2 ** 18, n_reps=10, seed=2)
+    out = {}
+    for name, T, form in (("mixed", np.float32, "tail"), ("fp64", np.float64, "erf")):
+        opts = dbg.QMC_opts(T, m=P2.m, n_reps=P2.n_reps, max_pts=P2.m, max_abs_err=0.0)
+        D = dbg.QMCDataSparse(P2.Sigma, P2.a, P2.b, opts, None, "Richtmyer", shifts=P2.shifts, phi_form=form)
+        step, kms, sm = _time_loop(D, steps)
+        out[name] = {"step_ms": step, "kernel_ms": kms, "log_p": sm.log_value,
+                     "sample_dims_per_s": P2.m * P2.n_reps * 4000 / (step * 1e-3)}
+        D.close()
+    out["rel_err_mixed_vs_fp64"] = abs(math.expm1(out["mixed"]["log_p"] - out["fp64"]["log_p"]))
+    _log(f"banded d=4000: mixed kernel {out['mixed']['kernel_ms']:.2f} ms, fp64 {out['fp64']['kernel_ms']:.2f} ms, "
+         f"rel err {out['rel_err_mixed_vs_fp64']:.2e}")
+    res["banded_d4000_bw16"] = out
+    return res
 
 
 def grad_leg(steps):
