@@ -219,6 +219,9 @@ struct dbg_plan {
     int32_t* rowval = nullptr;
     void* nzval = nullptr;
     int64_t nnz = 0;
+    int* reach_d = nullptr;  // pattern reach (largest g - r read by column g + 1)
+    int ring = 0;            // LDS ring rows of the sparse kernel, 0: global workspace
+    int ring_grid = 0;       // resident blocks of that ring kernel
     // points
     double* q = nullptr;
     double* shifts_f = nullptr;
@@ -276,7 +279,7 @@ struct dbg_plan {
     {
         for (void* p : G.all()) (void)hipFree(p);
         F.release();
-        void* ptrs[] = {sig, a0, b0, Uk, ucol32, ak, bk, invk, first, colcnt, colptr, rowval, nzval, q, shifts_f,
+        void* ptrs[] = {sig, a0, b0, Uk, ucol32, ak, bk, invk, first, colcnt, colptr, rowval, nzval, reach_d, q, shifts_f,
                         q64, shift32, shifts_u, V, Y, part_m, part_e, rep_m, rep_e, out};
         for (void* p : ptrs) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -320,6 +323,14 @@ int plan_factor(dbg_plan* P)
         }
         P->nnz = nnz;
         HIPCHK(csc_fill(P->F.M, n, n, P->colptr, P->mixed, P->rowval, P->nzval, s));
+        int reach = -1;
+        HIPCHK(csc_reach(P->colptr, P->rowval, n, P->reach_d, s));
+        HIPCHK(hipMemcpyAsync(&reach, P->reach_d, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        P->ring = sparse_ring(P->mixed, reach);
+        const int gen = (o.kind == DBG_KIND_SPARSEQMC) ? kGenSobol : o.generator;
+        const int form = (o.phi_form == DBG_PHI_TAIL) ? kPhiTail : kPhiErf;
+        if (P->ring > 0) HIPCHK(sparse_grid(P->mixed, gen, form, P->ring, P->ncu, &P->ring_grid));
     }
     if (o.kind == DBG_KIND_DENSE && P->mixed) {
         if (P->ucol32 == nullptr) HIPCHK(dalloc(&P->ucol32, (size_t)n * n * sizeof(float)));
@@ -360,7 +371,12 @@ int plan_build(dbg_plan* P, const double* Sigma, int64_t lda, const double* a, c
     } else {
         HIPCHK(dalloc(&P->colcnt, (size_t)n * sizeof(int64_t)));
         HIPCHK(dalloc(&P->colptr, (size_t)(n + 1) * sizeof(int64_t)));
+        HIPCHK(dalloc(&P->reach_d, sizeof(int)));
     }
+    int dev = 0, ncu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    P->ncu = ncu;
     rc = plan_factor(P);
     if (rc) return rc;
 
@@ -376,10 +392,6 @@ int plan_build(dbg_plan* P, const double* Sigma, int64_t lda, const double* a, c
     HIPCHK(dalloc(&P->V, (size_t)std::max(nd, 1) * 32 * sizeof(uint32_t)));
 
     // integration workspace: one y slot per resident block
-    int dev = 0, ncu = 0;
-    HIPCHK(hipGetDevice(&dev));
-    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    P->ncu = ncu;
     P->use_mfma = (o.kind == DBG_KIND_DENSE) && n >= kMfmaMinDim;
     P->tile = (o.kind != DBG_KIND_DENSE) ? sparse_tile()
                                           : (P->use_mfma ? dense_mfma_tile(P->mixed) : dense_valu_tile());
@@ -473,7 +485,8 @@ int run_loop(dbg_plan* P, int64_t block, int64_t n_pts0)
         A.y_slot = P->y_slot;
         A.part_m = P->part_m;
         A.part_e = P->part_e;
-        HIPCHK(launch_sparse(A, P->mixed, gen, P->grid, P->stream));
+        const int grid = P->ring > 0 ? (int)std::min<int64_t>(P->n_tiles, P->ring_grid) : P->grid;
+        HIPCHK(launch_sparse(A, P->mixed, gen, P->ring, grid, P->stream));
     }
     HIPCHK(hipEventRecord(P->ev1, P->stream));
     HIPCHK(launch_reduce_reps(P->part_m, P->part_e, P->tiles_per_rep, o.n_reps, n_pts0, o.m,

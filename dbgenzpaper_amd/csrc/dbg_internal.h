@@ -107,8 +107,11 @@ int dense_mfma_panel(bool mixed);
 // dimension from which the dense path uses the matrix-core kernel
 constexpr int kMfmaMinDim = 96;
 
-hipError_t launch_sparse(const SparseArgs& A, bool mixed, int gen, int grid, hipStream_t s);
+// ring: 0 (y rows in the global workspace) or the LDS ring size from sparse_ring
+hipError_t launch_sparse(const SparseArgs& A, bool mixed, int gen, int ring, int grid, hipStream_t s);
 int sparse_tile();
+int sparse_ring(bool mixed, int reach);
+hipError_t sparse_grid(bool mixed, int gen, int form, int ring, int ncu, int* grid);
 
 // ---- batched problems of uneven dimension --------------------------------
 struct BatchProblem {
@@ -201,5 +204,6 @@ hipError_t csc_count(const double* U, int64_t lda, int n, int64_t* colcount, hip
 hipError_t csc_fill(const double* U, int64_t lda, int n, const int64_t* colptr, bool mixed,
                     int32_t* rowval, void* nzval, hipStream_t s);
 hipError_t exclusive_scan_i64(const int64_t* in, int64_t* out, int n, hipStream_t s);
+hipError_t csc_reach(const int64_t* colptr, const int32_t* rowval, int n, int* out, hipStream_t s);
 
 }  // namespace dbg

@@ -213,4 +213,27 @@ hipError_t csc_fill(const double* U, int64_t lda, int n, const int64_t* colptr, 
     return hipGetLastError();
 }
 
+// Reach of the pattern: the largest g - r over the rows r that column g + 1
+// reads when dimension g + 1 is conditioned on y_0..y_g (rows ascend, so a
+// column's first row is its farthest); -1 when triu(U, 1) is empty.  One
+// block: a max over n columns.
+__global__ void csc_reach_kernel(const int64_t* __restrict__ colptr, const int32_t* __restrict__ rowval,
+                                 int n, int* out)
+{
+    __shared__ int s_max[4];
+    int r = -1;
+    for (int j = threadIdx.x; j < n; j += blockDim.x)
+        if (colptr[j + 1] > colptr[j]) r = max(r, j - 1 - rowval[colptr[j]]);
+    for (int off = 32; off > 0; off >>= 1) r = max(r, __shfl_xor(r, off, 64));
+    if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3]));
+}
+
+hipError_t csc_reach(const int64_t* colptr, const int32_t* rowval, int n, int* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(csc_reach_kernel, dim3(1), dim3(256), 0, s, colptr, rowval, n, out);
+    return hipGetLastError();
+}
+
 }  // namespace dbg

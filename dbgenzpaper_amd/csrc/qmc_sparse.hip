@@ -4,10 +4,16 @@
 // One sample per lane, 256 points of one shift per block.  For each sampled
 // dimension the conditional mean of the next dimension is the dot of the
 // stored column of triu(U, 1) with the sample's earlier y values, in CSC row
-// order.  Row indices and values are the same for every lane (scalar loads);
-// the y values come from the block's workspace slot, written moments earlier
-// by the same lane, so for a banded factor they are served from the L1/L2.
-// No contraction here: the kernel is bound by the special functions.
+// order.  Row indices and values are the same for every lane (scalar loads).
+//
+// Where y values live: a banded factor only ever reads the last few rows, so
+// when the pattern's reach (largest distance between the newest dimension and
+// a row it reads) fits, the rows sit in an LDS ring of RW rows x 256 lanes,
+// row r in slot r mod RW, overwritten RW dimensions later, after its last
+// reader.  Each lane reads back only its own column, so no barrier is needed,
+// and the block needs no global workspace.  A wider pattern keeps y in the
+// block's global workspace slot (RW = 0).  No contraction here: with the ring
+// the kernel is bound by the special functions.
 #include <type_traits>
 
 #include "dbg_device.h"
@@ -17,12 +23,26 @@ namespace dbg {
 constexpr int kTileS = 256;
 int sparse_tile() { return kTileS; }
 
-template <bool MIXED, int GEN, int FORM>
+// LDS ring sizes: RW x 256 x sizeof(T) stays within 64 KB per block
+constexpr int kRingMax32 = 64;
+constexpr int kRingMax64 = 32;
+
+int sparse_ring(bool mixed, int reach)
+{
+    const int cap = mixed ? kRingMax32 : kRingMax64;
+    for (int rw = 16; rw <= cap; rw *= 2)
+        if (reach < rw) return rw;
+    return 0;
+}
+
+template <bool MIXED, int GEN, int FORM, int RW>
 __global__ __launch_bounds__(kTileS) void sparse_kernel(SparseArgs A, DBG_POINT_PARAMS)
 {
     const PointArgs pts = DBG_WITH_TABLES(A.pts);
     using T = typename std::conditional<MIXED, float, double>::type;
     using PT = Path<MIXED, FORM>;
+    static_assert(RW == 0 || (RW & (RW - 1)) == 0, "ring size is a power of two");
+    __shared__ T ring[RW > 0 ? RW : 1][kTileS];
     const int tid = threadIdx.x;
     const int nd = A.n - 1;
     const int64_t* __restrict__ colptr = A.colptr;
@@ -31,7 +51,7 @@ __global__ __launch_bounds__(kTileS) void sparse_kernel(SparseArgs A, DBG_POINT_
     const T* __restrict__ ak = (const T*)A.ak;
     const T* __restrict__ bk = (const T*)A.bk;
     const T* __restrict__ invk = (const T*)A.invk;
-    T* __restrict__ Ys = (T*)A.Y + (size_t)blockIdx.x * (size_t)A.y_slot;
+    T* __restrict__ Ys = (RW > 0) ? nullptr : (T*)A.Y + (size_t)blockIdx.x * (size_t)A.y_slot;
 
     for (int64_t tile = blockIdx.x; tile < A.n_tiles; tile += gridDim.x) {
         const int k = (int)(tile / A.tiles_per_rep);
@@ -54,11 +74,17 @@ __global__ __launch_bounds__(kTileS) void sparse_kernel(SparseArgs A, DBG_POINT_
                 const double u = point_u<GEN>(pts, g, k, j);
                 y = A.tent ? path_sample<MIXED, FORM>(st, u) : PT::sample_w(st, u);
             }
-            Ys[(size_t)g * kTileS + tid] = y;
+            if constexpr (RW > 0) ring[g & (RW - 1)][tid] = y;
+            else Ys[(size_t)g * kTileS + tid] = y;
             T sv = T(0);
             const int64_t p1 = colptr[g + 2];
-            for (int64_t p = colptr[g + 1]; p < p1; ++p)
-                sv = fma(Ys[(size_t)rowval[p] * kTileS + tid], nzval[p], sv);
+            for (int64_t p = colptr[g + 1]; p < p1; ++p) {
+                const int r = rowval[p];
+                T yr;
+                if constexpr (RW > 0) yr = ring[r & (RW - 1)][tid];
+                else yr = Ys[(size_t)r * kTileS + tid];
+                sv = fma(yr, nzval[p], sv);
+            }
             PT::phi(st, sv, ak[g + 1], bk[g + 1], invk[g + 1]);
             PT::weigh(w, st);
         }
@@ -73,25 +99,50 @@ __global__ __launch_bounds__(kTileS) void sparse_kernel(SparseArgs A, DBG_POINT_
     }
 }
 
-hipError_t launch_sparse(const SparseArgs& A, bool mixed, int gen, int grid, hipStream_t s)
+using SparseKernel = void (*)(SparseArgs, DBG_POINT_PARAMS);
+
+template <bool MIXED, int GEN, int FORM>
+SparseKernel pick_ring(int ring)
 {
-    dim3 g(grid), b(kTileS);
-    if (mixed) {
-        if (gen == kGenRichtmyer)
-            hipLaunchKernelGGL((sparse_kernel<true, kGenRichtmyer, kPhiTail>), g, b, 0, s, A, DBG_POINT_ARGS(A.pts));
-        else
-            hipLaunchKernelGGL((sparse_kernel<true, kGenSobol, kPhiTail>), g, b, 0, s, A, DBG_POINT_ARGS(A.pts));
-    } else if (A.form == kPhiTail) {
-        if (gen == kGenRichtmyer)
-            hipLaunchKernelGGL((sparse_kernel<false, kGenRichtmyer, kPhiTail>), g, b, 0, s, A, DBG_POINT_ARGS(A.pts));
-        else
-            hipLaunchKernelGGL((sparse_kernel<false, kGenSobol, kPhiTail>), g, b, 0, s, A, DBG_POINT_ARGS(A.pts));
-    } else {
-        if (gen == kGenRichtmyer)
-            hipLaunchKernelGGL((sparse_kernel<false, kGenRichtmyer, kPhiErf>), g, b, 0, s, A, DBG_POINT_ARGS(A.pts));
-        else
-            hipLaunchKernelGGL((sparse_kernel<false, kGenSobol, kPhiErf>), g, b, 0, s, A, DBG_POINT_ARGS(A.pts));
+    switch (ring) {
+    case 0: return sparse_kernel<MIXED, GEN, FORM, 0>;
+    case 16: return sparse_kernel<MIXED, GEN, FORM, 16>;
+    case 32: return sparse_kernel<MIXED, GEN, FORM, 32>;
+    case 64:
+        if constexpr (MIXED) return sparse_kernel<MIXED, GEN, FORM, 64>;
+        else return nullptr;
+    default: return nullptr;
     }
+}
+
+SparseKernel pick(bool mixed, int gen, int form, int ring)
+{
+    const bool sob = gen != kGenRichtmyer;
+    if (mixed)
+        return sob ? pick_ring<true, kGenSobol, kPhiTail>(ring) : pick_ring<true, kGenRichtmyer, kPhiTail>(ring);
+    if (form == kPhiTail)
+        return sob ? pick_ring<false, kGenSobol, kPhiTail>(ring) : pick_ring<false, kGenRichtmyer, kPhiTail>(ring);
+    return sob ? pick_ring<false, kGenSobol, kPhiErf>(ring) : pick_ring<false, kGenRichtmyer, kPhiErf>(ring);
+}
+
+// Resident blocks of the ring kernel on the whole device (LDS and registers
+// permitting); the workspace kernel keeps the plan's grid.
+hipError_t sparse_grid(bool mixed, int gen, int form, int ring, int ncu, int* grid)
+{
+    const SparseKernel fn = pick(mixed, gen, mixed ? kPhiTail : form, ring);
+    if (fn == nullptr) return hipErrorInvalidValue;
+    int per_cu = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, kTileS, 0);
+    if (e != hipSuccess) return e;
+    *grid = ncu * std::max(per_cu, 1);
+    return hipSuccess;
+}
+
+hipError_t launch_sparse(const SparseArgs& A, bool mixed, int gen, int ring, int grid, hipStream_t s)
+{
+    const SparseKernel fn = pick(mixed, gen, mixed ? kPhiTail : A.form, ring);
+    if (fn == nullptr) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(kTileS), 0, s, A, DBG_POINT_ARGS(A.pts));
     return hipGetLastError();
 }
 

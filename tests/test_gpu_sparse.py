@@ -64,3 +64,20 @@ def test_banded_mixed(dbg, oracle):
     r = oracle_same_inputs(oracle, D64)
     print("banded mixed rel", rel(v, r.value), flush=True)
     assert rel(v, r.value) < 2e-5
+
+
+@pytest.mark.parametrize("bw", [1, 20, 40, 90])
+def test_banded_row_layouts(dbg, oracle, bw):
+    """Each place the sparse kernel keeps earlier y rows: band reach 0 and 19
+    (LDS rings of 16 and 32 rows), 39 (a 64-row ring in FP32, the global
+    workspace in FP64) and 89 (the global workspace in both)."""
+    P = W.banded(160, bw, m=1024, n_reps=2, seed=bw)
+    print(f"banded layouts bw={bw}", flush=True)
+    o64 = dbg.QMC_opts(np.float64, m=P.m, n_reps=P.n_reps, max_pts=P.m)
+    o32 = dbg.QMC_opts(np.float32, m=P.m, n_reps=P.n_reps, max_pts=P.m)
+    D64 = dbg.QMCDataSparse(P.Sigma, P.a, P.b, o64, None, "Richtmyer", shifts=P.shifts)
+    v64, _, _ = dbg.qmc_pnorm(D64)
+    v32, _, _ = dbg.qmc_pnorm(dbg.QMCDataSparse(P.Sigma, P.a, P.b, o32, None, "Richtmyer", shifts=P.shifts))
+    r = oracle_same_inputs(oracle, D64)
+    assert rel(v64, r.value) < 1e-12
+    assert rel(v32, r.value) < 1e-4
