@@ -214,6 +214,7 @@ def config_legs(steps):
     t_gpu = (time.perf_counter() - t0) / max(1, steps)
     from oracle import oracle as O
     from tests._helpers import oracle_same_inputs
+    oracle_same_inputs(O, D0)  # loads the library and starts its threads
     t0 = time.perf_counter()
     r0 = oracle_same_inputs(O, D0)
     t_cpu = time.perf_counter() - t0
