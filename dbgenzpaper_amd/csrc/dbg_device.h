@@ -17,25 +17,28 @@ constexpr double kSqrt2 = 1.4142135623730951;       // Float64(sqrt(2.0))
 constexpr double kEps64 = 2.220446049250313e-16;    // eps(Float64)
 
 // ---------------------------------------------------------------------------
-// Special functions behind call boundaries.  Inlined into the dimension loop,
-// their polynomial constants are hoisted out of the loop into vector
+// FP64 special functions behind call boundaries.  Inlined into the dimension
+// loop, their polynomial constants are hoisted out of the loop into vector
 // registers (two per FP64 constant), which exhausts the register file; as
 // calls they keep the constants in scalar instructions local to each call.
+// The FP32 ones of the mixed walk (erfcf, the tail quantile) are inlined: their
+// constants are single-register literals, and inlined the walk's scheduler can
+// overlap their latency with the rest of the step (measured 7 % faster at
+// d = 2500).
 // ---------------------------------------------------------------------------
 __device__ __attribute__((noinline)) double dbg_erfinv(double x);
 __device__ __attribute__((noinline)) double dbg_erf(double x);
 __device__ __attribute__((noinline)) double dbg_erfc(double x);
 __device__ __attribute__((noinline)) double dbg_erfcinv(double x);
-__device__ __attribute__((noinline)) float dbg_erfcf(float x);
 __device__ __attribute__((noinline)) float dbg_erfcinvf(float x);
 
 __device__ __attribute__((noinline)) double dbg_erfinv(double x) { return erfinv(x); }
 __device__ __attribute__((noinline)) double dbg_erf(double x) { return erf(x); }
 __device__ __attribute__((noinline)) double dbg_erfc(double x) { return erfc(x); }
 __device__ __attribute__((noinline)) double dbg_erfcinv(double x) { return erfcinv(x); }
-__device__ __attribute__((noinline)) float dbg_erfcf(float x) { return erfcf(x); }
+__device__ __forceinline__ float dbg_erfcf(float x) { return erfcf(x); }
 __device__ __attribute__((noinline)) float dbg_erfcinvf(float x) { return erfcinvf(x); }
-__device__ __attribute__((noinline)) float dbg_tail_quantile(float t, int k) { return tail_quantile_f32(t, k); }
+__device__ __forceinline__ float dbg_tail_quantile(float t, int k) { return tail_quantile_f32(t, k); }
 
 // ---------------------------------------------------------------------------
 // Point generators (src/qmc_generators.jl)
@@ -62,21 +65,118 @@ __device__ __forceinline__ uint32_t sobol_x(const uint32_t* __restrict__ V, uint
     return x;
 }
 
+// Per-dimension entries of the point tables (dimension g, shift k), read
+// through the scalar cache.  The walks fetch them one dimension ahead of their
+// use so that the load latency overlaps the previous dimension's work.
+struct PointOp {
+    uint64_t q64;  // Richtmyer, mixed: q in 64-bit fixed point
+    uint32_t s32;  // Richtmyer, mixed: shift in 32-bit fixed point
+    double q;      // Richtmyer, FP64: q
+    double sf;     // Richtmyer, FP64: shift
+    uint64_t su;   // Sobol: digital shift
+};
+
+template <bool MIXED, int GEN>
+__device__ __forceinline__ PointOp point_op(const PointArgs& P, int g, int k)
+{
+    PointOp o{};
+    const size_t gk = (size_t)g + (size_t)k * P.nd;
+    if constexpr (GEN == kGenRichtmyer) {
+        if constexpr (MIXED) {
+            o.q64 = P.q64[g];
+            o.s32 = P.shift32[gk];
+        } else {
+            o.q = P.q[g];
+            o.sf = P.shifts_f[gk];
+        }
+    } else {
+        o.su = P.shifts_u[gk];
+    }
+    return o;
+}
+
 // Uniform u in [0, 1] of one (point, dimension, shift) before the tent map:
 // Richtmyer shift mod 1 (src/qmc_generators.jl:158-159) or Sobol digital shift
 // (:197-198).  Computed in FP64 for both precisions.
 template <int GEN>
-__device__ __forceinline__ double point_u(const PointArgs& P, int g, int k, int64_t j)
+__device__ __forceinline__ double point_u(const PointArgs& P, const PointOp& o, int g, int64_t j)
 {
     if constexpr (GEN == kGenRichtmyer) {
-        const double shift = P.shifts_f[(size_t)g + (size_t)k * P.nd];
-        double u = __dadd_rn(shift, richtmyer_x(P.n_base + j, P.q[g]));
+        double u = __dadd_rn(o.sf, richtmyer_x(P.n_base + j, o.q));
         return u > 1.0 ? __dsub_rn(u, 1.0) : u;
     } else {
         const uint64_t mask = P.mask;
-        const uint64_t shift = P.shifts_u[(size_t)g + (size_t)k * P.nd] & mask;
+        const uint64_t shift = o.su & mask;
         const uint64_t x = (uint64_t)sobol_x(P.V + (size_t)g * 32, P.sobol_first + (uint64_t)j) & mask;
         return __dmul_rn((double)((x ^ shift) & mask), P.invscale);
+    }
+}
+
+template <int GEN>
+__device__ __forceinline__ double point_u(const PointArgs& P, int g, int k, int64_t j)
+{
+    return point_u<GEN>(P, point_op<false, GEN>(P, g, k), g, j);
+}
+
+// Sobol uniform of the FP64 path from the Sobol integer x = sobol_x(..)
+__device__ __forceinline__ double sobol_u_from_x(const PointArgs& P, uint32_t x, uint64_t su)
+{
+    const uint64_t mask = P.mask;
+    return __dmul_rn((double)((((uint64_t)x & mask) ^ (su & mask)) & mask), P.invscale);
+}
+
+// ---------------------------------------------------------------------------
+// Sobol integers of a walk sub-block from LDS tables.  With idx = H 2^8 + l,
+// the Gray code of idx is gray(l) ^ (bit 8 of idx) 2^7 in its low byte and
+// gray(H) above it, so
+//   sobol_x(V_g, idx) = low_g[gray(idx) & 255] ^ high_g(H),
+//   low_g[e]  = XOR of V_g[k] over the set bits k < 8 of e,
+//   high_g(H) = XOR of V_g[8 + b] over the set bits b of gray(H).
+// The 256 points of a tile have consecutive indices, so H takes two values
+// H0, H0 + 1 per tile.  The tables of SB dimensions (SB x 256 low entries,
+// then SB x 2 high entries) are built by the workgroup before the walk.
+// ---------------------------------------------------------------------------
+struct SobolLane {
+    uint32_t c;  // gray(idx) & 255, plus 256 (H - H0)
+    __device__ __forceinline__ uint32_t lo() const { return c & 255u; }
+    __device__ __forceinline__ uint32_t hi() const { return c >> 8; }
+};
+
+__device__ __forceinline__ SobolLane sobol_lane(const PointArgs& P, int64_t j0, int64_t j)
+{
+    const uint64_t idx = P.sobol_first + (uint64_t)j;
+    const uint64_t H0 = (P.sobol_first + (uint64_t)j0) >> 8;
+    SobolLane sl;
+    sl.c = (uint32_t)((idx ^ (idx >> 1)) & 255u) | ((uint32_t)((idx >> 8) - H0) << 8);
+    return sl;
+}
+
+// NT >= 256 threads; dimensions [gb, gb + cnt), cnt <= SB; j0: the tile's
+// first point index
+template <int SB, int NT>
+__device__ __forceinline__ void sobol_tables(const PointArgs& P, int gb, int cnt, int64_t j0,
+                                             uint32_t* __restrict__ s_sob, int tid)
+{
+    static_assert(NT >= 256 && 2 * SB <= NT, "one thread per low entry");
+    if (tid < 256) {
+        const uint32_t e = (uint32_t)tid;
+        for (int cc = 0; cc < cnt; ++cc) {  // uniform
+            const uint32_t* __restrict__ V = P.V + (size_t)(gb + cc) * 32;
+            uint32_t x = 0u;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) x ^= ((e >> k) & 1u) ? V[k] : 0u;
+            s_sob[cc * 256 + e] = x;
+        }
+    }
+    if (tid < 2 * cnt) {
+        const int cc = tid >> 1;
+        const uint64_t H = ((P.sobol_first + (uint64_t)j0) >> 8) + (uint64_t)(tid & 1);
+        const uint32_t gh = (uint32_t)(H ^ (H >> 1));
+        const uint32_t* __restrict__ V = P.V + (size_t)(gb + cc) * 32 + 8;
+        uint32_t x = 0u;
+#pragma unroll
+        for (int b = 0; b < 24; ++b) x ^= ((gh >> b) & 1u) ? V[b] : 0u;
+        s_sob[SB * 256 + tid] = x;
     }
 }
 
@@ -108,20 +208,33 @@ __device__ __forceinline__ PointArgs with_tables(PointArgs P, DBG_POINT_PARAMS)
 // shift in 32-bit fixed point, wrapping mod 1.  Sobol: the masked XOR of
 // the FP64 path, aligned to the top of the word.
 template <int GEN>
-__device__ __forceinline__ uint32_t point_u32(const PointArgs& P, int g, int k, int64_t j)
+__device__ __forceinline__ uint32_t point_u32(const PointArgs& P, const PointOp& o, int g, int64_t j)
 {
     if constexpr (GEN == kGenRichtmyer) {
-        const uint64_t Q = P.q64[g];
+        const uint64_t Q = o.q64;
         const uint64_t nj = (uint64_t)(P.n_base + j);
         const uint32_t nlo = (uint32_t)nj, nhi = (uint32_t)(nj >> 32);
         const uint32_t qlo = (uint32_t)Q, qhi = (uint32_t)(Q >> 32);
         const uint32_t x = __umulhi(nlo, qlo) + nlo * qhi + nhi * qlo;
-        return x + P.shift32[(size_t)g + (size_t)k * P.nd];
+        return x + o.s32;
     } else {
-        const uint64_t v = ((uint64_t)sobol_x(P.V + (size_t)g * 32, P.sobol_first + (uint64_t)j) ^
-                            P.shifts_u[(size_t)g + (size_t)k * P.nd]) & P.mask;
+        const uint64_t v = ((uint64_t)sobol_x(P.V + (size_t)g * 32, P.sobol_first + (uint64_t)j) ^ o.su) &
+                           P.mask;
         return (uint32_t)(P.sobol_bits <= 32 ? (v << (32 - P.sobol_bits)) : (v >> (P.sobol_bits - 32)));
     }
+}
+
+template <int GEN>
+__device__ __forceinline__ uint32_t point_u32(const PointArgs& P, int g, int k, int64_t j)
+{
+    return point_u32<GEN>(P, point_op<true, GEN>(P, g, k), g, j);
+}
+
+// Sobol fixed-point uniform of the mixed path from the Sobol integer x
+__device__ __forceinline__ uint32_t sobol_u32_from_x(const PointArgs& P, uint32_t x, uint64_t su)
+{
+    const uint64_t v = ((uint64_t)x ^ su) & P.mask;
+    return (uint32_t)(P.sobol_bits <= 32 ? (v << (32 - P.sobol_bits)) : (v >> (P.sobol_bits - 32)));
 }
 
 // w = |2u - 1| (the tent map of src/qmc_generators.jl:160) and 1 - w in FP32
@@ -409,16 +522,31 @@ __device__ __forceinline__ typename Path<MIXED, FORM>::T path_sample(const typen
 // FP64 forms follow the reference's FP64 arithmetic; the mixed form takes
 // the point in fixed point and the quantile in FP32.
 template <bool MIXED, int GEN, int FORM>
-__device__ __forceinline__ typename Path<MIXED, FORM>::T path_draw(const PointArgs& pts,
+__device__ __forceinline__ typename Path<MIXED, FORM>::T path_draw(const PointArgs& pts, const PointOp& op,
                                                                    const typename Path<MIXED, FORM>::State& st,
-                                                                   int g, int k, int64_t j)
+                                                                   int g, int64_t j)
 {
     if constexpr (MIXED) {
         float w, wc;
-        tent_w(point_u32<GEN>(pts, g, k, j), w, wc);
+        tent_w(point_u32<GEN>(pts, op, g, j), w, wc);
         return tail_sample_f(st, w, wc);
     } else {
-        return path_sample<MIXED, FORM>(st, point_u<GEN>(pts, g, k, j));
+        return path_sample<MIXED, FORM>(st, point_u<GEN>(pts, op, g, j));
+    }
+}
+
+// the same draw for a Sobol point given its Sobol integer x
+template <bool MIXED, int FORM>
+__device__ __forceinline__ typename Path<MIXED, FORM>::T path_draw_sobol(const PointArgs& pts, const PointOp& op,
+                                                                         const typename Path<MIXED, FORM>::State& st,
+                                                                         uint32_t x)
+{
+    if constexpr (MIXED) {
+        float w, wc;
+        tent_w(sobol_u32_from_x(pts, x, op.su), w, wc);
+        return tail_sample_f(st, w, wc);
+    } else {
+        return path_sample<MIXED, FORM>(st, sobol_u_from_x(pts, x, op.su));
     }
 }
 
@@ -437,34 +565,90 @@ __device__ __forceinline__ typename Path<MIXED, FORM>::T path_draw(const PointAr
 // lane needs the same column.
 // ---------------------------------------------------------------------------
 // ys: row stride of the y workspace Ys; srec (optional): the conditional
-// means s_{g+1} are also recorded there (same stride), for the adjoint.
+// means s_{g+1} are also recorded there (same stride), for the adjoint;
+// s_sob (optional, Sobol): the block's Sobol tables from sobol_tables, with
+// the lane's SobolLane.  Each step's scalar operands are fetched one step
+// ahead (StepOps).
+//
+// Scalar operands of one walk step (sampled dimension g of the block at gb):
+// the point-table entries, column g + 1 of the factor from row gb to g
+// (the in-block dot), and the bounds of target g + 1.
+template <bool MIXED, int GEN, int SB, typename T>
+struct StepOps {
+    PointOp pt;
+    T u[SB];
+    T a, b, inv;
+
+    template <int CC>
+    __device__ __forceinline__ void fetch(const PointArgs& pts, int gb, int k, const T* __restrict__ Ucol,
+                                          int64_t ldc, const T* __restrict__ ak, const T* __restrict__ bk,
+                                          const T* __restrict__ invk)
+    {
+        const int g = gb + CC;
+        pt = point_op<MIXED, GEN>(pts, g, k);
+        const T* __restrict__ ucol = Ucol + (size_t)(g + 1) * ldc + gb;
+#pragma unroll
+        for (int l = 0; l <= CC; ++l) u[l] = ucol[l];
+        a = ak[g + 1];
+        b = bk[g + 1];
+        inv = invk[g + 1];
+    }
+};
+
+template <bool MIXED, int GEN, int FORM, int TS, int SXS, int SB, int CC, typename T, typename State>
+__device__ __forceinline__ void walk_steps(const PointArgs& pts, int gb, int cnt, int k, int64_t j, int t,
+                                           State& st, Weight<T>& w, T (&yv)[SB], StepOps<MIXED, GEN, SB, T>& cur,
+                                           const T* __restrict__ s_x, T* __restrict__ s_y, T* __restrict__ Ys,
+                                           int64_t ys, const T* __restrict__ Ucol, int64_t ldc,
+                                           const T* __restrict__ ak, const T* __restrict__ bk,
+                                           const T* __restrict__ invk, T* __restrict__ srec,
+                                           const uint32_t* __restrict__ s_sob, SobolLane sl)
+{
+    if constexpr (CC < SB) {
+        if (CC < cnt) {  // uniform
+            using PT = Path<MIXED, FORM>;
+            StepOps<MIXED, GEN, SB, T> nxt;
+            if constexpr (CC + 1 < SB) {
+                // the next step's operands, in flight while this step computes
+                if (CC + 1 < cnt) nxt.template fetch<CC + 1>(pts, gb, k, Ucol, ldc, ak, bk, invk);
+            }
+            const int g = gb + CC;
+            T y;
+            if (GEN == kGenSobol && s_sob != nullptr)
+                y = path_draw_sobol<MIXED, FORM>(pts, cur.pt, st,
+                                                 s_sob[CC * 256 + sl.lo()] ^ s_sob[SB * 256 + 2 * CC + sl.hi()]);
+            else
+                y = path_draw<MIXED, GEN, FORM>(pts, cur.pt, st, g, j);
+            yv[CC] = y;
+            s_y[CC * TS + t] = y;
+            Ys[(size_t)g * ys + t] = y;
+            T acc = s_x[t * SXS + CC];
+#pragma unroll
+            for (int l = 0; l <= CC; ++l) acc = fma(yv[l], cur.u[l], acc);
+            if (srec) srec[(size_t)g * ys + t] = acc;
+            PT::phi(st, acc, cur.a, cur.b, cur.inv);
+            PT::weigh(w, st);
+            if constexpr (CC + 1 < SB)
+                walk_steps<MIXED, GEN, FORM, TS, SXS, SB, CC + 1>(pts, gb, cnt, k, j, t, st, w, yv, nxt, s_x, s_y,
+                                                                   Ys, ys, Ucol, ldc, ak, bk, invk, srec, s_sob, sl);
+        }
+    }
+}
+
 template <bool MIXED, int GEN, int FORM, int TS, int SXS, int SB, typename T, typename State>
 __device__ __forceinline__ void walk_block(const PointArgs& pts, int gb, int cnt, int k, int64_t j,
                                            int t, State& st, Weight<T>& w, const T* __restrict__ s_x,
                                            T* __restrict__ s_y, T* __restrict__ Ys, int64_t ys,
                                            const T* __restrict__ Ucol, int64_t ldc,
                                            const T* __restrict__ ak, const T* __restrict__ bk,
-                                           const T* __restrict__ invk, T* __restrict__ srec = nullptr)
+                                           const T* __restrict__ invk, T* __restrict__ srec = nullptr,
+                                           const uint32_t* __restrict__ s_sob = nullptr, SobolLane sl = {})
 {
-    using PT = Path<MIXED, FORM>;
     T yv[SB];
-#pragma unroll
-    for (int cc = 0; cc < SB; ++cc) {
-        if (cc < cnt) {  // uniform
-            const int g = gb + cc;
-            const T y = path_draw<MIXED, GEN, FORM>(pts, st, g, k, j);
-            yv[cc] = y;
-            s_y[cc * TS + t] = y;
-            Ys[(size_t)g * ys + t] = y;
-            T acc = s_x[t * SXS + cc];
-            const T* __restrict__ ucol = Ucol + (size_t)(g + 1) * ldc + gb;
-#pragma unroll
-            for (int l = 0; l <= cc; ++l) acc = fma(yv[l], ucol[l], acc);
-            if (srec) srec[(size_t)g * ys + t] = acc;
-            PT::phi(st, acc, ak[g + 1], bk[g + 1], invk[g + 1]);
-            PT::weigh(w, st);
-        }
-    }
+    StepOps<MIXED, GEN, SB, T> first;
+    if (cnt > 0) first.template fetch<0>(pts, gb, k, Ucol, ldc, ak, bk, invk);
+    walk_steps<MIXED, GEN, FORM, TS, SXS, SB, 0>(pts, gb, cnt, k, j, t, st, w, yv, first, s_x, s_y, Ys, ys, Ucol,
+                                                 ldc, ak, bk, invk, srec, s_sob, sl);
 }
 
 // ---------------------------------------------------------------------------

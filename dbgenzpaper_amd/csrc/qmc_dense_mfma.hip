@@ -31,25 +31,41 @@ constexpr int kSXS = kSB + 4;  // LDS row stride of the accumulator hand-off
 
 template <bool MIXED>
 struct MfmaCfg;
-// TS samples x PW targets per workgroup; PF: depth of the operand ring of the
-// panel contraction; OCC: waves per SIMD the register budget is cut for.
+// TS samples x PW targets per workgroup; PFY / PFU: depths of the operand
+// rings of the panel contraction (Y rows stream from HBM, U rows are shared by
+// every workgroup and come from L2); OCC: waves per SIMD the register budget
+// is cut for.
 template <>
 struct MfmaCfg<true> {   // FP32: 256 samples x 128 targets, 128 accumulators per lane
-    static constexpr int TS = 256, PW = 128, PF = 2, OCC = 2;
+    static constexpr int TS = 256, PW = 128, PFY = 4, PFU = 2, OCC = 2;
 };
 template <>
 struct MfmaCfg<false> {  // FP64: 128 samples x 128 targets, 128 accumulator registers per lane
-    static constexpr int TS = 128, PW = 128, PF = 4, OCC = 1;
+    static constexpr int TS = 128, PW = 128, PFY = 4, PFU = 4, OCC = 1;
 };
 
-// one K-step of the panel contraction: Y rows k0 + lq of the lane's sample
-// blocks, U rows k0 + lq of the panel's target blocks
-template <typename T, int NSB, int NTB>
-__device__ __forceinline__ void load_step(T (&av)[NSB], T (&bv)[NTB], const T* __restrict__ ya, int64_t ys,
-                                          const T* __restrict__ ub, int64_t ldu, int k0)
+// Sample order inside a wavefront's WS samples: the matrix-core row lr of
+// sample block sb is sample NSB * lr + sb, so the NSB operands a lane needs
+// from one Y row are adjacent (16 bytes: one vector load).
+template <typename T, int NSB>
+struct alignas(16) YVec {
+    T v[NSB];
+};
+
+// Y rows k0 + lq of the lane's NSB samples
+template <typename T, int NSB>
+__device__ __forceinline__ void load_y(T (&av)[NSB], const T* __restrict__ ya, int64_t ys, int k0)
 {
+    static_assert(sizeof(YVec<T, NSB>) == 16, "one 16-byte load per lane and K-step");
+    const YVec<T, NSB> v = *(const YVec<T, NSB>*)(ya + (size_t)k0 * ys);
 #pragma unroll
-    for (int sb = 0; sb < NSB; ++sb) av[sb] = ya[(size_t)k0 * ys + 16 * sb];
+    for (int sb = 0; sb < NSB; ++sb) av[sb] = v.v[sb];
+}
+
+// U rows k0 + lq of the panel's NTB target blocks
+template <typename T, int NTB>
+__device__ __forceinline__ void load_u(T (&bv)[NTB], const T* __restrict__ ub, int64_t ldu, int k0)
+{
 #pragma unroll
     for (int tb = 0; tb < NTB; ++tb) bv[tb] = ub[(size_t)k0 * ldu + 16 * tb];
 }
@@ -79,7 +95,12 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
     using V = typename Mfma<T>::V;
     constexpr int TS = MfmaCfg<MIXED>::TS;
     constexpr int PW = MfmaCfg<MIXED>::PW;
-    constexpr int kPF = MfmaCfg<MIXED>::PF;
+    // the record pass (which keeps s per step as well) and the Sobol walk have
+    // no room in the FP32 register budget for the deeper Y ring
+    constexpr int kPFY = (MIXED && (MODE == kModeRecord || GEN == kGenSobol)) ? 2 : MfmaCfg<MIXED>::PFY;
+    constexpr int kPFU = MfmaCfg<MIXED>::PFU;
+    constexpr int kUnroll = kPFY > kPFU ? kPFY : kPFU;
+    static_assert(kUnroll % kPFY == 0 && kUnroll % kPFU == 0, "ring depths must divide the unroll");
     constexpr int WS = TS / 4;         // samples per wavefront
     constexpr int NSB = WS / 16;       // 16-sample blocks per wavefront
     constexpr int NTB = PW / 16;       // 16-target blocks per panel
@@ -89,6 +110,8 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
     __shared__ T s_x[TS * kSXS];       // accumulator hand-off, [sample][16]
     __shared__ T s_y[kSB * TS];        // y of the current sub-block, [k][sample]
     __shared__ double s_red[REV ? 4 * kSB * 3 : 1];
+    constexpr bool SOBT = GEN == kGenSobol && !REV;  // walk points from LDS Sobol tables
+    __shared__ uint32_t s_sob[SOBT ? kSB * 256 + 2 * kSB : 1];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -107,6 +130,8 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
         const int64_t jt = tile - (int64_t)k * A.tiles_per_rep;
         const int64_t j = jt * TS + tid + 1;  // 1-based point index of the walker's sample
         const bool valid = walker && j <= A.m;
+        SobolLane sl{};
+        if constexpr (SOBT) sl = sobol_lane(pts, jt * TS + 1, j);
         T* __restrict__ Ys = REC ? Yall + (size_t)tile * TS : Yall + (size_t)blockIdx.x * (size_t)A.y_slot;
         T* __restrict__ srec = (REC || REV) ? srec_all + (size_t)tile * TS : nullptr;
         const T* __restrict__ yrec = REV ? yrec_all + (size_t)tile * TS : nullptr;
@@ -132,31 +157,38 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
 #pragma unroll
                 for (int tb = 0; tb < NTB; ++tb) acc[sb][tb] = V{0, 0, 0, 0};
             {
-                // operands of K-step k0 (4 dimensions) are loaded kPF - 1 steps
-                // ahead into a ring of register buffers, so the loads of
-                // later steps are in flight while the matrix cores run
-                const T* __restrict__ ya = Ys + (size_t)lq * ys + wv * WS + lr;
+                // operands of K-step k0 (4 dimensions) are loaded kPFY - 1 (Y)
+                // and kPFU - 1 (U) steps ahead into rings of register buffers,
+                // so the loads of later steps are in flight while the matrix
+                // cores run
+                const T* __restrict__ ya = Ys + (size_t)lq * ys + wv * WS + NSB * lr;
                 const T* __restrict__ ub = Uk + (size_t)lq * ldu + g0 + 1 + lr;
-                // g0 is a multiple of PW, hence of 4 kPF: no partial ring turn.
-                // Prefetches past the panel start re-read the last K-step
-                // (unconditional loads keep the wait counts exact).
-                static_assert(PW % (4 * kPF) == 0, "panel width must be a multiple of the ring");
-                T av[kPF][NSB], bv[kPF][NTB];
+                // g0 is a multiple of PW, hence of 4 kUnroll: no partial ring
+                // turn.  Prefetches past the panel start re-read the last
+                // K-step (unconditional loads keep the wait counts exact).
+                static_assert(PW % (4 * kUnroll) == 0, "panel width must be a multiple of the ring");
+                T av[kPFY][NSB], bv[kPFU][NTB];
                 if (g0 > 0) {
 #pragma unroll
-                    for (int q = 0; q < kPF - 1; ++q) load_step<T, NSB, NTB>(av[q], bv[q], ya, ys, ub, ldu, 4 * q);
-                }
-                for (int k0 = 0; k0 < g0; k0 += 4 * kPF) {
+                    for (int q = 0; q < kPFY - 1; ++q) load_y<T, NSB>(av[q], ya, ys, 4 * q);
 #pragma unroll
-                    for (int q = 0; q < kPF; ++q) {
-                        const int kn = min(k0 + 4 * (q + kPF - 1), g0 - 4);
-                        load_step<T, NSB, NTB>(av[(q + kPF - 1) % kPF], bv[(q + kPF - 1) % kPF], ya, ys, ub, ldu,
-                                               kn);
+                    for (int q = 0; q < kPFU - 1; ++q) load_u<T, NTB>(bv[q], ub, ldu, 4 * q);
+                }
+                for (int k0 = 0; k0 < g0; k0 += 4 * kUnroll) {
+#pragma unroll
+                    for (int q = 0; q < kUnroll; ++q) {
+                        load_y<T, NSB>(av[(q + kPFY - 1) % kPFY], ya, ys, min(k0 + 4 * (q + kPFY - 1), g0 - 4));
+                        load_u<T, NTB>(bv[(q + kPFU - 1) % kPFU], ub, ldu, min(k0 + 4 * (q + kPFU - 1), g0 - 4));
+                        // keep the loads ahead of this step's matrix-core work:
+                        // left to itself the scheduler interleaves them and
+                        // consumes loads a few instructions after issue
+                        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                         for (int sb = 0; sb < NSB; ++sb)
 #pragma unroll
                             for (int tb = 0; tb < NTB; ++tb)
-                                acc[sb][tb] = Mfma<T>::mac(av[q][sb], bv[q][tb], acc[sb][tb]);
+                                acc[sb][tb] = Mfma<T>::mac(av[q % kPFY][sb], bv[q % kPFU][tb], acc[sb][tb]);
+                        __builtin_amdgcn_sched_barrier(0);
                     }
                 }
             }
@@ -172,9 +204,10 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
                         for (int sb = 0; sb < NSB; ++sb)
 #pragma unroll
                             for (int r = 0; r < 4; ++r)
-                                s_x[(wv * WS + 16 * sb + Mfma<T>::row(lq, r)) * kSXS + lr] = acc[sb][tb][r];
+                                s_x[(wv * WS + NSB * Mfma<T>::row(lq, r) + sb) * kSXS + lr] = acc[sb][tb][r];
                     }
                 }
+                if constexpr (SOBT) sobol_tables<kSB, 256>(pts, gb, min(kSB, nd - gb), jt * TS + 1, s_sob, tid);
                 __syncthreads();
                 const int cnt = min(kSB, nd - gb);
                 if constexpr (REV) {
@@ -185,7 +218,8 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
                 } else {
                     if (walker)
                         walk_block<MIXED, GEN, FORM, TS, kSXS, kSB>(pts, gb, cnt, k, j, tid, st, w, s_x, s_y, Ys, ys,
-                                                                     Ucol, A.ldc, ak, bk, invk, srec);
+                                                                     Ucol, A.ldc, ak, bk, invk, srec,
+                                                                     SOBT ? s_sob : nullptr, sl);
                 }
                 __syncthreads();
                 if constexpr (REV) {
@@ -204,9 +238,7 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
 #pragma unroll
                     for (int ks = 0; ks < kSB / 4; ++ks) {
                         T av[NSB];
-#pragma unroll
-                        for (int sb = 0; sb < NSB; ++sb)
-                            av[sb] = s_y[(4 * ks + lq) * TS + wv * WS + 16 * sb + lr];
+                        load_y<T, NSB>(av, s_y + (size_t)lq * TS + wv * WS + NSB * lr, TS, 4 * ks);
                         const T* __restrict__ ub = Uk + (size_t)(gb + 4 * ks + lq) * ldu + g0 + 1 + lr;
 #pragma unroll
                         for (int tb = 0; tb < NTB; ++tb) {

@@ -137,3 +137,21 @@ def test_mixed_within_tolerance(dbg, oracle, gen):
     r = oracle_same_inputs(oracle, D64)
     print(f"mixed {gen} rel err {rel(v, r.value):.3e} (se {r.err3 / 3 / r.value:.3e})", flush=True)
     assert rel(v, r.value) < 2e-5
+
+
+@pytest.mark.parametrize("d,m,skip", [(150, 1500, None), (300, 1000, 3 * 2 ** 20 + 77),
+                                      (120, 700, 2 ** 31 - 5000)])
+def test_sobol_fp64_mfma(dbg, oracle, d, m, skip):
+    """FP64 Sobol on the matrix-core kernel, whose walk takes its points from
+    per-sub-block LDS tables (low Gray-code byte per lane, high part per tile):
+    equal to the oracle's direct XOR of direction integers.  Tiles straddle
+    256-point boundaries (m and the skips are not multiples of 256), and the
+    large skips set high bits of the index."""
+    P = W.dense_random(d, m=m, n_reps=2, seed=d, gen="Sobol")
+    print(f"sobol fp64 mfma d={d} m={m} skip={skip}", flush=True)
+    opts = dbg.QMC_opts(np.float64, m=P.m, n_reps=P.n_reps, max_pts=P.m, max_abs_err=0.0)
+    D = dbg.QMCData(P.Sigma, P.a, P.b, opts, None, "Sobol", shifts=P.shifts, sobol_skip=skip)
+    v, _, _ = dbg.qmc_pnorm(D)
+    r = oracle_same_inputs(oracle, D)
+    assert rel(v, r.value) < F64_TOL, (v, r.value)
+    assert np.allclose(D.qmc_reps, r.rep_means, rtol=F64_TOL, atol=0)
