@@ -35,14 +35,19 @@ struct MfmaCfg;
 // rings of the panel contraction (Y rows stream from HBM, U rows are shared by
 // every workgroup and come from L2); OCC: waves per SIMD the register budget
 // is cut for.
+// ULDS: U rows of the panel contraction go through LDS in chunks of kUC rows,
+// loaded once per workgroup (instead of once per wavefront) one chunk ahead.
 template <>
 struct MfmaCfg<true> {   // FP32: 256 samples x 128 targets, 128 accumulators per lane
     static constexpr int TS = 256, PW = 128, PFY = 4, PFU = 2, OCC = 2;
+    static constexpr bool ULDS = true;
 };
 template <>
 struct MfmaCfg<false> {  // FP64: 128 samples x 128 targets, 128 accumulator registers per lane
     static constexpr int TS = 128, PW = 128, PFY = 4, PFU = 4, OCC = 1;
+    static constexpr bool ULDS = false;
 };
+constexpr int kUC = 16;  // U rows per LDS chunk (four K-steps)
 
 // Sample order inside a wavefront's WS samples: the matrix-core row lr of
 // sample block sb is sample NSB * lr + sb, so the NSB operands a lane needs
@@ -110,6 +115,9 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
     __shared__ T s_x[TS * kSXS];       // accumulator hand-off, [sample][16]
     __shared__ T s_y[kSB * TS];        // y of the current sub-block, [k][sample]
     __shared__ double s_red[REV ? 4 * kSB * 3 : 1];
+    constexpr bool ULDS = MfmaCfg<MIXED>::ULDS;
+    constexpr int kUS = PW + 16;       // LDS row stride of a U chunk (conflict-free reads)
+    __shared__ T s_u[ULDS ? 2 * kUC * kUS : 1];
     constexpr bool SOBT = GEN == kGenSobol && !REV;  // walk points from LDS Sobol tables
     __shared__ uint32_t s_sob[SOBT ? kSB * 256 + 2 * kSB : 1];
 
@@ -156,7 +164,58 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
             for (int sb = 0; sb < NSB; ++sb)
 #pragma unroll
                 for (int tb = 0; tb < NTB; ++tb) acc[sb][tb] = V{0, 0, 0, 0};
-            {
+            if constexpr (ULDS) {
+                // Y rows: a ring of kPFY K-steps in registers (private to the
+                // wavefront).  U rows: chunks of kUC rows x PW targets in LDS,
+                // double-buffered; each thread stages 8 targets of one row of
+                // the next chunk in registers while the current chunk runs.
+                static_assert(PW == 128 && kUC == 16, "staging: 256 threads x 8 = kUC x PW");
+                static_assert(4 % kPFY == 0, "Y ring turns once per chunk");
+                const T* __restrict__ ya = Ys + (size_t)lq * ys + wv * WS + NSB * lr;
+                const int srow = tid >> 4, scol = (tid & 15) * 8;
+                const T* __restrict__ us = Uk + (size_t)srow * ldu + g0 + 1 + scol;
+                T* __restrict__ sw = s_u + srow * kUS + scol;
+                const T* __restrict__ sr = s_u + lq * kUS + lr;
+                T av[kPFY][NSB], stg[8];
+                const int nch = g0 / kUC;
+                if (nch > 0) {
+#pragma unroll
+                    for (int q = 0; q < kPFY - 1; ++q) load_y<T, NSB>(av[q], ya, ys, 4 * q);
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) stg[e] = us[e];
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) sw[e] = stg[e];
+                    __syncthreads();
+                }
+                for (int c = 0; c < nch; ++c) {
+                    const T* __restrict__ sb_u = sr + (c & 1) * (kUC * kUS);
+                    if (c + 1 < nch) {  // uniform
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) stg[e] = us[(size_t)(c + 1) * kUC * ldu + e];
+                    }
+#pragma unroll
+                    for (int ks = 0; ks < 4; ++ks) {
+                        const int k0 = (4 * c + ks) * 4;
+                        load_y<T, NSB>(av[(ks + kPFY - 1) % kPFY], ya, ys, min(k0 + 4 * (kPFY - 1), g0 - 4));
+                        T bv[NTB];
+#pragma unroll
+                        for (int tb = 0; tb < NTB; ++tb) bv[tb] = sb_u[4 * ks * kUS + 16 * tb];
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int sb = 0; sb < NSB; ++sb)
+#pragma unroll
+                            for (int tb = 0; tb < NTB; ++tb)
+                                acc[sb][tb] = Mfma<T>::mac(av[ks % kPFY][sb], bv[tb], acc[sb][tb]);
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
+                    if (c + 1 < nch) {
+                        T* __restrict__ swb = sw + ((c + 1) & 1) * (kUC * kUS);
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) swb[e] = stg[e];
+                    }
+                    __syncthreads();
+                }
+            } else {
                 // operands of K-step k0 (4 dimensions) are loaded kPFY - 1 (Y)
                 // and kPFU - 1 (U) steps ahead into rings of register buffers,
                 // so the loads of later steps are in flight while the matrix
