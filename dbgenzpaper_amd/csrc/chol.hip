@@ -99,6 +99,26 @@ hipError_t chol_standardize(const CholArgs& C, bool mirror, hipStream_t s)
     return hipGetLastError();
 }
 
+// acc - sum_{l = l0}^{l1 - 1} M[l, x] M[l, y], subtracted in ascending l (the
+// order of the reference's loops); the loads go out in batches of eight so a
+// batch costs one memory round trip
+__device__ __forceinline__ double sub_dot(const double* M, int64_t lda, double acc, int l0, int l1, int x, int y)
+{
+    int l = l0;
+    for (; l + 8 <= l1; l += 8) {
+        double a[8], b[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            a[q] = MM(l + q, x);
+            b[q] = MM(l + q, y);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc -= a[q] * b[q];
+    }
+    for (; l < l1; ++l) acc -= MM(l, x) * MM(l, y);
+    return acc;
+}
+
 // argmin key: NaN before any number, then smaller value, then smaller index
 __device__ __forceinline__ bool better(double v, int i, double bv, int bi)
 {
@@ -250,8 +270,7 @@ __device__ __forceinline__ void chol_window_body(const CholArgs& C, int w0, int 
             if (tid == 0) MM(i, i) = uii;
             // ---- row i of U inside the window (src/cholesky.jl:547-570)
             for (int c = i + 1 + tid; c < w1; c += kCholThreads) {
-                double acc = MM(i, c);
-                for (int l = ib0; l < i; ++l) acc -= MM(l, i) * MM(l, c);
+                const double acc = sub_dot(M, lda, MM(i, c), ib0, i, i, c);
                 const double u = acc / uii;
                 MM(i, c) = u;
                 s_dsum[c - w0] += u * u;
@@ -266,11 +285,7 @@ __device__ __forceinline__ void chol_window_body(const CholArgs& C, int w0, int 
             for (int64_t idx = tid; idx < (int64_t)wr * wr; idx += kCholThreads) {
                 const int x = ib1 + (int)(idx % wr);
                 const int y = ib1 + (int)(idx / wr);
-                if (x <= y) {
-                    double acc = MM(x, y);
-                    for (int l = ib0; l < ib1; ++l) acc -= MM(l, x) * MM(l, y);
-                    MM(x, y) = acc;
-                }
+                if (x <= y) MM(x, y) = sub_dot(M, lda, MM(x, y), ib0, ib1, x, y);
             }
             __syncthreads();
         }
