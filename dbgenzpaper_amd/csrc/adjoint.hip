@@ -183,6 +183,109 @@ __global__ __launch_bounds__(256) void ubar_gemm_kernel(const T* __restrict__ Y,
                 out[(size_t)(wr * 64 + 16 * i + Mfma<T>::row(lq, r)) * kUbTile + wc * 64 + 16 * j + lr] = acc[i][j][r];
 }
 
+// FP32 form of the same product with both operand tiles staged in LDS: per
+// stage of kUbK samples each thread loads 16 samples of one A row and one B
+// row (four 16-byte loads each) for the next stage while the four waves run
+// the current one from LDS, so every operand byte is fetched once per
+// workgroup instead of once per wave.  Chunk lengths are multiples of kUbK.
+constexpr int kUbK = 32;
+constexpr int kUbKS = kUbK + 4;  // LDS row stride (floats)
+
+__global__ __launch_bounds__(256, 2) void ubar_gemm_lds_kernel(const float* __restrict__ Y,
+                                                               const float* __restrict__ S, int64_t rs, int nd,
+                                                               const int2* __restrict__ pairs, int n_pairs,
+                                                               int64_t kchunk, int64_t kmax,
+                                                               float* __restrict__ part)
+{
+    using V = Mfma<float>::V;
+    using L4 = Vec4<float>::V;
+    __shared__ float s_a[2][kUbTile * kUbKS];
+    __shared__ float s_b[2][kUbTile * kUbKS];
+    const int pair = blockIdx.x, kc = blockIdx.y;
+    const int2 tp = pairs[pair];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    const int wr = wv & 1, wc = wv >> 1;
+    const int64_t k0 = (int64_t)kc * kchunk;
+    const int64_t k1 = min(kmax, k0 + kchunk);
+    // staging: thread -> tile row tid / 2, samples 16 (tid % 2) .. + 15 of a stage
+    const int srow = tid >> 1, shalf = (tid & 1) * 16;
+    const int la = min(tp.x * kUbTile + srow, nd - 1);  // rows past nd read row nd-1; discarded
+    const int lb = min(tp.y * kUbTile + srow, nd - 1);
+    const float* __restrict__ ga = Y + (size_t)la * rs + shalf;
+    const float* __restrict__ gb = S + (size_t)lb * rs + shalf;
+    float* __restrict__ wa = &s_a[0][srow * kUbKS + shalf];
+    float* __restrict__ wb = &s_b[0][srow * kUbKS + shalf];
+    V acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = V{0, 0, 0, 0};
+    const int nst = (int)((k1 - k0) / kUbK);
+    L4 sa[4], sb[4];
+    if (nst > 0) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            sa[e] = *(const L4*)(ga + k0 + 4 * e);
+            sb[e] = *(const L4*)(gb + k0 + 4 * e);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            *(L4*)(wa + 4 * e) = sa[e];
+            *(L4*)(wb + 4 * e) = sb[e];
+        }
+        __syncthreads();
+    }
+    for (int st = 0; st < nst; ++st) {
+        const int buf = st & 1;
+        if (st + 1 < nst) {  // uniform
+            const int64_t kn = k0 + (int64_t)(st + 1) * kUbK;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                sa[e] = *(const L4*)(ga + kn + 4 * e);
+                sb[e] = *(const L4*)(gb + kn + 4 * e);
+            }
+        }
+        const float* __restrict__ ra = &s_a[buf][(wr * 64 + lr) * kUbKS + 4 * lq];
+        const float* __restrict__ rb = &s_b[buf][(wc * 64 + lr) * kUbKS + 4 * lq];
+#pragma unroll
+        for (int h = 0; h < kUbK / 16; ++h) {
+            L4 a[4], b[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                a[q] = *(const L4*)(ra + 16 * q * kUbKS + 16 * h);
+                b[q] = *(const L4*)(rb + 16 * q * kUbKS + 16 * h);
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<float>::mac(a[i][c], b[j][c], acc[i][j]);
+        }
+        if (st + 1 < nst) {
+            const int nb = (st + 1) & 1;
+            float* __restrict__ wan = &s_a[nb][srow * kUbKS + shalf];
+            float* __restrict__ wbn = &s_b[nb][srow * kUbKS + shalf];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                *(L4*)(wan + 4 * e) = sa[e];
+                *(L4*)(wbn + 4 * e) = sb[e];
+            }
+        }
+        __syncthreads();
+    }
+    float* __restrict__ out = part + ((size_t)kc * n_pairs + pair) * (kUbTile * kUbTile);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                out[(size_t)(wr * 64 + 16 * i + Mfma<float>::row(lq, r)) * kUbTile + wc * 64 + 16 * j + lr] =
+                    acc[i][j][r];
+}
+
 // Ub[l, g + 1] += sum over chunks of the partial tiles (l <= g < nd), FP64
 template <typename T>
 __global__ __launch_bounds__(256) void ubar_reduce_kernel(const T* __restrict__ part, const int2* __restrict__ pairs,
@@ -207,7 +310,8 @@ hipError_t ubar_product(const void* Y, const void* S, int64_t rs, int nd, bool m
     dim3 g1(n_pairs, n_chunks), b(256);
     dim3 g2(n_pairs, 16);
     if (mixed) {
-        hipLaunchKernelGGL(ubar_gemm_kernel<float>, g1, b, 0, s, (const float*)Y, (const float*)S, rs, nd, pairs,
+        if (kchunk % kUbK != 0 || rs % kUbK != 0) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(ubar_gemm_lds_kernel, g1, b, 0, s, (const float*)Y, (const float*)S, rs, nd, pairs,
                            n_pairs, kchunk, rs, (float*)part);
         hipLaunchKernelGGL(ubar_reduce_kernel<float>, g2, b, 0, s, (const float*)part, pairs, n_pairs, n_chunks,
                            nd, Ub, n);
