@@ -669,6 +669,7 @@ __device__ __forceinline__ void walk_block(const PointArgs& pts, int gb, int cnt
 // of y_r are absent for r = d - 1, and s_0 = 0.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double wave_sum_f64(double v);
+__device__ __forceinline__ float wave_sum_f32(float v);
 
 template <typename T>
 __device__ __forceinline__ T scaled_pdf(T x, int k)
@@ -739,20 +740,44 @@ __device__ __forceinline__ void walk_block_rev(const PointArgs& pts, int n, int 
                                                double* __restrict__ red)
 {
     T sv[SB];
+    // the records of a step are read one step ahead (they stream from HBM);
+    // step cc overwrites record row r - 1 only after reading it, and the
+    // next step reads row r - 2
+    T s_nx = T(0), y_nx = T(0);
+    if (cnt > 0) {
+        const int r0 = n - 1 - gb;
+        s_nx = srec[(size_t)(r0 - 1) * rs + t];  // r >= 1 here
+        y_nx = (r0 <= n - 2) ? yrec[(size_t)r0 * rs + t] : T(0);
+    }
 #pragma unroll
     for (int cc = 0; cc < SB; ++cc) {
         if (cc < cnt) {  // uniform
             const int g = gb + cc;
             const int r = n - 1 - g;
-            const T s_r = srec[(size_t)(r - 1) * rs + t];   // r >= 1 here
-            const T y_r = (r <= n - 2) ? yrec[(size_t)r * rs + t] : T(0);
+            const T s_r = s_nx;
+            const T y_r = y_nx;
+            if (cc + 1 < cnt) {
+                s_nx = srec[(size_t)(r - 2) * rs + t];
+                y_nx = yrec[(size_t)(r - 1) * rs + t];  // r - 1 <= n - 2
+            }
             RevOut o;
             const T sbar = rev_step<MIXED, GEN>(pts, n, r, k, j, ybar, wt, s_r, y_r, ak[r], bk[r], invk[r], o);
             sv[cc] = sbar;
             s_y[cc * TS + t] = sbar;
             Ys[(size_t)g * TS + t] = sbar;
             srec[(size_t)(r - 1) * rs + t] = sbar;
-            const double a0 = wave_sum_f64(o.ga), b0 = wave_sum_f64(o.gb), d0 = wave_sum_f64(o.gd);
+            // per-index sums over the wavefront: FP32 partials of 64 FP32
+            // terms on the mixed path, FP64 on the FP64 path
+            double a0, b0, d0;
+            if constexpr (MIXED) {
+                a0 = wave_sum_f32((float)o.ga);
+                b0 = wave_sum_f32((float)o.gb);
+                d0 = wave_sum_f32((float)o.gd);
+            } else {
+                a0 = wave_sum_f64(o.ga);
+                b0 = wave_sum_f64(o.gb);
+                d0 = wave_sum_f64(o.gd);
+            }
             if ((t & 63) == 0) {
                 double* rr = red + ((t >> 6) * SB + cc) * 3;
                 rr[0] = a0;
@@ -806,6 +831,13 @@ __device__ __forceinline__ int wave_max_int(int v)
 {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ float wave_sum_f32(float v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
 
