@@ -6,6 +6,7 @@
 
 #include <string>
 #include <algorithm>
+#include <numeric>
 #include <vector>
 
 #include <rocblas/rocblas.h>
@@ -529,15 +530,18 @@ int grad_alloc(dbg_plan* P)
     HIPCHK(dalloc(&G.ufc, (size_t)n * n * T + 256 * T));
     {
         // split-K layout of Ubar = Y^T Sbar: upper-triangle pairs of 128-row tiles,
-        // enough sample chunks for about twelve workgroups per CU (three resident
-        // per CU, four rounds: a short last round)
+        // about twelve workgroups per CU in whole rounds of the two resident
+        // per CU (pairs x chunks a multiple of 2 ncu: no short last round)
         const int tb = ubar_tile();
         const int nt = (nd + tb - 1) / tb;
         std::vector<int2> pr;
         for (int ti = 0; ti < nt; ++ti)
             for (int tj = ti; tj < nt; ++tj) pr.push_back(make_int2(ti, tj));
         G.n_pairs = (int)pr.size();
-        const int want = std::max(1, (12 * ncu + G.n_pairs - 1) / G.n_pairs);
+        const int slots = 2 * ncu;
+        const int step = slots / std::gcd(G.n_pairs, slots);
+        int want = std::max(1, (12 * ncu + G.n_pairs - 1) / G.n_pairs);
+        want = (want + step - 1) / step * step;
         G.kchunk = std::max<int64_t>(32, ((G.rs + want - 1) / want + 31) / 32 * 32);
         G.n_chunks = (int)((G.rs + G.kchunk - 1) / G.kchunk);
         HIPCHK(dalloc(&G.pairs, pr.size() * sizeof(int2)));
