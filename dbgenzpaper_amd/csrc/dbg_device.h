@@ -834,11 +834,26 @@ __device__ __forceinline__ int wave_max_int(int v)
     return v;
 }
 
+// Sum over the 64 lanes with DPP moves (vector ALU only, no LDS round trips):
+// row_shr 1, 2, 4, 8 leave each 16-lane row's sum in its lane 15 (lanes
+// without a source read 0), row_bcast 15 / 31 carry rows 0 and 0-1 into rows
+// 1, 3 and 2-3, so lane 63 ends with the total, which is then broadcast.
+template <int CTRL, int ROW_MASK, bool BOUND_ZERO>
+__device__ __forceinline__ float dpp_f32(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROW_MASK,
+                                                                 0xF, BOUND_ZERO));
+}
+
 __device__ __forceinline__ float wave_sum_f32(float v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v += dpp_f32<0x111, 0xF, true>(v);  // row_shr:1
+    v += dpp_f32<0x112, 0xF, true>(v);  // row_shr:2
+    v += dpp_f32<0x114, 0xF, true>(v);  // row_shr:4
+    v += dpp_f32<0x118, 0xF, true>(v);  // row_shr:8
+    v += dpp_f32<0x142, 0xA, false>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_f32<0x143, 0xC, false>(v);  // row_bcast:31 into rows 2 and 3
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
 }
 
 __device__ __forceinline__ double wave_sum_f64(double v)
