@@ -686,8 +686,8 @@ struct RevOut {
 };
 
 template <bool MIXED, int GEN, typename T = typename std::conditional<MIXED, float, double>::type>
-__device__ __forceinline__ T rev_step(const PointArgs& pts, int n, int r, int k, int64_t j, T ybar, T wt,
-                                      T s_r, T y_r, T a_r, T b_r, T inv_r, RevOut& o)
+__device__ __forceinline__ T rev_step(const PointArgs& pts, const PointOp& op, int n, int r, int64_t j, T ybar,
+                                      T wt, T s_r, T y_r, T a_r, T b_r, T inv_r, RevOut& o)
 {
     const T za = (a_r - s_r) * inv_r, zb = (b_r - s_r) * inv_r;
     typename Path<MIXED, kPhiTail>::State st;
@@ -700,10 +700,10 @@ __device__ __forceinline__ T rev_step(const PointArgs& pts, int n, int r, int k,
         T w;
         if constexpr (MIXED) {
             float wf, wc;
-            tent_w(point_u32<GEN>(pts, r, k, j), wf, wc);
+            tent_w(point_u32<GEN>(pts, op, r, j), wf, wc);
             w = wf;
         } else {
-            w = fabs(fma(point_u<GEN>(pts, r, k, j), 2.0, -1.0));
+            w = fabs(fma(point_u<GEN>(pts, op, r, j), 2.0, -1.0));
         }
         const T ph = scaled_pdf(y_r, st.k);
         const T iy = ph > T(0) ? ybar / ph : T(0);
@@ -719,6 +719,97 @@ __device__ __forceinline__ T rev_step(const PointArgs& pts, int n, int r, int k,
     const T tb = isinf(be) ? T(0) : bebar * be;
     o.gd = (double)(-(ta + tb) * invU);
     return -(albar + bebar) * invU;
+}
+
+template <bool MIXED, int GEN, typename T = typename std::conditional<MIXED, float, double>::type>
+__device__ __forceinline__ T rev_step(const PointArgs& pts, int n, int r, int k, int64_t j, T ybar, T wt,
+                                      T s_r, T y_r, T a_r, T b_r, T inv_r, RevOut& o)
+{
+    const PointOp op = (r <= n - 2) ? point_op<MIXED, GEN>(pts, r, k) : PointOp{};
+    return rev_step<MIXED, GEN>(pts, op, n, r, j, ybar, wt, s_r, y_r, a_r, b_r, inv_r, o);
+}
+
+// Scalar operands of one reverse step (flipped index g = gb + CC, index
+// r = n - 1 - g): the point-table entries of r, the bounds of r and column
+// g + 1 of the flipped factor from row gb to g.
+template <bool MIXED, int GEN, int SB, typename T>
+struct RevOps {
+    PointOp pt;
+    T u[SB];
+    T a, b, inv;
+
+    template <int CC>
+    __device__ __forceinline__ void fetch(const PointArgs& pts, int n, int gb, int k, const T* __restrict__ Uf,
+                                          int64_t ldc, const T* __restrict__ ak, const T* __restrict__ bk,
+                                          const T* __restrict__ invk)
+    {
+        const int g = gb + CC;
+        const int r = n - 1 - g;
+        pt = (r <= n - 2) ? point_op<MIXED, GEN>(pts, r, k) : PointOp{};
+        const T* __restrict__ ucol = Uf + (size_t)(g + 1) * ldc + gb;
+#pragma unroll
+        for (int l = 0; l <= CC; ++l) u[l] = ucol[l];
+        a = ak[r];
+        b = bk[r];
+        inv = invk[r];
+    }
+};
+
+template <bool MIXED, int GEN, int TS, int SXS, int SB, int CC, typename T>
+__device__ __forceinline__ void rev_steps(const PointArgs& pts, int n, int gb, int cnt, int k, int64_t j, int t,
+                                          T& ybar, T wt, const T* __restrict__ s_x, T* __restrict__ s_y,
+                                          T* __restrict__ Ys, const T* __restrict__ Uf, int64_t ldc,
+                                          const T* __restrict__ ak, const T* __restrict__ bk,
+                                          const T* __restrict__ invk, T* __restrict__ srec,
+                                          const T* __restrict__ yrec, int64_t rs, double* __restrict__ red,
+                                          T (&sv)[SB], T& s_nx, T& y_nx, RevOps<MIXED, GEN, SB, T> (&ops)[2])
+{
+    if constexpr (CC < SB) {
+        if (CC < cnt) {  // uniform
+            const int g = gb + CC;
+            const int r = n - 1 - g;
+            const T s_r = s_nx;
+            const T y_r = y_nx;
+            const RevOps<MIXED, GEN, SB, T>& cur = ops[CC & 1];
+            if constexpr (CC + 1 < SB) {
+                if (CC + 1 < cnt) {
+                    s_nx = srec[(size_t)(r - 2) * rs + t];
+                    y_nx = yrec[(size_t)(r - 1) * rs + t];  // r - 1 <= n - 2
+                    ops[(CC + 1) & 1].template fetch<CC + 1>(pts, n, gb, k, Uf, ldc, ak, bk, invk);
+                }
+            }
+            RevOut o;
+            const T sbar = rev_step<MIXED, GEN>(pts, cur.pt, n, r, j, ybar, wt, s_r, y_r, cur.a, cur.b, cur.inv, o);
+            sv[CC] = sbar;
+            s_y[CC * TS + t] = sbar;
+            Ys[(size_t)g * TS + t] = sbar;
+            srec[(size_t)(r - 1) * rs + t] = sbar;
+            // per-index sums over the wavefront: FP32 partials of 64 FP32
+            // terms on the mixed path, FP64 on the FP64 path
+            double a0, b0, d0;
+            if constexpr (MIXED) {
+                a0 = wave_sum_f32((float)o.ga);
+                b0 = wave_sum_f32((float)o.gb);
+                d0 = wave_sum_f32((float)o.gd);
+            } else {
+                a0 = wave_sum_f64(o.ga);
+                b0 = wave_sum_f64(o.gb);
+                d0 = wave_sum_f64(o.gd);
+            }
+            if ((t & 63) == 0) {
+                double* rr = red + ((t >> 6) * SB + CC) * 3;
+                rr[0] = a0;
+                rr[1] = b0;
+                rr[2] = d0;
+            }
+            T acc = s_x[t * SXS + CC];
+#pragma unroll
+            for (int l = 0; l <= CC; ++l) acc = fma(sv[l], cur.u[l], acc);
+            ybar = acc;
+            rev_steps<MIXED, GEN, TS, SXS, SB, CC + 1>(pts, n, gb, cnt, k, j, t, ybar, wt, s_x, s_y, Ys, Uf, ldc, ak,
+                                                       bk, invk, srec, yrec, rs, red, sv, s_nx, y_nx, ops);
+        }
+    }
 }
 
 // Reverse walk of flipped indices [gb, gb + cnt) (flipped index g is r =
@@ -742,55 +833,18 @@ __device__ __forceinline__ void walk_block_rev(const PointArgs& pts, int n, int 
     T sv[SB];
     // the records of a step are read one step ahead (they stream from HBM);
     // step cc overwrites record row r - 1 only after reading it, and the
-    // next step reads row r - 2
+    // next step reads row r - 2.  The scalar operands are fetched one step
+    // ahead as well.
     T s_nx = T(0), y_nx = T(0);
+    RevOps<MIXED, GEN, SB, T> ops[2];
     if (cnt > 0) {
         const int r0 = n - 1 - gb;
         s_nx = srec[(size_t)(r0 - 1) * rs + t];  // r >= 1 here
         y_nx = (r0 <= n - 2) ? yrec[(size_t)r0 * rs + t] : T(0);
+        ops[0].template fetch<0>(pts, n, gb, k, Uf, ldc, ak, bk, invk);
     }
-#pragma unroll
-    for (int cc = 0; cc < SB; ++cc) {
-        if (cc < cnt) {  // uniform
-            const int g = gb + cc;
-            const int r = n - 1 - g;
-            const T s_r = s_nx;
-            const T y_r = y_nx;
-            if (cc + 1 < cnt) {
-                s_nx = srec[(size_t)(r - 2) * rs + t];
-                y_nx = yrec[(size_t)(r - 1) * rs + t];  // r - 1 <= n - 2
-            }
-            RevOut o;
-            const T sbar = rev_step<MIXED, GEN>(pts, n, r, k, j, ybar, wt, s_r, y_r, ak[r], bk[r], invk[r], o);
-            sv[cc] = sbar;
-            s_y[cc * TS + t] = sbar;
-            Ys[(size_t)g * TS + t] = sbar;
-            srec[(size_t)(r - 1) * rs + t] = sbar;
-            // per-index sums over the wavefront: FP32 partials of 64 FP32
-            // terms on the mixed path, FP64 on the FP64 path
-            double a0, b0, d0;
-            if constexpr (MIXED) {
-                a0 = wave_sum_f32((float)o.ga);
-                b0 = wave_sum_f32((float)o.gb);
-                d0 = wave_sum_f32((float)o.gd);
-            } else {
-                a0 = wave_sum_f64(o.ga);
-                b0 = wave_sum_f64(o.gb);
-                d0 = wave_sum_f64(o.gd);
-            }
-            if ((t & 63) == 0) {
-                double* rr = red + ((t >> 6) * SB + cc) * 3;
-                rr[0] = a0;
-                rr[1] = b0;
-                rr[2] = d0;
-            }
-            T acc = s_x[t * SXS + cc];
-            const T* __restrict__ ucol = Uf + (size_t)(g + 1) * ldc + gb;
-#pragma unroll
-            for (int l = 0; l <= cc; ++l) acc = fma(sv[l], ucol[l], acc);
-            ybar = acc;
-        }
-    }
+    rev_steps<MIXED, GEN, TS, SXS, SB, 0>(pts, n, gb, cnt, k, j, t, ybar, wt, s_x, s_y, Ys, Uf, ldc, ak, bk, invk,
+                                          srec, yrec, rs, red, sv, s_nx, y_nx, ops);
 }
 
 // ---------------------------------------------------------------------------
