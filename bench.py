@@ -39,7 +39,7 @@ PEAK_FP64_TFLOPS = 78.6
 # (d, N, shifts): rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (factors from
 # a plain 1 GiB device copy), profiles/dense_d2500_mixed_counters.md.  Other
 # shapes report null.
-TRAFFIC_BYTES = {(2500, 2 ** 18, 10): 2.868e11}
+TRAFFIC_BYTES = {(2500, 2 ** 18, 10): 2.839e11}
 
 
 def _log(msg):
@@ -60,6 +60,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-grad", action="store_true", help="skip the gradient leg (d = 1000 factor model)")
+    ap.add_argument("--grad-only", action="store_true",
+                    help="only run the gradient leg and print its record (for profiling it alone)")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the other configurations (d = 16, d = 500, Sobol at d = 2500, banded d = 4000)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -78,6 +80,10 @@ def main() -> int:
 
     import dbgenzpaper_amd as dbg
     from dbgenzpaper_amd import workloads as W
+
+    if args.grad_only:
+        print(json.dumps({"adjoint": grad_leg(args.steps)}), flush=True)
+        return 0
 
     d, m, R = args.d, args.m, args.reps
     _log(f"workload factor_model d={d} k=8 m={m} reps={R}")
