@@ -39,6 +39,24 @@ __device__ __attribute__((noinline)) double dbg_erfcinv(double x) { return erfci
 __device__ __forceinline__ float dbg_erfcf(float x) { return erfcf(x); }
 __device__ __attribute__((noinline)) float dbg_erfcinvf(float x) { return erfcinvf(x); }
 __device__ __forceinline__ float dbg_tail_quantile(float t, int k) { return tail_quantile_f32(t, k); }
+// the same two behind call boundaries, for the sparse walk (a short loop body
+// that is measured faster with calls)
+__device__ __attribute__((noinline)) float dbg_erfcf_call(float x) { return erfcf(x); }
+__device__ __attribute__((noinline)) float dbg_tail_quantile_call(float t, int k) { return tail_quantile_f32(t, k); }
+
+template <bool CALLS>
+__device__ __forceinline__ float erfcf_sel(float x)
+{
+    if constexpr (CALLS) return dbg_erfcf_call(x);
+    else return dbg_erfcf(x);
+}
+
+template <bool CALLS>
+__device__ __forceinline__ float tail_quantile_sel(float t, int k)
+{
+    if constexpr (CALLS) return dbg_tail_quantile_call(t, k);
+    else return dbg_tail_quantile(t, k);
+}
 
 // ---------------------------------------------------------------------------
 // Point generators (src/qmc_generators.jl)
@@ -364,6 +382,7 @@ __device__ __forceinline__ void tail_from_z(StateTail<double>& st, double za, do
     tail_set64(ea, eb, dc, mode, st);
 }
 
+template <bool CALLS = false>
 __device__ __forceinline__ void tail_from_z(StateTail<float>& st, float za, float zb)
 {
     const bool ia = isinf(za), ib = isinf(zb);
@@ -371,8 +390,8 @@ __device__ __forceinline__ void tail_from_z(StateTail<float>& st, float za, floa
         st.k = 0;
         st.ea = 0.0f;
         st.eb = 0.0f;
-        if (!ia) st.ea = 0.5f * dbg_erfcf(fabsf(za));
-        if (!ib) st.eb = 0.5f * dbg_erfcf(fabsf(zb));
+        if (!ia) st.ea = 0.5f * erfcf_sel<CALLS>(fabsf(za));
+        if (!ib) st.eb = 0.5f * erfcf_sel<CALLS>(fabsf(zb));
         if (zb <= 0.0f) {
             st.mode = 0;
             st.dc = st.eb - st.ea;
@@ -403,13 +422,14 @@ __device__ __forceinline__ double tail_sample_w(const StateTail<double>& st, dou
 
 // FP32 sample from the fixed-point point: w and 1 - w as FP32 values, the
 // quantile of the nearer tail mass t * 2^-k in FP32 (log-domain, any k)
+template <bool CALLS = false>
 __device__ __forceinline__ float tail_sample_f(const StateTail<float>& st, float w, float wc)
 {
     const float u_lo = fmaf(w, st.dc, st.ea);
     const float u_hi = fmaf(wc, st.dc, st.eb);
     const bool upper = (st.mode == 1) || (st.mode == 2 && u_hi < u_lo);
     const float t = fmaxf(upper ? u_hi : u_lo, 1.17549435e-38f);
-    const float y = dbg_tail_quantile(t, st.k);
+    const float y = tail_quantile_sel<CALLS>(t, st.k);
     return upper ? -y : y;
 }
 

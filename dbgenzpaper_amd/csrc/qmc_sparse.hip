@@ -69,7 +69,7 @@ __global__ __launch_bounds__(kTileS) void sparse_kernel(SparseArgs A, DBG_POINT_
                 float wf, wc;
                 if (A.tent) tent_w(u, wf, wc);
                 else plain_w(u, wf, wc);
-                y = tail_sample_f(st, wf, wc);
+                y = tail_sample_f<true>(st, wf, wc);
             } else {
                 const double u = point_u<GEN>(pts, g, k, j);
                 y = A.tent ? path_sample<MIXED, FORM>(st, u) : PT::sample_w(st, u);
@@ -85,7 +85,10 @@ __global__ __launch_bounds__(kTileS) void sparse_kernel(SparseArgs A, DBG_POINT_
                 else yr = Ys[(size_t)r * kTileS + tid];
                 sv = fma(yr, nzval[p], sv);
             }
-            PT::phi(st, sv, ak[g + 1], bk[g + 1], invk[g + 1]);
+            if constexpr (MIXED)  // Path<true>::phi with the special functions as calls
+                tail_from_z<true>(st, (ak[g + 1] - sv) * invk[g + 1], (bk[g + 1] - sv) * invk[g + 1]);
+            else
+                PT::phi(st, sv, ak[g + 1], bk[g + 1], invk[g + 1]);
             PT::weigh(w, st);
         }
         const double pm = valid ? (double)w.pm : 0.0;
