@@ -206,6 +206,10 @@ __global__ __launch_bounds__(256, 2) void ubar_gemm_lds_kernel(const float* __re
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int lr = lane & 15, lq = lane >> 4;
     const int wr = wv & 1, wc = wv >> 1;
+    // on a diagonal tile pair the wave of rows 64-127 x columns 0-63 lies below
+    // the diagonal (l > g): it stages operands but runs no products, and its
+    // part of the partial tile is never read
+    const bool idle = tp.x == tp.y && wr > wc;
     const int64_t k0 = (int64_t)kc * kchunk;
     const int64_t k1 = min(kmax, k0 + kchunk);
     // staging: thread -> tile row tid / 2, samples 16 (tid % 2) .. + 15 of a stage
@@ -250,6 +254,7 @@ __global__ __launch_bounds__(256, 2) void ubar_gemm_lds_kernel(const float* __re
         const float* __restrict__ rb = &s_b[buf][(wc * 64 + lr) * kUbKS + 4 * lq];
 #pragma unroll
         for (int h = 0; h < kUbK / 16; ++h) {
+            if (idle) break;  // uniform
             L4 a[4], b[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -275,6 +280,7 @@ __global__ __launch_bounds__(256, 2) void ubar_gemm_lds_kernel(const float* __re
         }
         __syncthreads();
     }
+    if (idle) return;
     float* __restrict__ out = part + ((size_t)kc * n_pairs + pair) * (kUbTile * kUbTile);
 #pragma unroll
     for (int i = 0; i < 4; ++i)

@@ -223,6 +223,7 @@ struct dbg_plan {
     int* reach_d = nullptr;  // pattern reach (largest g - r read by column g + 1)
     int ring = 0;            // LDS ring rows of the sparse kernel, 0: global workspace
     int ring_grid = 0;       // resident blocks of that ring kernel
+    bool a_inf = false;      // every lower bound is -inf
     // points
     double* q = nullptr;
     double* shifts_f = nullptr;
@@ -357,6 +358,8 @@ int plan_build(dbg_plan* P, const double* Sigma, int64_t lda, const double* a, c
     HIPCHK(hipMemcpy2DAsync(P->sig, (size_t)n * sizeof(double), Sigma, (size_t)lda * sizeof(double),
                             (size_t)n * sizeof(double), n, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(P->a0, a, n * sizeof(double), hipMemcpyHostToDevice, s));
+    P->a_inf = true;
+    for (int i = 0; i < n; ++i) P->a_inf = P->a_inf && std::isinf(a[i]) && a[i] < 0;
     HIPCHK(hipMemcpyAsync(P->b0, b, n * sizeof(double), hipMemcpyHostToDevice, s));
 
     const int nd = P->nd;
@@ -608,6 +611,7 @@ int run_grad(dbg_plan* P)
     HIPCHK(grad_norm(G.part_m, G.part_e, G.n_tiles, (double)o.m * o.n_reps, G.norm, s));
     // reverse sweep in flipped index order
     A.mode = kModeReverse;
+    A.a_inf = P->a_inf ? 1 : 0;
     A.Uk = G.ufk;
     A.Ucol = G.ufc;
     A.Y = G.ywork;

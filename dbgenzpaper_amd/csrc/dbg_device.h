@@ -782,7 +782,8 @@ __device__ __forceinline__ void rev_steps(const PointArgs& pts, int n, int gb, i
                                           const T* __restrict__ ak, const T* __restrict__ bk,
                                           const T* __restrict__ invk, T* __restrict__ srec,
                                           const T* __restrict__ yrec, int64_t rs, double* __restrict__ red,
-                                          T (&sv)[SB], T& s_nx, T& y_nx, RevOps<MIXED, GEN, SB, T> (&ops)[2])
+                                          T (&sv)[SB], T& s_nx, T& y_nx, RevOps<MIXED, GEN, SB, T> (&ops)[2],
+                                          bool a_inf)
 {
     if constexpr (CC < SB) {
         if (CC < cnt) {  // uniform
@@ -806,13 +807,13 @@ __device__ __forceinline__ void rev_steps(const PointArgs& pts, int n, int gb, i
             srec[(size_t)(r - 1) * rs + t] = sbar;
             // per-index sums over the wavefront: FP32 partials of 64 FP32
             // terms on the mixed path, FP64 on the FP64 path
-            double a0, b0, d0;
+            double a0 = 0.0, b0, d0;
             if constexpr (MIXED) {
-                a0 = wave_sum_f32((float)o.ga);
+                if (!a_inf) a0 = wave_sum_f32((float)o.ga);  // uniform; 0 when a' = -inf throughout
                 b0 = wave_sum_f32((float)o.gb);
                 d0 = wave_sum_f32((float)o.gd);
             } else {
-                a0 = wave_sum_f64(o.ga);
+                if (!a_inf) a0 = wave_sum_f64(o.ga);
                 b0 = wave_sum_f64(o.gb);
                 d0 = wave_sum_f64(o.gd);
             }
@@ -827,7 +828,7 @@ __device__ __forceinline__ void rev_steps(const PointArgs& pts, int n, int gb, i
             for (int l = 0; l <= CC; ++l) acc = fma(sv[l], cur.u[l], acc);
             ybar = acc;
             rev_steps<MIXED, GEN, TS, SXS, SB, CC + 1>(pts, n, gb, cnt, k, j, t, ybar, wt, s_x, s_y, Ys, Uf, ldc, ak,
-                                                       bk, invk, srec, yrec, rs, red, sv, s_nx, y_nx, ops);
+                                                       bk, invk, srec, yrec, rs, red, sv, s_nx, y_nx, ops, a_inf);
         }
     }
 }
@@ -848,7 +849,7 @@ __device__ __forceinline__ void walk_block_rev(const PointArgs& pts, int n, int 
                                                const T* __restrict__ ak, const T* __restrict__ bk,
                                                const T* __restrict__ invk, T* __restrict__ srec,
                                                const T* __restrict__ yrec, int64_t rs,
-                                               double* __restrict__ red)
+                                               double* __restrict__ red, bool a_inf)
 {
     T sv[SB];
     // the records of a step are read one step ahead (they stream from HBM);
@@ -864,7 +865,7 @@ __device__ __forceinline__ void walk_block_rev(const PointArgs& pts, int n, int 
         ops[0].template fetch<0>(pts, n, gb, k, Uf, ldc, ak, bk, invk);
     }
     rev_steps<MIXED, GEN, TS, SXS, SB, 0>(pts, n, gb, cnt, k, j, t, ybar, wt, s_x, s_y, Ys, Uf, ldc, ak, bk, invk,
-                                          srec, yrec, rs, red, sv, s_nx, y_nx, ops);
+                                          srec, yrec, rs, red, sv, s_nx, y_nx, ops, a_inf);
 }
 
 // ---------------------------------------------------------------------------

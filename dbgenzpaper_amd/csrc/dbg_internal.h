@@ -56,6 +56,7 @@ struct DenseArgs {
     double* ga;                // reverse: d log P / d a', b', U_rr accumulators (n each)
     double* gb;
     double* gd;
+    int a_inf;                 // reverse: every lower bound is -inf (d log P / d a' is 0)
 };
 enum { kModeIntegrate = 0, kModeRecord = 1, kModeReverse = 2 };
 

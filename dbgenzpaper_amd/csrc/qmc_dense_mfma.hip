@@ -116,6 +116,9 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
     __shared__ T s_y[kSB * TS];        // y of the current sub-block, [k][sample]
     __shared__ double s_red[REV ? 4 * kSB * 3 : 1];
     constexpr bool ULDS = MfmaCfg<MIXED>::ULDS;
+    // the in-panel update's factor rows staged in LDS during the walk (not for
+    // Sobol: its LDS tables leave no registers for the staging addresses)
+    constexpr bool UPDL = ULDS && GEN != kGenSobol;
     constexpr int kUS = PW + 16;       // LDS row stride of a U chunk (conflict-free reads)
     __shared__ T s_u[ULDS ? 2 * kUC * kUS : 1];
     constexpr bool SOBT = GEN == kGenSobol && !REV;  // walk points from LDS Sobol tables
@@ -267,18 +270,38 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
                     }
                 }
                 if constexpr (SOBT) sobol_tables<kSB, 256>(pts, gb, min(kSB, nd - gb), jt * TS + 1, s_sob, tid);
+                const bool later = b + 1 < NTB && gb + kSB < nd;  // the panel has targets after this block
                 __syncthreads();
+                if constexpr (UPDL) {
+                    // factor rows gb .. gb + 15 (targets g0 + 1 .. g0 + PW) of the update after
+                    // the walk go to LDS (the contraction's chunk buffer, idle until then) by
+                    // direct loads issued before the walk, so the walk hides their latency
+                    // (the barrier after the walk waits for them): wave wv moves rows
+                    // 4 wv .. 4 wv + 3, 64 targets per instruction
+                    if (later) {
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) {
+                            const int row = 4 * wv + (q >> 1), half = q & 1;
+                            __builtin_amdgcn_global_load_lds(Uk + (size_t)(gb + row) * ldu + g0 + 1 + 64 * half + lane,
+                                                             s_u + row * kUS + 64 * half, 4, 0, 0);
+                        }
+                    }
+                }
                 const int cnt = min(kSB, nd - gb);
                 if constexpr (REV) {
                     if (walker)
                         walk_block_rev<MIXED, GEN, TS, kSXS, kSB>(pts, n, gb, cnt, k, j, tid, ybar, wt, s_x, s_y, Ys,
                                                                   Ucol, A.ldc, ak, bk, invk, srec, yrec, A.rs,
-                                                                  s_red);
+                                                                  s_red, A.a_inf != 0);
                 } else {
                     if (walker)
                         walk_block<MIXED, GEN, FORM, TS, kSXS, kSB>(pts, gb, cnt, k, j, tid, st, w, s_x, s_y, Ys, ys,
                                                                      Ucol, A.ldc, ak, bk, invk, srec,
                                                                      SOBT ? s_sob : nullptr, sl);
+                }
+                if constexpr (UPDL) {
+                    // the direct loads above write LDS without the compiler tracking them
+                    if (later) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
                 }
                 __syncthreads();
                 if constexpr (REV) {
@@ -293,12 +316,13 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
                     }
                 }
                 // later targets of the panel += Y_sub * U[sub, later]
-                if (b + 1 < NTB && gb + kSB < nd) {
+                if (later) {
 #pragma unroll
                     for (int ks = 0; ks < kSB / 4; ++ks) {
                         T av[NSB];
                         load_y<T, NSB>(av, s_y + (size_t)lq * TS + wv * WS + NSB * lr, TS, 4 * ks);
-                        const T* __restrict__ ub = Uk + (size_t)(gb + 4 * ks + lq) * ldu + g0 + 1 + lr;
+                        const T* __restrict__ ub = UPDL ? s_u + (4 * ks + lq) * kUS + lr
+                                                        : Uk + (size_t)(gb + 4 * ks + lq) * ldu + g0 + 1 + lr;
 #pragma unroll
                         for (int tb = 0; tb < NTB; ++tb) {
                             if (tb > b) {
