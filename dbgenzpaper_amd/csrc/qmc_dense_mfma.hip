@@ -100,9 +100,8 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
     using V = typename Mfma<T>::V;
     constexpr int TS = MfmaCfg<MIXED>::TS;
     constexpr int PW = MfmaCfg<MIXED>::PW;
-    // the record pass (which keeps s per step as well) and the Sobol walk have
-    // no room in the FP32 register budget for the deeper Y ring
-    constexpr int kPFY = (MIXED && (MODE == kModeRecord || GEN == kGenSobol)) ? 2 : MfmaCfg<MIXED>::PFY;
+    // the Sobol walk has no room in the FP32 register budget for the deeper Y ring
+    constexpr int kPFY = (MIXED && GEN == kGenSobol) ? 2 : MfmaCfg<MIXED>::PFY;
     constexpr int kPFU = MfmaCfg<MIXED>::PFU;
     constexpr int kUnroll = kPFY > kPFU ? kPFY : kPFU;
     static_assert(kUnroll % kPFY == 0 && kUnroll % kPFU == 0, "ring depths must divide the unroll");
