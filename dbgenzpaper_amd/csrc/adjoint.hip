@@ -190,8 +190,10 @@ __global__ __launch_bounds__(256) void ubar_gemm_kernel(const T* __restrict__ Y,
 // workgroup instead of once per wave.  Chunk lengths are multiples of kUbK.
 constexpr int kUbK = 32;
 constexpr int kUbKS = kUbK + 4;  // LDS row stride (floats)
+constexpr int kUbE = kUbK / 8;   // 16-byte loads per thread and operand per stage
+constexpr int kUbOcc = 2;        // workgroups per CU the registers are budgeted for
 
-__global__ __launch_bounds__(256, 2) void ubar_gemm_lds_kernel(const float* __restrict__ Y,
+__global__ __launch_bounds__(256, kUbOcc) void ubar_gemm_lds_kernel(const float* __restrict__ Y,
                                                                const float* __restrict__ S, int64_t rs, int nd,
                                                                const int2* __restrict__ pairs, int n_pairs,
                                                                int64_t kchunk, int64_t kmax,
@@ -213,7 +215,7 @@ __global__ __launch_bounds__(256, 2) void ubar_gemm_lds_kernel(const float* __re
     const int64_t k0 = (int64_t)kc * kchunk;
     const int64_t k1 = min(kmax, k0 + kchunk);
     // staging: thread -> tile row tid / 2, samples 16 (tid % 2) .. + 15 of a stage
-    const int srow = tid >> 1, shalf = (tid & 1) * 16;
+    const int srow = tid >> 1, shalf = (tid & 1) * (kUbK / 2);
     const int la = min(tp.x * kUbTile + srow, nd - 1);  // rows past nd read row nd-1; discarded
     const int lb = min(tp.y * kUbTile + srow, nd - 1);
     const float* __restrict__ ga = Y + (size_t)la * rs + shalf;
@@ -226,15 +228,15 @@ __global__ __launch_bounds__(256, 2) void ubar_gemm_lds_kernel(const float* __re
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = V{0, 0, 0, 0};
     const int nst = (int)((k1 - k0) / kUbK);
-    L4 sa[4], sb[4];
+    L4 sa[kUbE], sb[kUbE];
     if (nst > 0) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < kUbE; ++e) {
             sa[e] = *(const L4*)(ga + k0 + 4 * e);
             sb[e] = *(const L4*)(gb + k0 + 4 * e);
         }
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < kUbE; ++e) {
             *(L4*)(wa + 4 * e) = sa[e];
             *(L4*)(wb + 4 * e) = sb[e];
         }
@@ -245,7 +247,7 @@ __global__ __launch_bounds__(256, 2) void ubar_gemm_lds_kernel(const float* __re
         if (st + 1 < nst) {  // uniform
             const int64_t kn = k0 + (int64_t)(st + 1) * kUbK;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
+            for (int e = 0; e < kUbE; ++e) {
                 sa[e] = *(const L4*)(ga + kn + 4 * e);
                 sb[e] = *(const L4*)(gb + kn + 4 * e);
             }
@@ -273,7 +275,7 @@ __global__ __launch_bounds__(256, 2) void ubar_gemm_lds_kernel(const float* __re
             float* __restrict__ wan = &s_a[nb][srow * kUbKS + shalf];
             float* __restrict__ wbn = &s_b[nb][srow * kUbKS + shalf];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
+            for (int e = 0; e < kUbE; ++e) {
                 *(L4*)(wan + 4 * e) = sa[e];
                 *(L4*)(wbn + 4 * e) = sb[e];
             }
