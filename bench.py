@@ -64,6 +64,8 @@ def main() -> int:
                     help="only run the gradient leg and print its record (for profiling it alone)")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the other configurations (d = 16, d = 500, Sobol at d = 2500, banded d = 4000)")
+    ap.add_argument("--contraction", default="auto", choices=("auto", "fp32", "split16"),
+                    help="arithmetic of the mixed dense panel contraction (auto: split16 where built)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes three times (the reference "
                          "kernel for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -89,9 +91,10 @@ def main() -> int:
     _log(f"workload factor_model d={d} k=8 m={m} reps={R}")
     P = W.factor_model(d, 8, m=m, n_reps=R, seed=1)
 
-    def make(T, m_, R_, shifts):
+    def make(T, m_, R_, shifts, contraction=None):
         opts = dbg.QMC_opts(T, m=m_, n_reps=R_, max_pts=m_, max_abs_err=0.0)
-        return dbg.QMCData(P.Sigma, P.a, P.b, opts, None, "Richtmyer", shifts=shifts, phi_form="tail")
+        return dbg.QMCData(P.Sigma, P.a, P.b, opts, None, "Richtmyer", shifts=shifts, phi_form="tail",
+                           contraction=contraction)
 
     def run(D, steps, warmup):
         for _ in range(warmup):
@@ -109,7 +112,7 @@ def main() -> int:
         t1 = time.perf_counter()
         return (t1 - t0) / steps, sum(kms) / len(kms), D.summary()
 
-    Dm = make(np.float32, m, R, P.shifts)
+    Dm = make(np.float32, m, R, P.shifts, args.contraction)
     _log("mixed plan ready")
     t_mixed, k_mixed, s_mixed = run(Dm, args.steps, args.warmup)
     _log(f"mixed: {t_mixed * 1e3:.1f} ms/step, kernel {k_mixed:.1f} ms, log P {s_mixed.log_value:.6f}")

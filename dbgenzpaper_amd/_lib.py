@@ -22,6 +22,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 GEN_RICHTMYER, GEN_SOBOL = 0, 1
 PHI_ERF, PHI_ERFC, PHI_TAIL = 0, 1, 2
 KIND_DENSE, KIND_SPARSE, KIND_SPARSEQMC = 0, 1, 2
+CONTRACT_AUTO, CONTRACT_FP32, CONTRACT_SPLIT16 = 0, 1, 2
 
 # every exported symbol of include/dbgenz.h
 EXPORTS = (
@@ -48,7 +49,8 @@ class Opts(C.Structure):
         ("block_size_j", C.c_int32), ("n_blocks", C.c_int32), ("n_reps", C.c_int32),
         ("n_bits", C.c_int32), ("max_pts", C.c_int64), ("max_abs_err", C.c_double),
         ("precision", C.c_int32), ("generator", C.c_int32), ("phi_form", C.c_int32),
-        ("kind", C.c_int32), ("sobol_skip", C.c_int64),
+        ("kind", C.c_int32), ("sobol_skip", C.c_int64), ("contraction", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

@@ -209,6 +209,8 @@ struct dbg_plan {
     // packed layouts
     void* Uk = nullptr;
     float* ucol32 = nullptr;  // FP32 copy of the factor for the mixed walk
+    void* uc16 = nullptr;     // split FP16 factor of the FP16 matrix-core contraction
+    bool split = false;       // mixed dense path on the split-FP16 kernel
     int64_t ldu = 0;
     void* ak = nullptr;
     void* bk = nullptr;
@@ -281,7 +283,7 @@ struct dbg_plan {
     {
         for (void* p : G.all()) (void)hipFree(p);
         F.release();
-        void* ptrs[] = {sig, a0, b0, Uk, ucol32, ak, bk, invk, first, colcnt, colptr, rowval, nzval, reach_d, q, shifts_f,
+        void* ptrs[] = {sig, a0, b0, Uk, ucol32, uc16, ak, bk, invk, first, colcnt, colptr, rowval, nzval, reach_d, q, shifts_f,
                         q64, shift32, shifts_u, V, Y, part_m, part_e, rep_m, rep_e, out};
         for (void* p : ptrs) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -337,6 +339,7 @@ int plan_factor(dbg_plan* P)
     if (o.kind == DBG_KIND_DENSE && P->mixed) {
         if (P->ucol32 == nullptr) HIPCHK(dalloc(&P->ucol32, (size_t)n * n * sizeof(float)));
         HIPCHK(convert_f32(P->F.M, (size_t)n * n, P->ucol32, s));
+        if (P->split) HIPCHK(pack_split16(P->ucol32, n, P->uc16, s));
     }
     HIPCHK(pack_dense(P->F.M, n, n, P->F.a, P->F.b, P->mixed, P->Uk, P->ldu, P->ak, P->bk, P->invk,
                       P->first, o.phi_form == DBG_PHI_ERFC ? DBG_PHI_ERFC : DBG_PHI_ERF, s));
@@ -381,6 +384,11 @@ int plan_build(dbg_plan* P, const double* Sigma, int64_t lda, const double* a, c
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     P->ncu = ncu;
+    P->use_mfma = (o.kind == DBG_KIND_DENSE) && n >= kMfmaMinDim;
+    // the FP16 matrix-core contraction: mixed dense path, Richtmyer points
+    P->split = P->use_mfma && P->mixed && o.generator == DBG_GEN_RICHTMYER && o.contraction != DBG_CONTRACT_FP32;
+    if (P->split)
+        HIPCHK(dalloc(&P->uc16, (size_t)split16_cols(n) * (size_t)split16_col_stride(n) * sizeof(uint16_t)));
     rc = plan_factor(P);
     if (rc) return rc;
 
@@ -396,13 +404,13 @@ int plan_build(dbg_plan* P, const double* Sigma, int64_t lda, const double* a, c
     HIPCHK(dalloc(&P->V, (size_t)std::max(nd, 1) * 32 * sizeof(uint32_t)));
 
     // integration workspace: one y slot per resident block
-    P->use_mfma = (o.kind == DBG_KIND_DENSE) && n >= kMfmaMinDim;
     P->tile = (o.kind != DBG_KIND_DENSE) ? sparse_tile()
-                                          : (P->use_mfma ? dense_mfma_tile(P->mixed) : dense_valu_tile());
+                                          : (P->split ? dense_split_tile()
+                                                      : (P->use_mfma ? dense_mfma_tile(P->mixed) : dense_valu_tile()));
     P->tiles_per_rep = (o.m + P->tile - 1) / P->tile;
     P->n_tiles = P->tiles_per_rep * o.n_reps;
     P->grid = (int)std::min<int64_t>(P->n_tiles, (int64_t)ncu * 2);
-    P->y_slot = (int64_t)std::max(nd, 1) * P->tile;
+    P->y_slot = P->split ? dense_split_slot(nd) : (int64_t)std::max(nd, 1) * P->tile;
     HIPCHK(dalloc(&P->Y, (size_t)P->grid * P->y_slot * T));
     HIPCHK(dalloc(&P->part_m, (size_t)P->n_tiles * sizeof(double)));
     HIPCHK(dalloc(&P->part_e, (size_t)P->n_tiles * sizeof(int)));
@@ -464,7 +472,9 @@ int run_loop(dbg_plan* P, int64_t block, int64_t n_pts0)
         A.y_slot = P->y_slot;
         A.part_m = P->part_m;
         A.part_e = P->part_e;
-        if (P->use_mfma)
+        if (P->split)
+            HIPCHK(launch_dense_split(A, gen, P->uc16, P->grid, P->stream));
+        else if (P->use_mfma)
             HIPCHK(launch_dense_mfma(A, P->mixed, gen, o.phi_form, P->grid, P->stream));
         else
             HIPCHK(launch_dense_valu(A, P->mixed, gen, o.phi_form, P->grid, P->stream));
@@ -658,7 +668,7 @@ extern "C" {
 
 const char* dbg_last_error(void) { return g_err.c_str(); }
 
-int32_t dbg_abi_version(void) { return 1; }
+int32_t dbg_abi_version(void) { return 2; }
 
 void dbg_opts_default(dbg_opts* o, int32_t precision)
 {
@@ -718,6 +728,11 @@ int dbg_plan_create(dbg_plan** plan, const dbg_opts* opts, int32_t n, const doub
     if (o.kind != DBG_KIND_DENSE && o.phi_form == DBG_PHI_ERFC)
         return fail(DBG_ERR_ARG, "the erfc form exists for the dense loop only");
     if (o.n_bits < 1 || o.n_bits > 64) return fail(DBG_ERR_ARG, "n_bits out of range");
+    if (o.contraction < DBG_CONTRACT_AUTO || o.contraction > DBG_CONTRACT_SPLIT16)
+        return fail(DBG_ERR_ARG, "unknown contraction");
+    if (o.contraction == DBG_CONTRACT_SPLIT16 &&
+        (o.precision != DBG_PREC_MIXED || o.kind != DBG_KIND_DENSE || o.generator != DBG_GEN_RICHTMYER))
+        return fail(DBG_ERR_ARG, "the split-FP16 contraction is built for the mixed dense Richtmyer path");
     dbg_plan* P = new dbg_plan();
     P->o = o;
     P->n = n;

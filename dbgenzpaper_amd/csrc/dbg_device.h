@@ -615,7 +615,8 @@ struct StepOps {
     }
 };
 
-template <bool MIXED, int GEN, int FORM, int TS, int SXS, int SB, int CC, typename T, typename State>
+template <bool MIXED, int GEN, int FORM, int TS, int SXS, int SB, int CC, bool YST = true, typename T,
+          typename State>
 __device__ __forceinline__ void walk_steps(const PointArgs& pts, int gb, int cnt, int k, int64_t j, int t,
                                            State& st, Weight<T>& w, T (&yv)[SB], StepOps<MIXED, GEN, SB, T>& cur,
                                            const T* __restrict__ s_x, T* __restrict__ s_y, T* __restrict__ Ys,
@@ -641,7 +642,7 @@ __device__ __forceinline__ void walk_steps(const PointArgs& pts, int gb, int cnt
                 y = path_draw<MIXED, GEN, FORM>(pts, cur.pt, st, g, j);
             yv[CC] = y;
             s_y[CC * TS + t] = y;
-            Ys[(size_t)g * ys + t] = y;
+            if constexpr (YST) Ys[(size_t)g * ys + t] = y;
             T acc = s_x[t * SXS + CC];
 #pragma unroll
             for (int l = 0; l <= CC; ++l) acc = fma(yv[l], cur.u[l], acc);
@@ -649,10 +650,28 @@ __device__ __forceinline__ void walk_steps(const PointArgs& pts, int gb, int cnt
             PT::phi(st, acc, cur.a, cur.b, cur.inv);
             PT::weigh(w, st);
             if constexpr (CC + 1 < SB)
-                walk_steps<MIXED, GEN, FORM, TS, SXS, SB, CC + 1>(pts, gb, cnt, k, j, t, st, w, yv, nxt, s_x, s_y,
+                walk_steps<MIXED, GEN, FORM, TS, SXS, SB, CC + 1, YST>(pts, gb, cnt, k, j, t, st, w, yv, nxt, s_x, s_y,
                                                                    Ys, ys, Ucol, ldc, ak, bk, invk, srec, s_sob, sl);
         }
     }
+}
+
+// The walk of one sub-block; with YST = false the y values are not stored
+// to the workspace row by row but left in yv for the caller (which stores
+// the whole sub-block at once in its own layout).
+template <bool MIXED, int GEN, int FORM, int TS, int SXS, int SB, bool YST, typename T, typename State>
+__device__ __forceinline__ void walk_block_yv(T (&yv)[SB], const PointArgs& pts, int gb, int cnt, int k,
+                                              int64_t j, int t, State& st, Weight<T>& w,
+                                              const T* __restrict__ s_x, T* __restrict__ s_y,
+                                              T* __restrict__ Ys, int64_t ys, const T* __restrict__ Ucol,
+                                              int64_t ldc, const T* __restrict__ ak, const T* __restrict__ bk,
+                                              const T* __restrict__ invk, T* __restrict__ srec = nullptr,
+                                              const uint32_t* __restrict__ s_sob = nullptr, SobolLane sl = {})
+{
+    StepOps<MIXED, GEN, SB, T> first;
+    if (cnt > 0) first.template fetch<0>(pts, gb, k, Ucol, ldc, ak, bk, invk);
+    walk_steps<MIXED, GEN, FORM, TS, SXS, SB, 0, YST>(pts, gb, cnt, k, j, t, st, w, yv, first, s_x, s_y, Ys, ys,
+                                                      Ucol, ldc, ak, bk, invk, srec, s_sob, sl);
 }
 
 template <bool MIXED, int GEN, int FORM, int TS, int SXS, int SB, typename T, typename State>
@@ -665,10 +684,8 @@ __device__ __forceinline__ void walk_block(const PointArgs& pts, int gb, int cnt
                                            const uint32_t* __restrict__ s_sob = nullptr, SobolLane sl = {})
 {
     T yv[SB];
-    StepOps<MIXED, GEN, SB, T> first;
-    if (cnt > 0) first.template fetch<0>(pts, gb, k, Ucol, ldc, ak, bk, invk);
-    walk_steps<MIXED, GEN, FORM, TS, SXS, SB, 0>(pts, gb, cnt, k, j, t, st, w, yv, first, s_x, s_y, Ys, ys, Ucol,
-                                                 ldc, ak, bk, invk, srec, s_sob, sl);
+    walk_block_yv<MIXED, GEN, FORM, TS, SXS, SB, true>(yv, pts, gb, cnt, k, j, t, st, w, s_x, s_y, Ys, ys, Ucol,
+                                                       ldc, ak, bk, invk, srec, s_sob, sl);
 }
 
 // ---------------------------------------------------------------------------

@@ -105,6 +105,14 @@ hipError_t sigma_chain(double* Ub, int n, const double* gd, double* X, hipStream
 void* blas_handle_void(hipStream_t s);
 int dense_mfma_tile(bool mixed);
 int dense_mfma_panel(bool mixed);
+// mixed dense kernel with the panel contraction on FP16 matrix cores
+// (split operands; Richtmyer points)
+hipError_t launch_dense_split(const DenseArgs& A, int gen, const void* uc, int grid, hipStream_t s);
+hipError_t pack_split16(const float* Ucol, int n, void* uc, hipStream_t s);
+int64_t split16_col_stride(int n);  // halfs per target column of the split factor
+int split16_cols(int n);            // target columns of the split factor
+int dense_split_tile();
+int64_t dense_split_slot(int nd);   // y workspace floats per workgroup slot
 // dimension from which the dense path uses the matrix-core kernel
 constexpr int kMfmaMinDim = 96;
 

@@ -153,7 +153,7 @@ class _QMCBase:
     _phi = L.PHI_ERF
 
     def __init__(self, C_mat, a, b, opts: QMCOpts | None, rng, qmc_type, *, shifts=None,
-                 sobol_skip=None, chol_sizes=None, phi_form=None, _clamp=True):
+                 sobol_skip=None, chol_sizes=None, phi_form=None, contraction=None, _clamp=True):
         Cm = _as_matrix(C_mat)
         n = Cm.shape[0]
         if opts is None:
@@ -188,6 +188,9 @@ class _QMCBase:
         o.phi_form = self._phi
         o.kind = self._kind
         o.sobol_skip = -1 if sobol_skip is None else int(sobol_skip)
+        # arithmetic of the mixed dense contraction: None/"auto", "fp32" or "split16"
+        o.contraction = {None: L.CONTRACT_AUTO, "auto": L.CONTRACT_AUTO, "fp32": L.CONTRACT_FP32,
+                         "split16": L.CONTRACT_SPLIT16}[contraction]
         self._opts_c = o
         plan = C.c_void_p()
         rc = L.lib().dbg_plan_create(C.byref(plan), C.byref(o), n, L.ptr(self._Sigma, C.c_double), n,

@@ -52,6 +52,13 @@ extern "C" {
 #define DBG_KIND_DENSE 0      /* QMCData: pivoted factor, dense loop */
 #define DBG_KIND_SPARSE 1     /* QMCDataSparse: unpivoted factor, CSC loop */
 #define DBG_KIND_SPARSEQMC 2  /* SparseQMC.QMCDataSparse: Sobol, no tent map */
+/* arithmetic of the dense panel contraction in the mixed mode (no reference
+ * counterpart: the reference's Float32 mode calls BLAS sgemm) */
+#define DBG_CONTRACT_AUTO 0     /* the library's choice: SPLIT16 where it is built, else FP32 */
+#define DBG_CONTRACT_FP32 1     /* FP32 matrix cores (v_mfma_f32_16x16x4f32) */
+#define DBG_CONTRACT_SPLIT16 2  /* FP16 matrix cores, operands split into high and low
+                                   FP16 parts, three products per pair, FP32 accumulation
+                                   (FP32-level accuracy; Richtmyer points only) */
 
 /* Options: QMCOpts{T} (src/qmc.jl:15-27, defaults of QMC_opts :111-125) plus
  * the choices the Julia constructors take as arguments. */
@@ -74,6 +81,8 @@ typedef struct dbg_opts {
     int32_t phi_form;         /* DBG_PHI_* */
     int32_t kind;             /* DBG_KIND_* */
     int64_t sobol_skip;       /* next! calls before the first Sobol point; < 0: as skip(s, 2m-1) */
+    int32_t contraction;      /* DBG_CONTRACT_* (mixed dense path) */
+    int32_t reserved;         /* zero */
 } dbg_opts;
 
 /* Result of qmc_pnorm!: (value, err, n_pts) of src/qmc.jl:829,849 plus the

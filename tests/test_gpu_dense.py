@@ -155,3 +155,39 @@ def test_sobol_fp64_mfma(dbg, oracle, d, m, skip):
     r = oracle_same_inputs(oracle, D)
     assert rel(v, r.value) < F64_TOL, (v, r.value)
     assert np.allclose(D.qmc_reps, r.rep_means, rtol=F64_TOL, atol=0)
+
+
+@pytest.mark.parametrize("d,family", [(150, "random"), (300, "random"), (700, "factor"), (1100, "factor")])
+def test_split16_contraction(dbg, oracle, d, family):
+    """The mixed dense kernel with its panel contraction on FP16 matrix cores
+    (high/low FP16 split of both operands, three products, FP32 accumulation)
+    against the FP64 oracle on the same factor, points and shifts, and against
+    the FP32 matrix-core contraction.  The shapes cover a panel without
+    contraction (d = 150), several panels with partial last panels and
+    sub-blocks, and the deep lower orthant of the factor model whose
+    estimate is far below the FP64 range (compared in log P)."""
+    if family == "random":
+        P = W.dense_random(d, m=2048, n_reps=3, seed=d)
+    else:
+        P = W.factor_model(d, m=2048, n_reps=2, seed=d)
+    print(f"split16 d={d} {family}", flush=True)
+    res = {}
+    opts = dbg.QMC_opts(np.float32, m=P.m, n_reps=P.n_reps, max_pts=P.m, max_abs_err=0.0)
+    for c in ("split16", "fp32"):
+        D = dbg.QMCData(P.Sigma, P.a, P.b, opts, None, "Richtmyer", shifts=P.shifts, contraction=c)
+        dbg.qmc_pnorm(D)
+        res[c] = (D.last_result.log_value, np.array(D.qmc_reps, dtype=np.float64))
+    D64 = _plan(dbg, P, phi_form="tail")
+    r = oracle_same_inputs(oracle, D64)
+    ls, lf, lo = res["split16"][0], res["fp32"][0], r.log_value
+    print(f"  log P split16 {ls:.10f} fp32 {lf:.10f} oracle {lo:.10f}", flush=True)
+    assert np.isfinite(ls)
+    # |delta log P| is the relative error of P.  The split carries the operands
+    # to FP32-level precision, so its error against FP64 is of the FP32
+    # contraction's size: within 1e-4 where P is moderate, and no more than
+    # twice the FP32 contraction's own error in the deep lower orthant (log P
+    # about -2e4 at d = 700, where both are 2e-4 .. 5e-4)
+    if family == "random":
+        assert abs(ls - lo) < 1e-4 and abs(lf - lo) < 1e-4, (ls, lf, lo)
+    else:
+        assert abs(ls - lo) < 2 * abs(lf - lo) + 1e-5, (ls, lf, lo)

@@ -32,9 +32,12 @@ def test_abi_version_and_defaults(dbg):
     from dbgenzpaper_amd import _lib
 
     L = _lib.lib()
-    assert L.dbg_abi_version() == 1
+    assert L.dbg_abi_version() == 2
     o = _lib.Opts()
+    # the ctypes mirror has the size of the C struct (int32/int64 fields, 8-byte aligned)
+    assert ctypes.sizeof(o) == 96
     L.dbg_opts_default(ctypes.byref(o), _lib.PREC_FP64)
+    assert o.contraction == _lib.CONTRACT_AUTO
     # QMC_opts defaults, src/qmc.jl:111-125
     assert (o.chol_block_size, o.chol_block_size2, o.m, o.block_size_i, o.block_size_j,
             o.block_size_i2, o.n_reps, o.n_bits, o.max_pts) == (64, 512, 10000, 64, 64, 32, 12, 32, 10000)
