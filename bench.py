@@ -34,12 +34,13 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 matrix
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP16_TFLOPS = 2500.0  # MI355X dense FP16 matrix
 
 # HBM bytes per launch of the mixed dense integration kernel, keyed by
-# (d, N, shifts): rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 (factors from
-# a plain 1 GiB device copy), profiles/dense_d2500_mixed_counters.md.  Other
-# shapes report null.
-TRAFFIC_BYTES = {(2500, 2 ** 18, 10): 2.839e11}
+# (contraction, d, N, shifts): rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00
+# (factors from a plain 1 GiB device copy), profiles/dense_d2500_mixed_counters.md.
+# Other shapes report null.
+TRAFFIC_BYTES = {("split16", 2500, 2 ** 18, 10): 2.845e11, ("fp32", 2500, 2 ** 18, 10): 2.839e11}
 
 
 def _log(msg):
@@ -112,6 +113,7 @@ def main() -> int:
         t1 = time.perf_counter()
         return (t1 - t0) / steps, sum(kms) / len(kms), D.summary()
 
+    contraction = "fp32" if args.contraction == "fp32" else "split16"  # auto: split16 at this shape
     Dm = make(np.float32, m, R, P.shifts, args.contraction)
     _log("mixed plan ready")
     t_mixed, k_mixed, s_mixed = run(Dm, args.steps, args.warmup)
@@ -129,21 +131,39 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 (contraction, special functions) + fp64 (factor, points, reductions)",
+        "dtype": ("fp32-equivalent contraction (FP16 matrix cores, high/low split of both operands, three "
+                  "products, FP32 accumulation) + fp32 special functions + fp64 (factor, points, reductions)"
+                  if contraction == "split16" else
+                  "fp32 (contraction, special functions) + fp64 (factor, points, reductions)"),
         "data": "synthetic (seeded factor model, random-init inputs of the configured shape)",
         "config": {"workload": f"factor_model d={d} k=8 lower orthant u~U(0.01,0.10), Richtmyer "
                                f"N={m} x {R} shifts, step = Cholesky + integration + reductions",
                    "d": d, "points": m, "shifts": R, "parallelism": "single GPU"},
         "log_p_mixed": s_mixed.log_value,
         "kernel_ms_mixed": k_mixed,
+        "contraction": contraction,
     }
     flops = float(m) * R * d * (d - 1)
     achieved = flops / (k_mixed * 1e-3) / 1e12
+    # achieved: the algorithmic d(d-1) flops per sample (FP32-accurate multiply-adds)
+    # against the FP32 matrix peak, the precision the contraction delivers; the
+    # split form runs them as three FP16 products each on the FP16 pipe
     out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                        "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                       "traffic": TRAFFIC_BYTES.get((d, m, R)),
-                       "kernel": "dense integration (mixed)", "flops_per_launch": flops}
+                       "traffic": TRAFFIC_BYTES.get((contraction, d, m, R)),
+                       "kernel": f"dense integration (mixed, {contraction} contraction)",
+                       "flops_per_launch": flops}
+    if contraction == "split16":
+        out["roofline"]["fp16_pipe"] = {"executed_tflops": 3 * achieved, "peak": PEAK_FP16_TFLOPS,
+                                        "frac": 3 * achieved / PEAK_FP16_TFLOPS}
     Dm.close()
+    if contraction == "split16" and not args.no_fp64:
+        # the same job with the FP32 matrix-core contraction, for comparison
+        Df = make(np.float32, m, R, P.shifts, "fp32")
+        t_f, k_f, s_f = run(Df, max(1, args.steps), min(args.warmup, 1))
+        _log(f"mixed (fp32 contraction): {t_f * 1e3:.1f} ms/step, kernel {k_f:.1f} ms, log P {s_f.log_value:.6f}")
+        out["fp32_contraction"] = {"ms_per_step": t_f * 1e3, "kernel_ms": k_f, "log_p": s_f.log_value}
+        Df.close()
 
     if not args.no_fp64:
         D64 = make(np.float64, m, R, P.shifts)
@@ -156,6 +176,10 @@ def main() -> int:
         out["log_p_fp64"] = s64.log_value
         out["speedup_mixed_vs_fp64"] = t64 / t_mixed
         out["rel_err_mixed_vs_fp64_full"] = abs(math.expm1(s_mixed.log_value - s64.log_value))
+        if "fp32_contraction" in out:
+            fc = out["fp32_contraction"]
+            fc["rel_err_vs_fp64_full"] = abs(math.expm1(fc["log_p"] - s64.log_value))
+            fc["speedup_vs_fp64"] = t64 * 1e3 / fc["ms_per_step"]
         D64.close()
 
     if not args.no_grad:
@@ -284,12 +308,26 @@ def grad_leg(steps):
     grd = sum(gm) / len(gm)
     _log(f"adjoint d={d}: forward kernel {fwd:.1f} ms, gradient {grd:.1f} ms, log P {g.log_value:.4f}")
     D.close()
+    # the forward with the FP32 matrix-core contraction (the arithmetic of the
+    # gradient's own record and reverse sweeps)
+    Df = dbg.QMCData(P.Sigma, P.a, P.b, opts, None, "Richtmyer", shifts=P.shifts, phi_form="tail",
+                     contraction="fp32")
+    Df.loop(0)
+    Df.sync()
+    ff = []
+    for _ in range(max(1, steps)):
+        Df.loop(0)
+        Df.sync()
+        ff.append(Df.last_kernel_ms())
+    Df.close()
+    fwd32 = sum(ff) / len(ff)
     # the gradient re-runs the forward (with records) first: the adjoint proper
     # is the rest, estimated as the total minus one forward kernel
     return {"workload": f"factor_model d={d} k=16 lower orthant u~U(0.02,0.20), Richtmyer N={m} x {R} shifts, "
                         "FP32-mixed, gradient w.r.t. a, b, Sigma",
             "forward_kernel_ms": fwd, "gradient_ms": grd, "ratio": grd / fwd,
             "adjoint_only_ms": grd - fwd, "adjoint_only_ratio": (grd - fwd) / fwd, "log_p": g.log_value,
+            "forward_kernel_ms_fp32_contraction": fwd32, "ratio_vs_fp32_contraction_forward": grd / fwd32,
             "grad_b_norm": float(np.linalg.norm(g.b)), "grad_sigma_norm": float(np.linalg.norm(g.Sigma))}
 
 

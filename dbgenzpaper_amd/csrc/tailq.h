@@ -26,6 +26,26 @@
 
 namespace dbg {
 
+// n / d for the positive, well-scaled denominators of the rational
+// approximations below: a reciprocal (the device's approximate v_rcp_f32,
+// within 1 ulp; the host's division) refined by one Newton step on the
+// quotient, q = q0 + (n - d q0) / d, which leaves the result within about one
+// ulp of the correctly rounded quotient at a fraction of the cost of a full
+// IEEE division (no special cases: d is never 0, inf or subnormal here).
+DBG_HD inline float div_pos(float n, float d)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float r = __builtin_amdgcn_rcpf(d);
+#elif defined(DBG_RCP_ULP_TEST)
+    // host check of the refinement: the reciprocal off by DBG_RCP_ULP_TEST ulps
+    const float r = nextafterf(1.0f / d, DBG_RCP_ULP_TEST > 0 ? INFINITY : 0.0f);
+#else
+    const float r = 1.0f / d;
+#endif
+    const float q0 = n * r;
+    return fmaf(fmaf(-d, q0, n), r, q0);
+}
+
 // Horner evaluation with fused multiply-adds, highest coefficient first
 template <int N>
 DBG_HD inline float horner_f32(float x, const float (&c)[N])
@@ -48,7 +68,7 @@ DBG_HD inline float tail_quantile_f32(float t, int k)
         const float B[8] = {5.2264952788528545610e+3f, 2.8729085735721942674e+4f, 3.9307895800092710610e+4f,
                             2.1213794301586595867e+4f, 5.3941960214247511077e+3f, 6.8718700749205790830e+2f,
                             4.2313330701600911252e+1f, 1.0f};
-        return q * horner_f32(r, A) / horner_f32(r, B);
+        return q * div_pos(horner_f32(r, A), horner_f32(r, B));
     }
     // -log p = -log t + k log 2
     const float s = fmaf((float)k, 0.693147180559945309f, -logf(t));
@@ -62,7 +82,7 @@ DBG_HD inline float tail_quantile_f32(float t, int k)
                             1.48103976427480074590e-1f, 6.89767334985100004550e-1f, 1.67638483018380384940e+0f,
                             2.05319162663775882187e+0f, 1.0f};
         r -= 1.6f;
-        val = horner_f32(r, C) / horner_f32(r, D);
+        val = div_pos(horner_f32(r, C), horner_f32(r, D));
     } else {
         const float E[8] = {2.01033439929228813265e-7f, 2.71155556874348757815e-5f, 1.24266094738807843860e-3f,
                             2.65321895265761230930e-2f, 2.96560571828504891230e-1f, 1.78482653991729133580e+0f,
@@ -71,7 +91,7 @@ DBG_HD inline float tail_quantile_f32(float t, int k)
                             7.86869131145613259100e-4f, 1.48753612908506148525e-2f, 1.36929880922735805310e-1f,
                             5.99832206555887937690e-1f, 1.0f};
         r -= 5.0f;
-        val = horner_f32(r, E) / horner_f32(r, F);
+        val = div_pos(horner_f32(r, E), horner_f32(r, F));
     }
     return -val;
 }

@@ -117,13 +117,17 @@ def _host_build(tmp_path, src, name):
     return ctypes.CDLL(str(so))
 
 
-def test_fp32_tail_quantile_against_mpmath(tmp_path):
+@pytest.mark.parametrize("rcp_ulp", [0, 1, -1])
+def test_fp32_tail_quantile_against_mpmath(tmp_path, rcp_ulp):
     """The mixed path's FP32 inverse CDF (AS 241 in FP32, -log p formed as
     -log t + k log 2) against a 40-digit quantile, from p = 0.5 down to
     p = 2^-3000 given as scaled masses t 2^-k.  Bound: 4 FP32 ulps of
-    max(|y|, 1)."""
+    max(|y|, 1).  The rational approximations divide by one refined
+    reciprocal; rcp_ulp = +-1 builds it from a reciprocal one ulp off, as the
+    device's approximate reciprocal may be."""
     mpmath = pytest.importorskip("mpmath")
-    lib = _host_build(tmp_path, '#include "tailq.h"\n'
+    pre = f"#define DBG_RCP_ULP_TEST {rcp_ulp}\n" if rcp_ulp else ""
+    lib = _host_build(tmp_path, pre + '#include "tailq.h"\n'
                       'extern "C" float tq(float t, int k) { return dbg::tail_quantile_f32(t, k); }\n', "tq")
     lib.tq.restype = ctypes.c_float
     lib.tq.argtypes = [ctypes.c_float, ctypes.c_int]
