@@ -307,6 +307,298 @@ __global__ __launch_bounds__(kCholThreads) void chol_window_kernel(CholArgs C, i
     chol_window_body<kCholThreads, kMaxWin>(C, w0, w1, ib0, min(w1, ib0 + nb), nb, pivot, *C.eps0, false);
 }
 
+// ---------------------------------------------------------------------------
+// The same inner block with the window's columns in registers: thread t owns
+// column c = ib0 + t of [ib0, w1) (at most kRegCols) and keeps the rows of U
+// this block has produced for it, U[ib0 + q, c] for q < i - ib0, in ur[q].
+// A pivot step then needs global memory only for the symmetric swap (which
+// also hands each thread the new M[i, c]) and the stores of row i: the
+// window's diagonal sits in LDS with the other per-index state, and the
+// in-block dot of row i (src/cholesky.jl:547-570) multiplies the broadcast
+// column i (LDS) with the thread's own registers, in the same ascending
+// order and the same operations as chol_window_body, so the results and the
+// pivot order are identical.
+// ---------------------------------------------------------------------------
+constexpr int kRegCols = 512;  // threads = window columns from the block start
+constexpr int kRegNb = 64;     // rows per inner block held in registers
+
+// Register layout: the block's rows for the thread's column in groups of 16
+// pivots.  The group being filled is `cur`; completed groups move to g0..g2.
+// The pivot loop is unrolled over the 16 positions of a group, so every
+// register index is a compile-time constant (a runtime index would put the
+// array in scratch memory).
+struct ColRegs {
+    double g0[16], g1[16], g2[16], cur[16];
+};
+
+// dst[0 .. 16 ng + E) = the column's rows (ng completed groups, then E of cur)
+template <int E>
+__device__ __forceinline__ void regs_store(const ColRegs& r, int ng, double* dst)
+{
+    if (ng > 0) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) dst[e] = r.g0[e];
+    }
+    if (ng > 1) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) dst[16 + e] = r.g1[e];
+    }
+    if (ng > 2) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) dst[32 + e] = r.g2[e];
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) dst[16 * ng + e] = r.cur[e];
+}
+
+template <int E>
+__device__ __forceinline__ void regs_load(ColRegs& r, int ng, const double* src)
+{
+    if (ng > 0) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) r.g0[e] = src[e];
+    }
+    if (ng > 1) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) r.g1[e] = src[16 + e];
+    }
+    if (ng > 2) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) r.g2[e] = src[32 + e];
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) r.cur[e] = src[16 * ng + e];
+}
+
+// acc - sum_{l < 16 ng + E} ci[l] U[l], subtracted in ascending l
+template <int E>
+__device__ __forceinline__ double regs_dot(const ColRegs& r, int ng, const double* ci, double acc)
+{
+    if (ng > 0) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc -= ci[e] * r.g0[e];
+    }
+    if (ng > 1) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc -= ci[16 + e] * r.g1[e];
+    }
+    if (ng > 2) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc -= ci[32 + e] * r.g2[e];
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) acc -= ci[16 * ng + e] * r.cur[e];
+    return acc;
+}
+
+struct RegShared {
+    double dsum[kRegCols], sv[kRegCols], a[kRegCols], b[kRegCols], dg[kRegCols];
+    int perm[kRegCols];
+    double ci[kRegNb], xch[kRegNb];
+    double rv[kRegCols / 64];
+    int ri[kRegCols / 64];
+    double uii, yi;
+    int im, stop;
+};
+
+// One pivot i = ib0 + 16 ng + E of the inner block; returns false when the
+// block ends (rank deficiency).  All branches are uniform.
+template <int E>
+__device__ __forceinline__ bool reg_pivot(const CholArgs& C, RegShared& S, ColRegs& R, int ng, int ib0, int w1,
+                                          int pivot, double eps0)
+{
+    double* M = C.M;
+    const int64_t lda = C.lda;
+    const int n = C.n;
+    const int tid = threadIdx.x;
+    const int q = 16 * ng + E;
+    const int i = ib0 + q;
+    const int c = ib0 + tid;
+    const bool own = c < w1;
+    const double c2 = 1.0 / sqrt(2.0);
+    const double is2pi = 2.0 / sqrt(2.0 * 3.14159265358979323846);
+    // ---- pivot search over the rest of the window (src/cholesky.jl:500-516)
+    double bv = NAN;
+    int bi = INT32_MAX;
+    double best_order = 0.0;
+    if (pivot) {
+        bv = INFINITY;
+        if (own && c >= i) {
+            double m_kk = S.dg[tid] - S.dsum[tid];
+            m_kk = m_kk < 0 ? 0.0 : sqrt(m_kk);
+            const double a_k = (S.a[tid] - S.sv[tid]) / m_kk * c2;
+            const double b_k = (S.b[tid] - S.sv[tid]) / m_kk * c2;
+            const double o = erf(b_k) - erf(a_k);
+            if (better(o, c, bv, bi)) {
+                bv = o;
+                bi = c;
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(bv, off, 64);
+            const int oi = __shfl_xor(bi, off, 64);
+            if (better(ov, oi, bv, bi)) {
+                bv = ov;
+                bi = oi;
+            }
+        }
+        if ((tid & 63) == 0) {
+            S.rv[tid >> 6] = bv;
+            S.ri[tid >> 6] = bi;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        int im = i;
+        if (pivot) {
+            bv = S.rv[0];
+            bi = S.ri[0];
+            for (int w = 1; w < kRegCols / 64; ++w)
+                if (better(S.rv[w], S.ri[w], bv, bi)) {
+                    bv = S.rv[w];
+                    bi = S.ri[w];
+                }
+            im = bi;
+            best_order = bv;
+        }
+        const int a0 = i - ib0, a1 = im - ib0;
+        const double m_ii = S.dg[a1] - S.dsum[a1];  // read before the swap
+        if (im != i) {
+            int tp = S.perm[a0]; S.perm[a0] = S.perm[a1]; S.perm[a1] = tp;
+            double t;
+            t = S.dsum[a0]; S.dsum[a0] = S.dsum[a1]; S.dsum[a1] = t;
+            t = S.sv[a0]; S.sv[a0] = S.sv[a1]; S.sv[a1] = t;
+            t = S.a[a0]; S.a[a0] = S.a[a1]; S.a[a1] = t;
+            t = S.b[a0]; S.b[a0] = S.b[a1]; S.b[a1] = t;
+            t = S.dg[a0]; S.dg[a0] = S.dg[a1]; S.dg[a1] = t;
+        }
+        S.im = im;
+        if (pivot && !(m_ii > eps0)) {
+            S.stop = 1;
+            *C.rank = i;
+        } else {
+            const double uii = sqrt(m_ii);
+            S.uii = uii;
+            const double a_norm = (S.a[a0] - S.sv[a0]) / uii;
+            const double b_norm = (S.b[a0] - S.sv[a0]) / uii;
+            S.yi = pivot ? (exp(-(a_norm * a_norm) / 2) - exp(-(b_norm * b_norm) / 2)) * is2pi / best_order : 0.0;
+        }
+    }
+    __syncthreads();
+    const int im = S.im;
+    if (S.stop) return false;
+    // ---- symmetric swap i <-> im; each window thread keeps the new M[i, c]
+    double mic = 0.0;
+    bool have = false;
+    if (im != i) {
+        for (int r = tid; r < i; r += kRegCols) {  // rows r < i: columns i and im
+            const double t = MM(r, i);
+            MM(r, i) = MM(r, im);
+            MM(r, im) = t;
+        }
+        if (own && c > i && c < im) {  // M[i, c] <-> M[c, im]
+            const double x = MM(i, c), y = MM(c, im);
+            MM(i, c) = y;
+            MM(c, im) = x;
+            mic = y;
+            have = true;
+        } else if (own && c > im) {  // M[i, c] <-> M[im, c]
+            const double x = MM(i, c), y = MM(im, c);
+            MM(i, c) = y;
+            MM(im, c) = x;
+            mic = y;
+            have = true;
+        }
+        for (int cc = w1 + tid; cc < n; cc += kRegCols) {  // columns beyond the window
+            const double t = MM(i, cc);
+            MM(i, cc) = MM(im, cc);
+            MM(im, cc) = t;
+        }
+        if (tid == 0) MM(im, im) = MM(i, i);  // MM(i, i) becomes U_ii below
+        // the block rows of columns i and im are in registers as well: column
+        // i's new rows come from the thread of im, which takes column i's old ones
+        if (tid == im - ib0) regs_store<E>(R, ng, S.ci);
+        if (tid == i - ib0) regs_store<E>(R, ng, S.xch);
+    } else if (tid == i - ib0) {
+        regs_store<E>(R, ng, S.ci);
+    }
+    __syncthreads();
+    if (im != i && tid == im - ib0) regs_load<E>(R, ng, S.xch);
+    const double uii = S.uii;
+    const double yi = S.yi;
+    if (tid == 0) MM(i, i) = uii;
+    // ---- row i of U inside the window (src/cholesky.jl:547-570)
+    if (own && c > i) {
+        const double acc = regs_dot<E>(R, ng, S.ci, have ? mic : MM(i, c));
+        const double u = acc / uii;
+        MM(i, c) = u;
+        R.cur[E] = u;
+        S.dsum[tid] += u * u;
+        S.sv[tid] += yi * u;
+    }
+    __syncthreads();
+    return true;
+}
+
+template <int E>
+__device__ __forceinline__ bool reg_pivots(const CholArgs& C, RegShared& S, ColRegs& R, int ng, int ib0, int ib1,
+                                           int w1, int pivot, double eps0)
+{
+    if constexpr (E < 16) {
+        if (ib0 + 16 * ng + E >= ib1) return false;  // uniform: the block is done
+        if (!reg_pivot<E>(C, S, R, ng, ib0, w1, pivot, eps0)) return false;
+        return reg_pivots<E + 1>(C, S, R, ng, ib0, ib1, w1, pivot, eps0);
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(kRegCols, 1) void chol_window_reg_kernel(CholArgs C, int w0, int w1, int ib0, int nb,
+                                                                      int pivot)
+{
+    if (*C.rank < C.n) return;  // an earlier pivot found the matrix rank deficient
+    __shared__ RegShared S;
+    const double* M = C.M;
+    const int64_t lda = C.lda;
+    const int tid = threadIdx.x;
+    const int ib1 = min(w1, ib0 + nb);
+    const int c = ib0 + tid;
+    const bool own = c < w1;
+    const double eps0 = *C.eps0;
+    if (own) {
+        S.sv[tid] = C.s_vec[c];
+        S.a[tid] = C.a[c];
+        S.b[tid] = C.b[c];
+        S.perm[tid] = C.perm[c];
+        S.dg[tid] = MM(c, c);
+        S.dsum[tid] = 0.0;  // d_vec reset at the block start (src/cholesky.jl:494-496)
+    }
+    if (tid == 0) S.stop = 0;
+    ColRegs R;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) R.g0[e] = R.g1[e] = R.g2[e] = R.cur[e] = 0.0;
+    __syncthreads();
+    for (int ng = 0; ng < kRegNb / 16; ++ng) {
+        if (!reg_pivots<0>(C, S, R, ng, ib0, ib1, w1, pivot, eps0)) break;
+        // the completed group moves to its slot
+        if (ng == 0) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) R.g0[e] = R.cur[e];
+        } else if (ng == 1) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) R.g1[e] = R.cur[e];
+        } else if (ng == 2) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) R.g2[e] = R.cur[e];
+        }
+    }
+    if (own) {
+        C.s_vec[c] = S.sv[tid];
+        C.a[c] = S.a[tid];
+        C.b[c] = S.b[tid];
+        C.perm[c] = S.perm[tid];
+    }
+}
+
 hipError_t chol_syrk(const CholArgs& C, int k0, int k1, int r0, int r1, hipStream_t s);
 rocblas_handle blas_handle(hipStream_t s);
 void* blas_handle_void(hipStream_t s) { return (void*)blas_handle(s); }
@@ -315,8 +607,12 @@ hipError_t chol_window(const CholArgs& C, int w0, int w1, int nb, bool pivot, hi
 {
     for (int ib0 = w0; ib0 < w1; ib0 += nb) {
         const int ib1 = std::min(w1, ib0 + nb);
-        hipLaunchKernelGGL(chol_window_kernel, dim3(1), dim3(kCholThreads), 0, s, C, w0, w1, ib0, nb,
-                           pivot ? 1 : 0);
+        if (w1 - ib0 <= kRegCols && nb <= kRegNb)
+            hipLaunchKernelGGL(chol_window_reg_kernel, dim3(1), dim3(kRegCols), 0, s, C, w0, w1, ib0, nb,
+                               pivot ? 1 : 0);
+        else
+            hipLaunchKernelGGL(chol_window_kernel, dim3(1), dim3(kCholThreads), 0, s, C, w0, w1, ib0, nb,
+                               pivot ? 1 : 0);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         // deferred update of the rest of the window by the tiled kernel
