@@ -385,8 +385,8 @@ int plan_build(dbg_plan* P, const double* Sigma, int64_t lda, const double* a, c
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     P->ncu = ncu;
     P->use_mfma = (o.kind == DBG_KIND_DENSE) && n >= kMfmaMinDim;
-    // the FP16 matrix-core contraction: mixed dense path, Richtmyer points
-    P->split = P->use_mfma && P->mixed && o.generator == DBG_GEN_RICHTMYER && o.contraction != DBG_CONTRACT_FP32;
+    // the FP16 matrix-core contraction: mixed dense path
+    P->split = P->use_mfma && P->mixed && o.contraction != DBG_CONTRACT_FP32;
     if (P->split)
         HIPCHK(dalloc(&P->uc16, (size_t)split16_cols(n) * (size_t)split16_col_stride(n) * sizeof(uint16_t)));
     rc = plan_factor(P);
@@ -730,9 +730,8 @@ int dbg_plan_create(dbg_plan** plan, const dbg_opts* opts, int32_t n, const doub
     if (o.n_bits < 1 || o.n_bits > 64) return fail(DBG_ERR_ARG, "n_bits out of range");
     if (o.contraction < DBG_CONTRACT_AUTO || o.contraction > DBG_CONTRACT_SPLIT16)
         return fail(DBG_ERR_ARG, "unknown contraction");
-    if (o.contraction == DBG_CONTRACT_SPLIT16 &&
-        (o.precision != DBG_PREC_MIXED || o.kind != DBG_KIND_DENSE || o.generator != DBG_GEN_RICHTMYER))
-        return fail(DBG_ERR_ARG, "the split-FP16 contraction is built for the mixed dense Richtmyer path");
+    if (o.contraction == DBG_CONTRACT_SPLIT16 && (o.precision != DBG_PREC_MIXED || o.kind != DBG_KIND_DENSE))
+        return fail(DBG_ERR_ARG, "the split-FP16 contraction is built for the mixed dense path");
     dbg_plan* P = new dbg_plan();
     P->o = o;
     P->n = n;

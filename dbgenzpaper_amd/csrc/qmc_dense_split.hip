@@ -131,6 +131,12 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
     __shared__ __attribute__((aligned(16))) _Float16 s_uc[2 * kChunk];
     float* const s_u = reinterpret_cast<float*>(s_uc);
     static_assert(kSB * kUS * sizeof(float) <= sizeof(s_uc), "staging fits the chunk buffers");
+    // Sobol points: the sub-block's tables (low Gray-code byte per lane, high
+    // part per tile), also in the chunk buffers, after the staging rows
+    constexpr bool SOBT = GEN == kGenSobol;
+    uint32_t* const s_sob = reinterpret_cast<uint32_t*>(s_u + kSB * kUS);
+    static_assert(kSB * kUS * sizeof(float) + (kSB * 256 + 2 * kSB) * sizeof(uint32_t) <= sizeof(s_uc),
+                  "Sobol tables fit the chunk buffers");
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -147,6 +153,8 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
         const int64_t jt = tile - (int64_t)k * A.tiles_per_rep;
         const int64_t j = jt * kTS + tid + 1;  // 1-based point index of this thread's sample
         const bool valid = j <= A.m;
+        SobolLane sl{};
+        if constexpr (SOBT) sl = sobol_lane(pts, jt * kTS + 1, j);
         _Float16* __restrict__ Ys16 = reinterpret_cast<_Float16*>(Yall + (size_t)blockIdx.x * (size_t)A.y_slot);
 
         typename Path<true, kPhiTail>::State st;
@@ -257,6 +265,7 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
                                 s_x[(wv * kWS + 16 * sb + 4 * lq + r) * kSXS + lr] = acc[sb][tb][r];
                     }
                 }
+                if constexpr (SOBT) sobol_tables<kSB, 256>(pts, gb, min(kSB, nd - gb), jt * kTS + 1, s_sob, tid);
                 const bool later = b + 1 < kNTB && gb + kSB < nd;  // the panel has targets after this block
                 __syncthreads();
                 if (later) {
@@ -274,7 +283,8 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
                 float yv[kSB];
                 walk_block_yv<true, GEN, kPhiTail, kTS, kSXS, kSB, false>(yv, pts, gb, cnt, k, j, tid, st, w, s_x,
                                                                         s_y, (float*)nullptr, 0, Ucol, A.ldc, ak,
-                                                                        bk, invk);
+                                                                        bk, invk, (float*)nullptr,
+                                                                        SOBT ? s_sob : nullptr, sl);
                 store_y16(Ys16, gb, tid, yv, cnt);
                 // the direct loads above write LDS without the compiler tracking them
                 if (later) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -342,10 +352,14 @@ hipError_t pack_split16(const float* Ucol, int n, void* uc, hipStream_t s)
 hipError_t launch_dense_split(const DenseArgs& A, int gen, const void* uc, int grid, hipStream_t s)
 {
     const int64_t ucs = split16_col_stride(A.n);
-    if (gen != kGenRichtmyer) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((dense_split_kernel<kGenRichtmyer>), dim3(grid), dim3(256), 0, s, A, (const _Float16*)uc, ucs,
-                       (const float*)A.Uk, (const float*)A.Ucol, (const float*)A.ak, (const float*)A.bk,
-                       (const float*)A.invk, (float*)A.Y, A.part_m, A.part_e, DBG_POINT_ARGS(A.pts));
+    if (gen == kGenRichtmyer)
+        hipLaunchKernelGGL((dense_split_kernel<kGenRichtmyer>), dim3(grid), dim3(256), 0, s, A, (const _Float16*)uc,
+                           ucs, (const float*)A.Uk, (const float*)A.Ucol, (const float*)A.ak, (const float*)A.bk,
+                           (const float*)A.invk, (float*)A.Y, A.part_m, A.part_e, DBG_POINT_ARGS(A.pts));
+    else
+        hipLaunchKernelGGL((dense_split_kernel<kGenSobol>), dim3(grid), dim3(256), 0, s, A, (const _Float16*)uc, ucs,
+                           (const float*)A.Uk, (const float*)A.Ucol, (const float*)A.ak, (const float*)A.bk,
+                           (const float*)A.invk, (float*)A.Y, A.part_m, A.part_e, DBG_POINT_ARGS(A.pts));
     return hipGetLastError();
 }
 

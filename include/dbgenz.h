@@ -58,7 +58,7 @@ extern "C" {
 #define DBG_CONTRACT_FP32 1     /* FP32 matrix cores (v_mfma_f32_16x16x4f32) */
 #define DBG_CONTRACT_SPLIT16 2  /* FP16 matrix cores, operands split into high and low
                                    FP16 parts, three products per pair, FP32 accumulation
-                                   (FP32-level accuracy; Richtmyer points only) */
+                                   (FP32-level accuracy) */
 
 /* Options: QMCOpts{T} (src/qmc.jl:15-27, defaults of QMC_opts :111-125) plus
  * the choices the Julia constructors take as arguments. */

@@ -157,8 +157,10 @@ def test_sobol_fp64_mfma(dbg, oracle, d, m, skip):
     assert np.allclose(D.qmc_reps, r.rep_means, rtol=F64_TOL, atol=0)
 
 
-@pytest.mark.parametrize("d,family", [(150, "random"), (300, "random"), (700, "factor"), (1100, "factor")])
-def test_split16_contraction(dbg, oracle, d, family):
+@pytest.mark.parametrize("d,family,gen", [(150, "random", "Richtmyer"), (300, "random", "Richtmyer"),
+                                          (700, "factor", "Richtmyer"), (1100, "factor", "Richtmyer"),
+                                          (300, "random", "Sobol"), (700, "factor", "Sobol")])
+def test_split16_contraction(dbg, oracle, d, family, gen):
     """The mixed dense kernel with its panel contraction on FP16 matrix cores
     (high/low FP16 split of both operands, three products, FP32 accumulation)
     against the FP64 oracle on the same factor, points and shifts, and against
@@ -167,17 +169,17 @@ def test_split16_contraction(dbg, oracle, d, family):
     sub-blocks, and the deep lower orthant of the factor model whose
     estimate is far below the FP64 range (compared in log P)."""
     if family == "random":
-        P = W.dense_random(d, m=2048, n_reps=3, seed=d)
+        P = W.dense_random(d, m=2048, n_reps=3, seed=d, gen=gen)
     else:
-        P = W.factor_model(d, m=2048, n_reps=2, seed=d)
-    print(f"split16 d={d} {family}", flush=True)
+        P = W.factor_model(d, m=2048, n_reps=2, seed=d, gen=gen)
+    print(f"split16 d={d} {family} {gen}", flush=True)
     res = {}
     opts = dbg.QMC_opts(np.float32, m=P.m, n_reps=P.n_reps, max_pts=P.m, max_abs_err=0.0)
     for c in ("split16", "fp32"):
-        D = dbg.QMCData(P.Sigma, P.a, P.b, opts, None, "Richtmyer", shifts=P.shifts, contraction=c)
+        D = dbg.QMCData(P.Sigma, P.a, P.b, opts, None, gen, shifts=P.shifts, contraction=c)
         dbg.qmc_pnorm(D)
         res[c] = (D.last_result.log_value, np.array(D.qmc_reps, dtype=np.float64))
-    D64 = _plan(dbg, P, phi_form="tail")
+    D64 = _plan(dbg, P, gen=gen, phi_form="tail")
     r = oracle_same_inputs(oracle, D64)
     ls, lf, lo = res["split16"][0], res["fp32"][0], r.log_value
     print(f"  log P split16 {ls:.10f} fp32 {lf:.10f} oracle {lo:.10f}", flush=True)
