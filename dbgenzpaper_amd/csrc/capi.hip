@@ -616,8 +616,16 @@ int run_grad(dbg_plan* P)
     A.mode = kModeRecord;
     A.Uk = P->Uk;
     A.Ucol = P->mixed ? (const void*)P->ucol32 : (const void*)P->F.M;
-    A.Y = G.yrec;
-    HIPCHK(launch_dense_adjoint(A, P->mixed, gen, G.grid, s));
+    if (P->split) {
+        // the split-FP16 kernel: its own y workspace for the contraction, FP32 records
+        A.Y = P->Y;
+        A.y_slot = P->y_slot;
+        HIPCHK(launch_dense_split(A, gen, P->uc16, P->grid, s));  // P->Y holds P->grid slots
+        A.y_slot = (int64_t)nd * G.tile;
+    } else {
+        A.Y = G.yrec;
+        HIPCHK(launch_dense_adjoint(A, P->mixed, gen, G.grid, s));
+    }
     HIPCHK(grad_norm(G.part_m, G.part_e, G.n_tiles, (double)o.m * o.n_reps, G.norm, s));
     // reverse sweep in flipped index order
     A.mode = kModeReverse;

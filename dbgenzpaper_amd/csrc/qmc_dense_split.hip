@@ -106,7 +106,10 @@ int dense_split_tile() { return kTS; }
 int dense_split_panel() { return kPW; }
 int64_t dense_split_slot(int nd) { return (int64_t)((nd + kSB - 1) / kSB) * kSB * kTS; }  // floats
 
-template <int GEN>
+// MODE kModeIntegrate: the estimate.  kModeRecord: the same, keeping y and
+// the conditional means of every sample (FP32 rows of A.rs samples in
+// yrec_all / srec_all) and the per-sample weights, for the adjoint.
+template <int GEN, int MODE>
 __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const _Float16* __restrict__ uc,
                                                             int64_t ucs, const float* __restrict__ Uk,
                                                             const float* __restrict__ Ucol,
@@ -115,8 +118,11 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
                                                             const float* __restrict__ invk,
                                                             float* __restrict__ Yall,
                                                             double* __restrict__ part_m,
-                                                            int* __restrict__ part_e, DBG_POINT_PARAMS)
+                                                            int* __restrict__ part_e,
+                                                            float* __restrict__ yrec_all,
+                                                            float* __restrict__ srec_all, DBG_POINT_PARAMS)
 {
+    constexpr bool REC = MODE == kModeRecord;
     using V = floatx4;
     // walk: accumulator hand-off [sample][16] and y of the current sub-block
     // [k][sample]; contraction: the y chunk, per wavefront 4 k-groups x
@@ -156,6 +162,8 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
         SobolLane sl{};
         if constexpr (SOBT) sl = sobol_lane(pts, jt * kTS + 1, j);
         _Float16* __restrict__ Ys16 = reinterpret_cast<_Float16*>(Yall + (size_t)blockIdx.x * (size_t)A.y_slot);
+        float* __restrict__ yrec = REC ? yrec_all + (size_t)tile * kTS : nullptr;
+        float* __restrict__ srec = REC ? srec_all + (size_t)tile * kTS : nullptr;
 
         typename Path<true, kPhiTail>::State st;
         Weight<float> w;
@@ -281,10 +289,9 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
                 }
                 const int cnt = min(kSB, nd - gb);
                 float yv[kSB];
-                walk_block_yv<true, GEN, kPhiTail, kTS, kSXS, kSB, false>(yv, pts, gb, cnt, k, j, tid, st, w, s_x,
-                                                                        s_y, (float*)nullptr, 0, Ucol, A.ldc, ak,
-                                                                        bk, invk, (float*)nullptr,
-                                                                        SOBT ? s_sob : nullptr, sl);
+                walk_block_yv<true, GEN, kPhiTail, kTS, kSXS, kSB, REC>(yv, pts, gb, cnt, k, j, tid, st, w, s_x, s_y,
+                                                                      yrec, REC ? A.rs : 0, Ucol, A.ldc, ak, bk,
+                                                                      invk, srec, SOBT ? s_sob : nullptr, sl);
                 store_y16(Ys16, gb, tid, yv, cnt);
                 // the direct loads above write LDS without the compiler tracking them
                 if (later) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -312,6 +319,10 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
             }
         }
         const double pm = valid ? (double)w.pm : 0.0;
+        if constexpr (REC) {
+            ((float*)A.wm)[(size_t)tile * kTS + tid] = valid ? w.pm : 0.0f;
+            A.we[(size_t)tile * kTS + tid] = w.pe;
+        }
         double S;
         int E;
         block_scaled_sum<256>(pm, w.pe, S, E);
@@ -349,17 +360,26 @@ hipError_t pack_split16(const float* Ucol, int n, void* uc, hipStream_t s)
     return hipGetLastError();
 }
 
+template <int GEN, int MODE>
+static void launch_s(const DenseArgs& A, const void* uc, int64_t ucs, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL((dense_split_kernel<GEN, MODE>), dim3(grid), dim3(256), 0, s, A, (const _Float16*)uc, ucs,
+                       (const float*)A.Uk, (const float*)A.Ucol, (const float*)A.ak, (const float*)A.bk,
+                       (const float*)A.invk, (float*)A.Y, A.part_m, A.part_e, (float*)A.yrec, (float*)A.srec,
+                       DBG_POINT_ARGS(A.pts));
+}
+
 hipError_t launch_dense_split(const DenseArgs& A, int gen, const void* uc, int grid, hipStream_t s)
 {
     const int64_t ucs = split16_col_stride(A.n);
-    if (gen == kGenRichtmyer)
-        hipLaunchKernelGGL((dense_split_kernel<kGenRichtmyer>), dim3(grid), dim3(256), 0, s, A, (const _Float16*)uc,
-                           ucs, (const float*)A.Uk, (const float*)A.Ucol, (const float*)A.ak, (const float*)A.bk,
-                           (const float*)A.invk, (float*)A.Y, A.part_m, A.part_e, DBG_POINT_ARGS(A.pts));
-    else
-        hipLaunchKernelGGL((dense_split_kernel<kGenSobol>), dim3(grid), dim3(256), 0, s, A, (const _Float16*)uc, ucs,
-                           (const float*)A.Uk, (const float*)A.Ucol, (const float*)A.ak, (const float*)A.bk,
-                           (const float*)A.invk, (float*)A.Y, A.part_m, A.part_e, DBG_POINT_ARGS(A.pts));
+    const bool rec = A.mode == kModeRecord;
+    if (gen == kGenRichtmyer) {
+        if (rec) launch_s<kGenRichtmyer, kModeRecord>(A, uc, ucs, grid, s);
+        else launch_s<kGenRichtmyer, kModeIntegrate>(A, uc, ucs, grid, s);
+    } else {
+        if (rec) launch_s<kGenSobol, kModeRecord>(A, uc, ucs, grid, s);
+        else launch_s<kGenSobol, kModeIntegrate>(A, uc, ucs, grid, s);
+    }
     return hipGetLastError();
 }
 
