@@ -294,6 +294,162 @@ __global__ __launch_bounds__(256, kUbOcc) void ubar_gemm_lds_kernel(const float*
                     acc[i][j][r];
 }
 
+// Mixed form on the FP16 matrix cores: per stage of 32 samples (one
+// instruction's K) both operands are split into high and low FP16 parts as in
+// the split-FP16 integration kernel (y' = 2^6 y; three products per pair,
+// FP32 accumulation).  The sbar values carry the sample weights and span far
+// more than the FP16 range, so each staged row of S (32 samples of one index
+// g) gets its own power-of-two scale that brings its largest magnitude into
+// [2^9, 2^10); the three products of a stage start from zero and the stage's
+// result is added to the FP32 tile sums with each column's scale.  The
+// staging threads convert the FP32 records; the tiles go to LDS as [row][32
+// high | 32 low] halves with the 16-byte granules of row r permuted by r mod 8
+// (conflict-free operand reads).
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+constexpr int kUsRow = 64;  // halves per staged row (32 samples, high and low)
+
+__device__ __forceinline__ void ub_split(float v, _Float16& hi, _Float16& lo)
+{
+    v = fabsf(v) > 65504.0f ? copysignf(65504.0f, v) : v;
+    hi = (_Float16)v;
+    lo = (_Float16)(v - (float)hi);
+}
+
+__global__ __launch_bounds__(256, 2) void ubar_gemm_split_kernel(const float* __restrict__ Y,
+                                                                const float* __restrict__ S, int64_t rs, int nd,
+                                                                const int2* __restrict__ pairs, int n_pairs,
+                                                                int64_t kchunk, int64_t kmax,
+                                                                float* __restrict__ part)
+{
+    using V = Mfma<float>::V;
+    using L4 = Vec4<float>::V;
+    __shared__ __attribute__((aligned(16))) _Float16 s_a[2][kUbTile * kUsRow];
+    __shared__ __attribute__((aligned(16))) _Float16 s_b[2][kUbTile * kUsRow];
+    __shared__ float s_sc[2][kUbTile];  // column scale of each staged S row (2^-10+e, with y's 2^-6)
+    // XCD-aware order: workgroups are dealt to the eight XCDs round-robin by
+    // dispatch index, so consecutive logical indices (the pairs of one chunk,
+    // which read the same sample range of a few row tiles) are given to one
+    // XCD and share its L2 (when the grid divides by 8; else dispatch order).
+    const int64_t did = blockIdx.x + (int64_t)gridDim.x * blockIdx.y;
+    const int64_t total = (int64_t)gridDim.x * gridDim.y;
+    const int64_t lid = (total & 7) ? did : (did & 7) * (total >> 3) + (did >> 3);
+    const int pair = (int)(lid % n_pairs), kc = (int)(lid / n_pairs);
+    const int2 tp = pairs[pair];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    const int wr = wv & 1, wc = wv >> 1;
+    const bool idle = tp.x == tp.y && wr > wc;  // below the diagonal: stages operands only
+    const int64_t k0 = (int64_t)kc * kchunk;
+    const int64_t k1 = min(kmax, k0 + kchunk);
+    // staging: thread -> tile row tid / 2, samples 16 (tid % 2) .. + 15 of a stage
+    const int srow = tid >> 1, shalf = tid & 1;
+    const int la = min(tp.x * kUbTile + srow, nd - 1);  // rows past nd read row nd-1; discarded
+    const int lb = min(tp.y * kUbTile + srow, nd - 1);
+    const float* __restrict__ ga = Y + (size_t)la * rs + 16 * shalf;
+    const float* __restrict__ gb = S + (size_t)lb * rs + 16 * shalf;
+    // granule positions of this thread's staged samples: high granules 2h, 2h+1, low 4+2h, 5+2h
+    const int sw = srow & 7;
+    const int phA = ((2 * shalf) ^ sw) * 8, phB = ((2 * shalf + 1) ^ sw) * 8;
+    const int plA = ((4 + 2 * shalf) ^ sw) * 8, plB = ((5 + 2 * shalf) ^ sw) * 8;
+    // operand reads: row 16 i + lr (mod 8 = lr mod 8), granules lq / 4 + lq
+    const int rh = (lq ^ (lr & 7)) * 8, rl = ((4 + lq) ^ (lr & 7)) * 8;
+    V tot[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) tot[i][j] = V{0, 0, 0, 0};
+    const int nst = (int)((k1 - k0) / kUbK);
+    L4 va[4], vb[4];
+    auto fetch = [&](int64_t kn) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            va[e] = *(const L4*)(ga + kn + 4 * e);
+            vb[e] = *(const L4*)(gb + kn + 4 * e);
+        }
+    };
+    auto stage = [&](int buf) {
+        half8 h0, h1, l0, l1;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            _Float16 hi, lo;
+            ub_split(va[e >> 2][e & 3] * 64.0f, hi, lo);
+            if (e < 8) { h0[e] = hi; l0[e] = lo; } else { h1[e - 8] = hi; l1[e - 8] = lo; }
+        }
+        _Float16* wa = &s_a[buf][srow * kUsRow];
+        *(half8*)(wa + phA) = h0;
+        *(half8*)(wa + phB) = h1;
+        *(half8*)(wa + plA) = l0;
+        *(half8*)(wa + plB) = l1;
+        // the row's scale: largest magnitude of its 32 samples (this thread and
+        // its partner) to [2^9, 2^10)
+        float mx = 0.0f;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) mx = fmaxf(mx, fabsf(vb[e >> 2][e & 3]));
+        mx = fmaxf(mx, __shfl_xor(mx, 1, 64));
+        int ex = 0;
+        (void)frexpf(mx, &ex);
+        const int sh = (mx > 0.0f && isfinite(mx)) ? 10 - ex : 0;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            _Float16 hi, lo;
+            ub_split(ldexpf(vb[e >> 2][e & 3], sh), hi, lo);
+            if (e < 8) { h0[e] = hi; l0[e] = lo; } else { h1[e - 8] = hi; l1[e - 8] = lo; }
+        }
+        _Float16* wb = &s_b[buf][srow * kUsRow];
+        *(half8*)(wb + phA) = h0;
+        *(half8*)(wb + phB) = h1;
+        *(half8*)(wb + plA) = l0;
+        *(half8*)(wb + plB) = l1;
+        if (shalf == 0) s_sc[buf][srow] = ldexpf(1.0f, -sh - 6);
+    };
+    if (nst > 0) {
+        fetch(k0);
+        stage(0);
+        __syncthreads();
+    }
+    for (int st = 0; st < nst; ++st) {
+        const int buf = st & 1;
+        if (st + 1 < nst) fetch(k0 + (int64_t)(st + 1) * kUbK);  // uniform
+        if (!idle) {
+            const _Float16* ra = &s_a[buf][(wr * 64 + lr) * kUsRow];
+            const _Float16* rb = &s_b[buf][(wc * 64 + lr) * kUsRow];
+            half8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                ah[q] = *(const half8*)(ra + 16 * q * kUsRow + rh);
+                al[q] = *(const half8*)(ra + 16 * q * kUsRow + rl);
+                bh[q] = *(const half8*)(rb + 16 * q * kUsRow + rh);
+                bl[q] = *(const half8*)(rb + 16 * q * kUsRow + rl);
+            }
+            float sc[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) sc[j] = s_sc[buf][wc * 64 + 16 * j + lr];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    V acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[j], V{0, 0, 0, 0}, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc, 0, 0, 0);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) tot[i][j][r] = fmaf(acc[r], sc[j], tot[i][j][r]);
+                }
+        }
+        if (st + 1 < nst) stage(buf ^ 1);
+        __syncthreads();
+    }
+    if (idle) return;
+    float* __restrict__ out = part + ((size_t)kc * n_pairs + pair) * (kUbTile * kUbTile);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                out[(size_t)(wr * 64 + 16 * i + Mfma<float>::row(lq, r)) * kUbTile + wc * 64 + 16 * j + lr] =
+                    tot[i][j][r];
+}
+
 // Ub[l, g + 1] += sum over chunks of the partial tiles (l <= g < nd), FP64
 template <typename T>
 __global__ __launch_bounds__(256) void ubar_reduce_kernel(const T* __restrict__ part, const int2* __restrict__ pairs,
@@ -313,14 +469,19 @@ __global__ __launch_bounds__(256) void ubar_reduce_kernel(const T* __restrict__ 
 }
 
 hipError_t ubar_product(const void* Y, const void* S, int64_t rs, int nd, bool mixed, const int2* pairs,
-                        int n_pairs, int n_chunks, int64_t kchunk, void* part, double* Ub, int n, hipStream_t s)
+                        int n_pairs, int n_chunks, int64_t kchunk, void* part, double* Ub, int n, hipStream_t s,
+                        bool split16)
 {
     dim3 g1(n_pairs, n_chunks), b(256);
     dim3 g2(n_pairs, 16);
     if (mixed) {
         if (kchunk % kUbK != 0 || rs % kUbK != 0) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(ubar_gemm_lds_kernel, g1, b, 0, s, (const float*)Y, (const float*)S, rs, nd, pairs,
-                           n_pairs, kchunk, rs, (float*)part);
+        if (split16)
+            hipLaunchKernelGGL(ubar_gemm_split_kernel, g1, b, 0, s, (const float*)Y, (const float*)S, rs, nd, pairs,
+                               n_pairs, kchunk, rs, (float*)part);
+        else
+            hipLaunchKernelGGL(ubar_gemm_lds_kernel, g1, b, 0, s, (const float*)Y, (const float*)S, rs, nd, pairs,
+                               n_pairs, kchunk, rs, (float*)part);
         hipLaunchKernelGGL(ubar_reduce_kernel<float>, g2, b, 0, s, (const float*)part, pairs, n_pairs, n_chunks,
                            nd, Ub, n);
     } else {

@@ -636,7 +636,7 @@ int run_grad(dbg_plan* P)
     HIPCHK(launch_dense_adjoint(A, P->mixed, gen, G.grid, s));
     // Ubar[l, g + 1] = sum_samples y_l sbar_{g+1}: split-K on the matrix cores, FP64 chunk sums
     HIPCHK(ubar_product(G.yrec, G.srec, G.rs, nd, P->mixed, G.pairs, G.n_pairs, G.n_chunks, G.kchunk, G.part,
-                        G.ub, n, s));
+                        G.ub, n, s, P->split));
     rocblas_handle h = (rocblas_handle)blas_handle_void(s);
     if (!h) return fail(DBG_ERR_DEVICE, "rocBLAS handle");
     HIPCHK(sigma_chain(G.ub, n, G.acc + 2 * n, nullptr, s, 0, nullptr, nullptr, nullptr, nullptr, nullptr,

@@ -93,8 +93,10 @@ hipError_t launch_dense_mfma(const DenseArgs& A, bool mixed, int gen, int form, 
 hipError_t launch_dense_adjoint(const DenseArgs& A, bool mixed, int gen, int grid, hipStream_t s);
 hipError_t pack_flip(const double* U, int n, bool mixed, void* Ufk, int64_t ldu, void* Ufc, hipStream_t s);
 hipError_t grad_norm(const double* pm, const int* pe, int64_t n_tiles, double total, double* norm, hipStream_t s);
+// split16: the mixed product on FP16 matrix cores (high/low split, per-row scales of S)
 hipError_t ubar_product(const void* Y, const void* S, int64_t rs, int nd, bool mixed, const int2* pairs,
-                        int n_pairs, int n_chunks, int64_t kchunk, void* part, double* Ub, int n, hipStream_t s);
+                        int n_pairs, int n_chunks, int64_t kchunk, void* part, double* Ub, int n, hipStream_t s,
+                        bool split16);
 int ubar_tile();
 // step 0: Ubar diagonal; 1: Phi(X); 2: C_bar -> G, ga, gb
 hipError_t sigma_chain(double* Ub, int n, const double* gd, double* X, hipStream_t s, int step,
