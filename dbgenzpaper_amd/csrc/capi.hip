@@ -265,6 +265,7 @@ struct dbg_plan {
         double* acc = nullptr;  // 3 n: d log P / d a', b', U_rr
         void* ufk = nullptr;    // flipped factor, packed rows (T)
         void* ufc = nullptr;    // flipped factor, column-major (T)
+        void* uc16f = nullptr;  // flipped factor, split FP16 layout (mixed split path)
         void* part = nullptr;   // split-K partial tiles of Y^T Sbar (T)
         int2* pairs = nullptr;  // upper-triangle tile pairs
         int n_pairs = 0, n_chunks = 0;
@@ -275,7 +276,7 @@ struct dbg_plan {
         double* gab = nullptr;  // 2 n: d log P / d a, b
         std::vector<void*> all() const
         {
-            return {yrec, srec, wm, we, ywork, part_m, part_e, norm, acc, ufk, ufc, part, pairs, ub, x, g, gab};
+            return {yrec, srec, wm, we, ywork, part_m, part_e, norm, acc, ufk, ufc, uc16f, part, pairs, ub, x, g, gab};
         }
     } G;
 
@@ -541,6 +542,8 @@ int grad_alloc(dbg_plan* P)
     HIPCHK(dalloc(&G.acc, (size_t)3 * n * sizeof(double)));
     HIPCHK(dalloc(&G.ufk, (size_t)nd * P->ldu * T));
     HIPCHK(dalloc(&G.ufc, (size_t)n * n * T + 256 * T));
+    if (P->split)
+        HIPCHK(dalloc(&G.uc16f, (size_t)split16_cols(n) * (size_t)split16_col_stride(n) * sizeof(uint16_t)));
     {
         // split-K layout of Ubar = Y^T Sbar: upper-triangle pairs of 128-row tiles,
         // about twelve workgroups per CU in whole rounds of the two resident
@@ -632,8 +635,16 @@ int run_grad(dbg_plan* P)
     A.a_inf = P->a_inf ? 1 : 0;
     A.Uk = G.ufk;
     A.Ucol = G.ufc;
-    A.Y = G.ywork;
-    HIPCHK(launch_dense_adjoint(A, P->mixed, gen, G.grid, s));
+    if (P->split) {
+        // the split-FP16 kernel on the flipped factor
+        HIPCHK(pack_split16((const float*)G.ufc, n, G.uc16f, s));
+        A.Y = P->Y;
+        A.y_slot = P->y_slot;
+        HIPCHK(launch_dense_split(A, gen, G.uc16f, P->grid, s));
+    } else {
+        A.Y = G.ywork;
+        HIPCHK(launch_dense_adjoint(A, P->mixed, gen, G.grid, s));
+    }
     // Ubar[l, g + 1] = sum_samples y_l sbar_{g+1}: split-K on the matrix cores, FP64 chunk sums
     HIPCHK(ubar_product(G.yrec, G.srec, G.rs, nd, P->mixed, G.pairs, G.n_pairs, G.n_chunks, G.kchunk, G.part,
                         G.ub, n, s, P->split));

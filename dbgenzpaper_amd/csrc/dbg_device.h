@@ -792,7 +792,11 @@ struct RevOps {
     }
 };
 
-template <bool MIXED, int GEN, int TS, int SXS, int SB, int CC, typename T>
+// unsc: the factor that undoes a per-sample scale of wt (and so of every
+// sbar and ybar of the sample) where values leave the sample: the recorded
+// sbar and the per-index sums (1 without such a scale).  YST = false leaves
+// the sub-block's sbar in sv for the caller instead of storing them to Ys.
+template <bool MIXED, int GEN, int TS, int SXS, int SB, int CC, bool YST = true, typename T>
 __device__ __forceinline__ void rev_steps(const PointArgs& pts, int n, int gb, int cnt, int k, int64_t j, int t,
                                           T& ybar, T wt, const T* __restrict__ s_x, T* __restrict__ s_y,
                                           T* __restrict__ Ys, const T* __restrict__ Uf, int64_t ldc,
@@ -800,7 +804,7 @@ __device__ __forceinline__ void rev_steps(const PointArgs& pts, int n, int gb, i
                                           const T* __restrict__ invk, T* __restrict__ srec,
                                           const T* __restrict__ yrec, int64_t rs, double* __restrict__ red,
                                           T (&sv)[SB], T& s_nx, T& y_nx, RevOps<MIXED, GEN, SB, T> (&ops)[2],
-                                          bool a_inf)
+                                          bool a_inf, T unsc)
 {
     if constexpr (CC < SB) {
         if (CC < cnt) {  // uniform
@@ -820,19 +824,19 @@ __device__ __forceinline__ void rev_steps(const PointArgs& pts, int n, int gb, i
             const T sbar = rev_step<MIXED, GEN>(pts, cur.pt, n, r, j, ybar, wt, s_r, y_r, cur.a, cur.b, cur.inv, o);
             sv[CC] = sbar;
             s_y[CC * TS + t] = sbar;
-            Ys[(size_t)g * TS + t] = sbar;
-            srec[(size_t)(r - 1) * rs + t] = sbar;
+            if constexpr (YST) Ys[(size_t)g * TS + t] = sbar;
+            srec[(size_t)(r - 1) * rs + t] = sbar * unsc;
             // per-index sums over the wavefront: FP32 partials of 64 FP32
             // terms on the mixed path, FP64 on the FP64 path
             double a0 = 0.0, b0, d0;
             if constexpr (MIXED) {
-                if (!a_inf) a0 = wave_sum_f32((float)o.ga);  // uniform; 0 when a' = -inf throughout
-                b0 = wave_sum_f32((float)o.gb);
-                d0 = wave_sum_f32((float)o.gd);
+                if (!a_inf) a0 = wave_sum_f32((float)o.ga * unsc);  // uniform; 0 when a' = -inf throughout
+                b0 = wave_sum_f32((float)o.gb * unsc);
+                d0 = wave_sum_f32((float)o.gd * unsc);
             } else {
-                if (!a_inf) a0 = wave_sum_f64(o.ga);
-                b0 = wave_sum_f64(o.gb);
-                d0 = wave_sum_f64(o.gd);
+                if (!a_inf) a0 = wave_sum_f64(o.ga * unsc);
+                b0 = wave_sum_f64(o.gb * unsc);
+                d0 = wave_sum_f64(o.gd * unsc);
             }
             if ((t & 63) == 0) {
                 double* rr = red + ((t >> 6) * SB + CC) * 3;
@@ -844,8 +848,9 @@ __device__ __forceinline__ void rev_steps(const PointArgs& pts, int n, int gb, i
 #pragma unroll
             for (int l = 0; l <= CC; ++l) acc = fma(sv[l], cur.u[l], acc);
             ybar = acc;
-            rev_steps<MIXED, GEN, TS, SXS, SB, CC + 1>(pts, n, gb, cnt, k, j, t, ybar, wt, s_x, s_y, Ys, Uf, ldc, ak,
-                                                       bk, invk, srec, yrec, rs, red, sv, s_nx, y_nx, ops, a_inf);
+            rev_steps<MIXED, GEN, TS, SXS, SB, CC + 1, YST>(pts, n, gb, cnt, k, j, t, ybar, wt, s_x, s_y, Ys, Uf, ldc,
+                                                            ak, bk, invk, srec, yrec, rs, red, sv, s_nx, y_nx, ops,
+                                                            a_inf, unsc);
         }
     }
 }
@@ -858,17 +863,16 @@ __device__ __forceinline__ void rev_steps(const PointArgs& pts, int n, int gb, i
 // to the flipped workspace (s_y, Ys) for the contraction and to the record
 // srec[(r - 1) * rs + t] in place of s_r for the gradient of U.  Per index,
 // the three sums over the workgroup's samples are left in red[wave][cc][3].
-template <bool MIXED, int GEN, int TS, int SXS, int SB, typename T>
-__device__ __forceinline__ void walk_block_rev(const PointArgs& pts, int n, int gb, int cnt, int k, int64_t j,
-                                               int t, T& ybar, T wt, const T* __restrict__ s_x,
-                                               T* __restrict__ s_y, T* __restrict__ Ys,
-                                               const T* __restrict__ Uf, int64_t ldc,
-                                               const T* __restrict__ ak, const T* __restrict__ bk,
-                                               const T* __restrict__ invk, T* __restrict__ srec,
-                                               const T* __restrict__ yrec, int64_t rs,
-                                               double* __restrict__ red, bool a_inf)
+template <bool MIXED, int GEN, int TS, int SXS, int SB, bool YST, typename T>
+__device__ __forceinline__ void walk_block_rev_sv(T (&sv)[SB], const PointArgs& pts, int n, int gb, int cnt, int k,
+                                                  int64_t j, int t, T& ybar, T wt, const T* __restrict__ s_x,
+                                                  T* __restrict__ s_y, T* __restrict__ Ys,
+                                                  const T* __restrict__ Uf, int64_t ldc,
+                                                  const T* __restrict__ ak, const T* __restrict__ bk,
+                                                  const T* __restrict__ invk, T* __restrict__ srec,
+                                                  const T* __restrict__ yrec, int64_t rs,
+                                                  double* __restrict__ red, bool a_inf, T unsc)
 {
-    T sv[SB];
     // the records of a step are read one step ahead (they stream from HBM);
     // step cc overwrites record row r - 1 only after reading it, and the
     // next step reads row r - 2.  The scalar operands are fetched one step
@@ -881,8 +885,23 @@ __device__ __forceinline__ void walk_block_rev(const PointArgs& pts, int n, int 
         y_nx = (r0 <= n - 2) ? yrec[(size_t)r0 * rs + t] : T(0);
         ops[0].template fetch<0>(pts, n, gb, k, Uf, ldc, ak, bk, invk);
     }
-    rev_steps<MIXED, GEN, TS, SXS, SB, 0>(pts, n, gb, cnt, k, j, t, ybar, wt, s_x, s_y, Ys, Uf, ldc, ak, bk, invk,
-                                          srec, yrec, rs, red, sv, s_nx, y_nx, ops, a_inf);
+    rev_steps<MIXED, GEN, TS, SXS, SB, 0, YST>(pts, n, gb, cnt, k, j, t, ybar, wt, s_x, s_y, Ys, Uf, ldc, ak, bk,
+                                               invk, srec, yrec, rs, red, sv, s_nx, y_nx, ops, a_inf, unsc);
+}
+
+template <bool MIXED, int GEN, int TS, int SXS, int SB, typename T>
+__device__ __forceinline__ void walk_block_rev(const PointArgs& pts, int n, int gb, int cnt, int k, int64_t j,
+                                               int t, T& ybar, T wt, const T* __restrict__ s_x,
+                                               T* __restrict__ s_y, T* __restrict__ Ys,
+                                               const T* __restrict__ Uf, int64_t ldc,
+                                               const T* __restrict__ ak, const T* __restrict__ bk,
+                                               const T* __restrict__ invk, T* __restrict__ srec,
+                                               const T* __restrict__ yrec, int64_t rs,
+                                               double* __restrict__ red, bool a_inf)
+{
+    T sv[SB];
+    walk_block_rev_sv<MIXED, GEN, TS, SXS, SB, true>(sv, pts, n, gb, cnt, k, j, t, ybar, wt, s_x, s_y, Ys, Uf, ldc,
+                                                      ak, bk, invk, srec, yrec, rs, red, a_inf, T(1));
 }
 
 // ---------------------------------------------------------------------------

@@ -79,16 +79,17 @@ __device__ __forceinline__ void split16(float v, _Float16& hi, _Float16& lo)
     lo = (_Float16)(v - (float)hi);
 }
 
-// the sub-block's 16 y values of one sample -> two k-groups of the slot
+// the sub-block's 16 y values of one sample (times the prescale) -> two
+// k-groups of the slot
 __device__ __forceinline__ void store_y16(_Float16* __restrict__ Ys16, int gb, int t, const float (&yv)[kSB],
-                                          int cnt)
+                                          int cnt, float scale)
 {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         half8 hi, lo;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            const float v = (8 * h + e < cnt) ? yv[8 * h + e] * kYScale : 0.0f;
+            const float v = (8 * h + e < cnt) ? yv[8 * h + e] * scale : 0.0f;
             _Float16 a, b;
             split16(v, a, b);
             hi[e] = a;
@@ -109,6 +110,12 @@ int64_t dense_split_slot(int nd) { return (int64_t)((nd + kSB - 1) / kSB) * kSB 
 // MODE kModeIntegrate: the estimate.  kModeRecord: the same, keeping y and
 // the conditional means of every sample (FP32 rows of A.rs samples in
 // yrec_all / srec_all) and the per-sample weights, for the adjoint.
+// kModeReverse: the adjoint's reverse sweep in flipped index order (uc, Uk,
+// Ucol hold the flipped factor; the "y" of a step is sbar), as in the FP32
+// matrix-core kernel, with each sample's normalised weight wt scaled to
+// [1/2, 1) by a power of two so that its sbar values (all proportional to
+// wt) suit the FP16 split; the scale is undone where values leave the sample
+// (recorded sbar, per-index sums).
 template <int GEN, int MODE>
 __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const _Float16* __restrict__ uc,
                                                             int64_t ucs, const float* __restrict__ Uk,
@@ -123,6 +130,11 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
                                                             float* __restrict__ srec_all, DBG_POINT_PARAMS)
 {
     constexpr bool REC = MODE == kModeRecord;
+    constexpr bool REV = MODE == kModeReverse;
+    // prescale of the split operand: y' = 2^6 y; in the reverse sweep the
+    // scaled sbar are O(1) .. O(100) and go unscaled
+    constexpr float kOpScale = REV ? 1.0f : kYScale;
+    constexpr float kBack = REV ? 1.0f / kUScale : kAccScale;
     using V = floatx4;
     // walk: accumulator hand-off [sample][16] and y of the current sub-block
     // [k][sample]; contraction: the y chunk, per wavefront 4 k-groups x
@@ -139,7 +151,8 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
     static_assert(kSB * kUS * sizeof(float) <= sizeof(s_uc), "staging fits the chunk buffers");
     // Sobol points: the sub-block's tables (low Gray-code byte per lane, high
     // part per tile), also in the chunk buffers, after the staging rows
-    constexpr bool SOBT = GEN == kGenSobol;
+    constexpr bool SOBT = GEN == kGenSobol && !REV;
+    __shared__ double s_red[REV ? 4 * kSB * 3 : 1];
     uint32_t* const s_sob = reinterpret_cast<uint32_t*>(s_u + kSB * kUS);
     static_assert(kSB * kUS * sizeof(float) + (kSB * 256 + 2 * kSB) * sizeof(uint32_t) <= sizeof(s_uc),
                   "Sobol tables fit the chunk buffers");
@@ -162,12 +175,26 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
         SobolLane sl{};
         if constexpr (SOBT) sl = sobol_lane(pts, jt * kTS + 1, j);
         _Float16* __restrict__ Ys16 = reinterpret_cast<_Float16*>(Yall + (size_t)blockIdx.x * (size_t)A.y_slot);
-        float* __restrict__ yrec = REC ? yrec_all + (size_t)tile * kTS : nullptr;
-        float* __restrict__ srec = REC ? srec_all + (size_t)tile * kTS : nullptr;
+        float* __restrict__ yrec = (REC || REV) ? yrec_all + (size_t)tile * kTS : nullptr;
+        float* __restrict__ srec = (REC || REV) ? srec_all + (size_t)tile * kTS : nullptr;
 
         typename Path<true, kPhiTail>::State st;
         Weight<float> w;
-        Path<true, kPhiTail>::init(st, w, A.first);
+        float ybar = 0.0f, wt = 0.0f, unsc = 0.0f;
+        if constexpr (REV) {
+            if (valid) {
+                const size_t idx = (size_t)tile * kTS + tid;
+                const float pmv = ((const float*)A.wm)[idx];
+                if (pmv != 0.0f) {
+                    const double w64 = ldexp((double)pmv, A.we[idx] - (int)A.norm[0]) * A.norm[1];
+                    int e = 0;
+                    wt = (float)frexp(w64, &e);
+                    unsc = ldexpf(1.0f, e);  // 0 below the FP32 range, as wt itself would be
+                }
+            }
+        } else {
+            Path<true, kPhiTail>::init(st, w, A.first);
+        }
 
         for (int g0 = 0; g0 < nd; g0 += kPW) {
             // ---- S = Y[:, 0:g0] * U[0:g0, g0+1 : g0+kPW+1] on FP16 matrix cores
@@ -255,7 +282,7 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
 #pragma unroll
                 for (int sb = 0; sb < kNSB; ++sb)
 #pragma unroll
-                    for (int tb = 0; tb < kNTB; ++tb) acc[sb][tb] *= kAccScale;
+                    for (int tb = 0; tb < kNTB; ++tb) acc[sb][tb] *= kBack;
             }
             // ---- walk the panel in sub-blocks of 16 dimensions
             for (int b = 0; b < kNTB; ++b) {
@@ -289,13 +316,30 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
                 }
                 const int cnt = min(kSB, nd - gb);
                 float yv[kSB];
-                walk_block_yv<true, GEN, kPhiTail, kTS, kSXS, kSB, REC>(yv, pts, gb, cnt, k, j, tid, st, w, s_x, s_y,
-                                                                      yrec, REC ? A.rs : 0, Ucol, A.ldc, ak, bk,
-                                                                      invk, srec, SOBT ? s_sob : nullptr, sl);
-                store_y16(Ys16, gb, tid, yv, cnt);
+                if constexpr (REV)
+                    walk_block_rev_sv<true, GEN, kTS, kSXS, kSB, false>(yv, pts, n, gb, cnt, k, j, tid, ybar, wt, s_x,
+                                                                        s_y, (float*)nullptr, Ucol, A.ldc, ak, bk,
+                                                                        invk, srec, yrec, A.rs, s_red,
+                                                                        A.a_inf != 0, unsc);
+                else
+                    walk_block_yv<true, GEN, kPhiTail, kTS, kSXS, kSB, REC>(yv, pts, gb, cnt, k, j, tid, st, w, s_x,
+                                                                          s_y, yrec, REC ? A.rs : 0, Ucol, A.ldc, ak,
+                                                                          bk, invk, srec, SOBT ? s_sob : nullptr, sl);
+                store_y16(Ys16, gb, tid, yv, cnt, kOpScale);
                 // the direct loads above write LDS without the compiler tracking them
                 if (later) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
                 __syncthreads();
+                if constexpr (REV) {
+                    // per-index sums of the four waves -> the global accumulators
+                    if (tid < 3 * cnt) {
+                        const int cc = tid / 3, q = tid - 3 * cc;
+                        double v = 0.0;
+#pragma unroll
+                        for (int ww = 0; ww < kTS / 64; ++ww) v += s_red[(ww * kSB + cc) * 3 + q];
+                        double* dst = q == 0 ? A.ga : (q == 1 ? A.gb : A.gd);
+                        atomicAdd(dst + (n - 1 - (gb + cc)), v);
+                    }
+                }
                 // later targets of the panel += Y_sub * U[sub, later] (FP32 matrix cores)
                 if (later) {
 #pragma unroll
@@ -317,6 +361,20 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
                 }
                 __syncthreads();
             }
+        }
+        if constexpr (REV) {
+            // index 0: s_0 = 0, no sbar to pass on
+            RevOut o{0.0, 0.0, 0.0};
+            (void)rev_step<true, GEN>(pts, n, 0, k, j, ybar, wt, 0.0f, n >= 2 ? yrec[tid] : 0.0f, ak[0], bk[0],
+                                      invk[0], o);
+            const double a0 = wave_sum_f64(o.ga * unsc), b0 = wave_sum_f64(o.gb * unsc),
+                         d0 = wave_sum_f64(o.gd * unsc);
+            if (lane == 0) {
+                atomicAdd(A.ga, a0);
+                atomicAdd(A.gb, b0);
+                atomicAdd(A.gd, d0);
+            }
+            continue;
         }
         const double pm = valid ? (double)w.pm : 0.0;
         if constexpr (REC) {
@@ -372,12 +430,13 @@ static void launch_s(const DenseArgs& A, const void* uc, int64_t ucs, int grid, 
 hipError_t launch_dense_split(const DenseArgs& A, int gen, const void* uc, int grid, hipStream_t s)
 {
     const int64_t ucs = split16_col_stride(A.n);
-    const bool rec = A.mode == kModeRecord;
     if (gen == kGenRichtmyer) {
-        if (rec) launch_s<kGenRichtmyer, kModeRecord>(A, uc, ucs, grid, s);
+        if (A.mode == kModeRecord) launch_s<kGenRichtmyer, kModeRecord>(A, uc, ucs, grid, s);
+        else if (A.mode == kModeReverse) launch_s<kGenRichtmyer, kModeReverse>(A, uc, ucs, grid, s);
         else launch_s<kGenRichtmyer, kModeIntegrate>(A, uc, ucs, grid, s);
     } else {
-        if (rec) launch_s<kGenSobol, kModeRecord>(A, uc, ucs, grid, s);
+        if (A.mode == kModeRecord) launch_s<kGenSobol, kModeRecord>(A, uc, ucs, grid, s);
+        else if (A.mode == kModeReverse) launch_s<kGenSobol, kModeReverse>(A, uc, ucs, grid, s);
         else launch_s<kGenSobol, kModeIntegrate>(A, uc, ucs, grid, s);
     }
     return hipGetLastError();
