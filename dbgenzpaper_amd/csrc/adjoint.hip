@@ -307,6 +307,7 @@ __global__ __launch_bounds__(256, kUbOcc) void ubar_gemm_lds_kernel(const float*
 // (conflict-free operand reads).
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 constexpr int kUsRow = 64;  // halves per staged row (32 samples, high and low)
+constexpr int kUbTile16 = 256;  // output tile of the split form
 
 __device__ __forceinline__ void ub_split(float v, _Float16& hi, _Float16& lo)
 {
@@ -315,17 +316,30 @@ __device__ __forceinline__ void ub_split(float v, _Float16& hi, _Float16& lo)
     lo = (_Float16)(v - (float)hi);
 }
 
-__global__ __launch_bounds__(256, 2) void ubar_gemm_split_kernel(const float* __restrict__ Y,
-                                                                const float* __restrict__ S, int64_t rs, int nd,
-                                                                const int2* __restrict__ pairs, int n_pairs,
-                                                                int64_t kchunk, int64_t kmax,
-                                                                float* __restrict__ part)
+// TT: output tile (128: four wavefronts of 64 x 64, two workgroups per CU;
+// 256: eight wavefronts of 64 x 128, one workgroup per CU, which reads each
+// operand row for half as many tile pairs)
+template <int TT>
+struct UbSplitCfg {
+    static constexpr int NT = 2 * TT;           // threads: two per staged row
+    static constexpr int WC = TT == 256 ? 128 : 64;  // columns per wavefront
+    static constexpr int NJ = WC / 16;
+    static constexpr int NWC = TT / WC;         // wavefront columns
+    static constexpr int OCC = TT == 256 ? 1 : 2;
+};
+
+template <int TT>
+__global__ __launch_bounds__(UbSplitCfg<TT>::NT, UbSplitCfg<TT>::OCC) void ubar_gemm_split_kernel(
+    const float* __restrict__ Y, const float* __restrict__ S, int64_t rs, int nd, const int2* __restrict__ pairs,
+    int n_pairs, int64_t kchunk, int64_t kmax, float* __restrict__ part)
 {
     using V = Mfma<float>::V;
     using L4 = Vec4<float>::V;
-    __shared__ __attribute__((aligned(16))) _Float16 s_a[2][kUbTile * kUsRow];
-    __shared__ __attribute__((aligned(16))) _Float16 s_b[2][kUbTile * kUsRow];
-    __shared__ float s_sc[2][kUbTile];  // column scale of each staged S row (2^-10+e, with y's 2^-6)
+    using Cfg = UbSplitCfg<TT>;
+    constexpr int NJ = Cfg::NJ;
+    __shared__ __attribute__((aligned(16))) _Float16 s_a[2][TT * kUsRow];
+    __shared__ __attribute__((aligned(16))) _Float16 s_b[2][TT * kUsRow];
+    __shared__ float s_sc[2][TT];  // column scale of each staged S row (2^-10+e, with y's 2^-6)
     // XCD-aware order: workgroups are dealt to the eight XCDs round-robin by
     // dispatch index, so consecutive logical indices (the pairs of one chunk,
     // which read the same sample range of a few row tiles) are given to one
@@ -337,14 +351,16 @@ __global__ __launch_bounds__(256, 2) void ubar_gemm_split_kernel(const float* __
     const int2 tp = pairs[pair];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int lr = lane & 15, lq = lane >> 4;
-    const int wr = wv & 1, wc = wv >> 1;
-    const bool idle = tp.x == tp.y && wr > wc;  // below the diagonal: stages operands only
+    const int wr = wv / Cfg::NWC, wc = wv % Cfg::NWC;
+    // on a diagonal tile pair, wavefronts whose rows lie below all their columns
+    // stage operands but run no products
+    const bool idle = tp.x == tp.y && wr * 64 > wc * Cfg::WC + Cfg::WC - 1;
     const int64_t k0 = (int64_t)kc * kchunk;
     const int64_t k1 = min(kmax, k0 + kchunk);
     // staging: thread -> tile row tid / 2, samples 16 (tid % 2) .. + 15 of a stage
     const int srow = tid >> 1, shalf = tid & 1;
-    const int la = min(tp.x * kUbTile + srow, nd - 1);  // rows past nd read row nd-1; discarded
-    const int lb = min(tp.y * kUbTile + srow, nd - 1);
+    const int la = min(tp.x * TT + srow, nd - 1);  // rows past nd read row nd-1; discarded
+    const int lb = min(tp.y * TT + srow, nd - 1);
     const float* __restrict__ ga = Y + (size_t)la * rs + 16 * shalf;
     const float* __restrict__ gb = S + (size_t)lb * rs + 16 * shalf;
     // granule positions of this thread's staged samples: high granules 2h, 2h+1, low 4+2h, 5+2h
@@ -353,11 +369,11 @@ __global__ __launch_bounds__(256, 2) void ubar_gemm_split_kernel(const float* __
     const int plA = ((4 + 2 * shalf) ^ sw) * 8, plB = ((5 + 2 * shalf) ^ sw) * 8;
     // operand reads: row 16 i + lr (mod 8 = lr mod 8), granules lq / 4 + lq
     const int rh = (lq ^ (lr & 7)) * 8, rl = ((4 + lq) ^ (lr & 7)) * 8;
-    V tot[4][4];
+    V tot[4][NJ];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) tot[i][j] = V{0, 0, 0, 0};
+        for (int j = 0; j < NJ; ++j) tot[i][j] = V{0, 0, 0, 0};
     const int nst = (int)((k1 - k0) / kUbK);
     L4 va[4], vb[4];
     auto fetch = [&](int64_t kn) {
@@ -412,58 +428,57 @@ __global__ __launch_bounds__(256, 2) void ubar_gemm_split_kernel(const float* __
         if (st + 1 < nst) fetch(k0 + (int64_t)(st + 1) * kUbK);  // uniform
         if (!idle) {
             const _Float16* ra = &s_a[buf][(wr * 64 + lr) * kUsRow];
-            const _Float16* rb = &s_b[buf][(wc * 64 + lr) * kUsRow];
-            half8 ah[4], al[4], bh[4], bl[4];
+            const _Float16* rb = &s_b[buf][(wc * Cfg::WC + lr) * kUsRow];
+            half8 ah[4], al[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 ah[q] = *(const half8*)(ra + 16 * q * kUsRow + rh);
                 al[q] = *(const half8*)(ra + 16 * q * kUsRow + rl);
-                bh[q] = *(const half8*)(rb + 16 * q * kUsRow + rh);
-                bl[q] = *(const half8*)(rb + 16 * q * kUsRow + rl);
             }
-            float sc[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) sc[j] = s_sc[buf][wc * 64 + 16 * j + lr];
+            for (int j = 0; j < NJ; ++j) {
+                const half8 bh = *(const half8*)(rb + 16 * j * kUsRow + rh);
+                const half8 bl = *(const half8*)(rb + 16 * j * kUsRow + rl);
+                const float sc = s_sc[buf][wc * Cfg::WC + 16 * j + lr];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+                for (int i = 0; i < 4; ++i) {
+                    V acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh, V{0, 0, 0, 0}, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh, acc, 0, 0, 0);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    V acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[j], V{0, 0, 0, 0}, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc, 0, 0, 0);
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) tot[i][j][r] = fmaf(acc[r], sc[j], tot[i][j][r]);
+                    for (int r = 0; r < 4; ++r) tot[i][j][r] = fmaf(acc[r], sc, tot[i][j][r]);
                 }
+            }
         }
         if (st + 1 < nst) stage(buf ^ 1);
         __syncthreads();
     }
     if (idle) return;
-    float* __restrict__ out = part + ((size_t)kc * n_pairs + pair) * (kUbTile * kUbTile);
+    float* __restrict__ out = part + ((size_t)kc * n_pairs + pair) * (TT * TT);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < NJ; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-                out[(size_t)(wr * 64 + 16 * i + Mfma<float>::row(lq, r)) * kUbTile + wc * 64 + 16 * j + lr] =
+                out[(size_t)(wr * 64 + 16 * i + Mfma<float>::row(lq, r)) * TT + wc * Cfg::WC + 16 * j + lr] =
                     tot[i][j][r];
 }
 
 // Ub[l, g + 1] += sum over chunks of the partial tiles (l <= g < nd), FP64
-template <typename T>
+template <typename T, int TT>
 __global__ __launch_bounds__(256) void ubar_reduce_kernel(const T* __restrict__ part, const int2* __restrict__ pairs,
                                                           int n_pairs, int n_chunks, int nd, double* __restrict__ Ub,
                                                           int n)
 {
     const int pair = blockIdx.x;
     const int2 tp = pairs[pair];
-    for (int e = blockIdx.y * 256 + threadIdx.x; e < kUbTile * kUbTile; e += gridDim.y * 256) {
-        const int i = e / kUbTile, j = e - i * kUbTile;
-        const int l = tp.x * kUbTile + i, g = tp.y * kUbTile + j;
+    for (int e = blockIdx.y * 256 + threadIdx.x; e < TT * TT; e += gridDim.y * 256) {
+        const int i = e / TT, j = e - i * TT;
+        const int l = tp.x * TT + i, g = tp.y * TT + j;
         if (l >= nd || g >= nd || l > g) continue;
         double sum = 0.0;
-        for (int kc = 0; kc < n_chunks; ++kc) sum += (double)part[((size_t)kc * n_pairs + pair) * (kUbTile * kUbTile) + e];
+        for (int kc = 0; kc < n_chunks; ++kc) sum += (double)part[((size_t)kc * n_pairs + pair) * (TT * TT) + e];
         Ub[(size_t)l + (size_t)(g + 1) * n] += sum;
     }
 }
@@ -476,23 +491,27 @@ hipError_t ubar_product(const void* Y, const void* S, int64_t rs, int nd, bool m
     dim3 g2(n_pairs, 16);
     if (mixed) {
         if (kchunk % kUbK != 0 || rs % kUbK != 0) return hipErrorInvalidValue;
-        if (split16)
-            hipLaunchKernelGGL(ubar_gemm_split_kernel, g1, b, 0, s, (const float*)Y, (const float*)S, rs, nd, pairs,
-                               n_pairs, kchunk, rs, (float*)part);
-        else
+        if (split16) {
+            hipLaunchKernelGGL(ubar_gemm_split_kernel<kUbTile16>, g1, dim3(UbSplitCfg<kUbTile16>::NT), 0, s,
+                               (const float*)Y, (const float*)S, rs, nd, pairs, n_pairs, kchunk, rs, (float*)part);
+            hipLaunchKernelGGL((ubar_reduce_kernel<float, kUbTile16>), g2, b, 0, s, (const float*)part, pairs,
+                               n_pairs, n_chunks, nd, Ub, n);
+        } else {
             hipLaunchKernelGGL(ubar_gemm_lds_kernel, g1, b, 0, s, (const float*)Y, (const float*)S, rs, nd, pairs,
                                n_pairs, kchunk, rs, (float*)part);
-        hipLaunchKernelGGL(ubar_reduce_kernel<float>, g2, b, 0, s, (const float*)part, pairs, n_pairs, n_chunks,
-                           nd, Ub, n);
+            hipLaunchKernelGGL((ubar_reduce_kernel<float, kUbTile>), g2, b, 0, s, (const float*)part, pairs,
+                               n_pairs, n_chunks, nd, Ub, n);
+        }
     } else {
         hipLaunchKernelGGL(ubar_gemm_kernel<double>, g1, b, 0, s, (const double*)Y, (const double*)S, rs, nd,
                            pairs, n_pairs, kchunk, rs, (double*)part);
-        hipLaunchKernelGGL(ubar_reduce_kernel<double>, g2, b, 0, s, (const double*)part, pairs, n_pairs, n_chunks,
-                           nd, Ub, n);
+        hipLaunchKernelGGL((ubar_reduce_kernel<double, kUbTile>), g2, b, 0, s, (const double*)part, pairs, n_pairs,
+                           n_chunks, nd, Ub, n);
     }
     return hipGetLastError();
 }
-int ubar_tile() { return kUbTile; }
+int ubar_tile(bool split16) { return split16 ? kUbTile16 : kUbTile; }
+int ubar_wgs_per_cu(bool split16) { return split16 ? UbSplitCfg<kUbTile16>::OCC : kUbOcc; }
 
 // diagonal of Ubar from the reverse sweep's accumulator
 __global__ void set_diag_kernel(double* Ub, int n, const double* __restrict__ d)

@@ -545,18 +545,22 @@ int grad_alloc(dbg_plan* P)
     if (P->split)
         HIPCHK(dalloc(&G.uc16f, (size_t)split16_cols(n) * (size_t)split16_col_stride(n) * sizeof(uint16_t)));
     {
-        // split-K layout of Ubar = Y^T Sbar: upper-triangle pairs of 128-row tiles,
-        // about twelve workgroups per CU in whole rounds of the two resident
-        // per CU (pairs x chunks a multiple of 2 ncu: no short last round)
-        const int tb = ubar_tile();
+        // split-K layout of Ubar = Y^T Sbar: upper-triangle pairs of row tiles
+        // (128 rows, two resident workgroups per CU; 256 rows and one for the
+        // split form), chunks in whole rounds of the resident slots (pairs x
+        // chunks a multiple of the slots: no short last round)
+        const int tb = ubar_tile(P->split);
         const int nt = (nd + tb - 1) / tb;
         std::vector<int2> pr;
         for (int ti = 0; ti < nt; ++ti)
             for (int tj = ti; tj < nt; ++tj) pr.push_back(make_int2(ti, tj));
         G.n_pairs = (int)pr.size();
-        const int slots = 2 * ncu;
+        const int slots = ubar_wgs_per_cu(P->split) * ncu;
         const int step = slots / std::gcd(G.n_pairs, slots);
-        int want = std::max(1, (12 * ncu + G.n_pairs - 1) / G.n_pairs);
+        // about six workgroups per slot (three for the split form, whose
+        // 256-wide partial tiles make the FP64 chunk reduction the larger cost)
+        const int per_slot = P->split ? 3 : 6;
+        int want = std::max(1, (per_slot * slots + G.n_pairs - 1) / G.n_pairs);
         want = (want + step - 1) / step * step;
         G.kchunk = std::max<int64_t>(32, ((G.rs + want - 1) / want + 31) / 32 * 32);
         G.n_chunks = (int)((G.rs + G.kchunk - 1) / G.kchunk);

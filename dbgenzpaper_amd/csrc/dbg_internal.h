@@ -97,7 +97,8 @@ hipError_t grad_norm(const double* pm, const int* pe, int64_t n_tiles, double to
 hipError_t ubar_product(const void* Y, const void* S, int64_t rs, int nd, bool mixed, const int2* pairs,
                         int n_pairs, int n_chunks, int64_t kchunk, void* part, double* Ub, int n, hipStream_t s,
                         bool split16);
-int ubar_tile();
+int ubar_tile(bool split16);        // output tile of the Ubar product
+int ubar_wgs_per_cu(bool split16);  // its resident workgroups per CU
 // step 0: Ubar diagonal; 1: Phi(X); 2: C_bar -> G, ga, gb
 hipError_t sigma_chain(double* Ub, int n, const double* gd, double* X, hipStream_t s, int step,
                        const double* Cb, const int32_t* perm, const double* scale, const double* Sig,
