@@ -193,3 +193,21 @@ def test_split16_contraction(dbg, oracle, d, family, gen):
         assert abs(ls - lo) < 1e-4 and abs(lf - lo) < 1e-4, (ls, lf, lo)
     else:
         assert abs(ls - lo) < 2 * abs(lf - lo) + 1e-5, (ls, lf, lo)
+
+
+@pytest.mark.parametrize("d,m", [(96, 700), (130, 300), (257, 1000), (400, 513)])
+def test_split16_edge_shapes(dbg, oracle, d, m):
+    """Split-FP16 kernel at the edges of its blocking: d = 96 (smallest matrix-
+    core shape, one panel, no contraction), a second panel of one dimension
+    (d = 130), the first panel with eight factor chunks (d = 257), and point
+    counts that leave part of the last 256-sample tile empty."""
+    P = W.dense_random(d, m=m, n_reps=2, seed=d + 1)
+    print(f"split16 edge d={d} m={m}", flush=True)
+    opts = dbg.QMC_opts(np.float32, m=P.m, n_reps=P.n_reps, max_pts=P.m, max_abs_err=0.0)
+    vals = {}
+    for c in ("split16", "fp32"):
+        D = dbg.QMCData(P.Sigma, P.a, P.b, opts, None, "Richtmyer", shifts=P.shifts, contraction=c)
+        vals[c], _, _ = dbg.qmc_pnorm(D)
+    r = oracle_same_inputs(oracle, _plan(dbg, P, phi_form="tail"))
+    print(f"  split16 {vals['split16']:.12g} fp32 {vals['fp32']:.12g} oracle {r.value:.12g}", flush=True)
+    assert rel(vals["split16"], r.value) < 1e-4 and rel(vals["fp32"], r.value) < 1e-4

@@ -60,14 +60,17 @@ def test_grad_fp64_matches_oracle(dbg, oracle, n, m, R, gen):
     assert np.allclose(g.Sigma, g.Sigma.T, rtol=0, atol=1e-12 * np.abs(g.Sigma).max())
 
 
-@pytest.mark.parametrize("n", [64, 200])
-def test_grad_mixed_close_to_oracle(dbg, oracle, n):
-    """FP32 forward and sweep (FP64 sums): within 2e-3 of the FP64 oracle in norm."""
-    print(f"grad mixed n={n}", flush=True)
-    P = W.factor_model(n, 4, m=1024, n_reps=2, seed=n, u_range=(0.05, 0.3))
-    D = _plan(dbg, P.Sigma, P.a, P.b, np.float32, 1024, 2)
+@pytest.mark.parametrize("n,gen", [(64, "Richtmyer"), (200, "Richtmyer"), (300, "Sobol")])
+def test_grad_mixed_close_to_oracle(dbg, oracle, n, gen):
+    """Mixed forward and sweep (FP64 sums; from d = 96 the split-FP16 matrix-core
+    kernel in record and reverse modes and the split Ubar product): within
+    2e-3 of the FP64 oracle in norm."""
+    print(f"grad mixed n={n} {gen}", flush=True)
+    P = W.factor_model(n, 4, m=1024, n_reps=2, seed=n, u_range=(0.05, 0.3), gen=gen)
+    D = _plan(dbg, P.Sigma, P.a, P.b, np.float32, 1024, 2, gen)
+    D.set_shifts(P.shifts)
     g = D.grad()
-    ro = _oracle_grad(oracle, D, "Richtmyer")
+    ro = _oracle_grad(oracle, D, gen)
     assert abs(g.log_value - ro.log_value) < 1e-4 * abs(ro.log_value)
     assert _relnorm(g.b_std, ro.gb) < 2e-3
     assert _relnorm(np.triu(g.U), np.triu(ro.gU)) < 2e-3

@@ -155,3 +155,32 @@ def test_fp32_tail_quantile_against_mpmath(tmp_path, rcp_ulp):
     # p = 1/2 exactly and the central/tail switch at |p - 1/2| = 0.425
     assert lib.tq(0.5, 0) == 0.0
     assert worst < 4 * np.finfo(np.float32).eps, worst
+
+
+def test_plan_option_validation(dbg):
+    """dbg_plan_create rejects invalid options before touching the device:
+    the contraction option (DBG_CONTRACT_*) is checked against the path."""
+    from dbgenzpaper_amd import _lib
+
+    L = _lib.lib()
+    n = 8
+    S = np.asfortranarray(np.eye(n))
+    a, b = -np.ones(n), np.ones(n)
+    h = ctypes.c_void_p()
+
+    def create(prec, **fields):
+        o = _lib.Opts()
+        L.dbg_opts_default(ctypes.byref(o), prec)
+        for k, v in fields.items():
+            setattr(o, k, v)
+        return L.dbg_plan_create(ctypes.byref(h), ctypes.byref(o), n, _lib.ptr(S, ctypes.c_double), n,
+                                 _lib.ptr(a, ctypes.c_double), _lib.ptr(b, ctypes.c_double))
+
+    assert create(_lib.PREC_MIXED, contraction=7) == _lib.DBG_ERR_ARG
+    assert b"unknown contraction" in L.dbg_last_error()
+    assert create(_lib.PREC_FP64, contraction=_lib.CONTRACT_SPLIT16) == _lib.DBG_ERR_ARG
+    assert b"split-FP16" in L.dbg_last_error()
+    assert create(_lib.PREC_MIXED, contraction=_lib.CONTRACT_SPLIT16, kind=_lib.KIND_SPARSE) == _lib.DBG_ERR_ARG
+    assert create(_lib.PREC_MIXED, n_bits=0) == _lib.DBG_ERR_ARG
+    assert create(_lib.PREC_MIXED, m=0) == _lib.DBG_ERR_ARG
+    assert not h.value
