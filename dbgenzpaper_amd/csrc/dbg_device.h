@@ -21,7 +21,7 @@ constexpr double kEps64 = 2.220446049250313e-16;    // eps(Float64)
 // loop, their polynomial constants are hoisted out of the loop into vector
 // registers (two per FP64 constant), which exhausts the register file; as
 // calls they keep the constants in scalar instructions local to each call.
-// The FP32 ones of the mixed walk (erfcf, the tail quantile) are inlined: their
+// The FP32 ones of the mixed walk (tail mass, tail quantile) are inlined: their
 // constants are single-register literals, and inlined the walk's scheduler can
 // overlap their latency with the rest of the step (measured 7 % faster at
 // d = 2500).
@@ -30,25 +30,22 @@ __device__ __attribute__((noinline)) double dbg_erfinv(double x);
 __device__ __attribute__((noinline)) double dbg_erf(double x);
 __device__ __attribute__((noinline)) double dbg_erfc(double x);
 __device__ __attribute__((noinline)) double dbg_erfcinv(double x);
-__device__ __attribute__((noinline)) float dbg_erfcinvf(float x);
 
 __device__ __attribute__((noinline)) double dbg_erfinv(double x) { return erfinv(x); }
 __device__ __attribute__((noinline)) double dbg_erf(double x) { return erf(x); }
 __device__ __attribute__((noinline)) double dbg_erfc(double x) { return erfc(x); }
 __device__ __attribute__((noinline)) double dbg_erfcinv(double x) { return erfcinv(x); }
-__device__ __forceinline__ float dbg_erfcf(float x) { return erfcf(x); }
-__device__ __attribute__((noinline)) float dbg_erfcinvf(float x) { return erfcinvf(x); }
 __device__ __forceinline__ float dbg_tail_quantile(float t, int k) { return tail_quantile_f32(t, k); }
 // the same two behind call boundaries, for the sparse walk (a short loop body
 // that is measured faster with calls)
-__device__ __attribute__((noinline)) float dbg_erfcf_call(float x) { return erfcf(x); }
+__device__ __attribute__((noinline)) float dbg_tail_mass_call(float z, int& k) { return tail_mass_f32(z, k); }
 __device__ __attribute__((noinline)) float dbg_tail_quantile_call(float t, int k) { return tail_quantile_f32(t, k); }
 
 template <bool CALLS>
-__device__ __forceinline__ float erfcf_sel(float x)
+__device__ __forceinline__ float tail_mass_sel(float z, int& k)
 {
-    if constexpr (CALLS) return dbg_erfcf_call(x);
-    else return dbg_erfcf(x);
+    if constexpr (CALLS) return dbg_tail_mass_call(z, k);
+    else return tail_mass_f32(z, k);
 }
 
 template <bool CALLS>
@@ -313,9 +310,10 @@ struct Weight {
 //     and the sample's tail mass t is formed from whichever tail it is nearer
 //     to (u_lo = ea + w dc or u_hi = eb + (1 - w) dc), y = -+sqrt2 erfcinv(2t).
 //     Nothing cancels in the tails and no interval is cut at 9.
-//   In FP32 the three masses carry a binary scale 2^-k: past |z| = 9 (where
-//   erfcf leaves the normal FP32 range) they are formed in FP64 and scaled
-//   into FP32; Phi^{-1} of masses below 1e-6 (or scaled) is taken in FP64.
+//   In FP32 the three masses carry a binary scale 2^-k, and both the masses
+//   (tail_mass_f32) and Phi^{-1} of a scaled mass (tail_quantile_f32) are
+//   computed in FP32 for any |z| (capped at 30) and any k: no FP64 and no
+//   data-dependent branch in the walk's dimension step.
 // ---------------------------------------------------------------------------
 struct StateF64 {
     double c, dc;
@@ -382,32 +380,25 @@ __device__ __forceinline__ void tail_from_z(StateTail<double>& st, double za, do
     tail_set64(ea, eb, dc, mode, st);
 }
 
+// FP32 masses without FP64 or a branch: each bound's mass erfc(|z|)/2 comes as
+// m 2^-k (tail_mass_f32); a one-sided interval takes the scale of its larger
+// mass (the bound nearer 0), so that ea, eb and dc are formed with the
+// smaller mass shifted into that scale; an interval that straddles 0 has k = 0.
 template <bool CALLS = false>
 __device__ __forceinline__ void tail_from_z(StateTail<float>& st, float za, float zb)
 {
-    const bool ia = isinf(za), ib = isinf(zb);
-    if ((ia || fabsf(za) <= 9.0f) && (ib || fabsf(zb) <= 9.0f)) {
-        st.k = 0;
-        st.ea = 0.0f;
-        st.eb = 0.0f;
-        if (!ia) st.ea = 0.5f * erfcf_sel<CALLS>(fabsf(za));
-        if (!ib) st.eb = 0.5f * erfcf_sel<CALLS>(fabsf(zb));
-        if (zb <= 0.0f) {
-            st.mode = 0;
-            st.dc = st.eb - st.ea;
-        } else if (za >= 0.0f) {
-            st.mode = 1;
-            st.dc = st.ea - st.eb;
-        } else {
-            st.mode = 2;
-            st.dc = (1.0f - st.ea) - st.eb;
-        }
-    } else {
-        double ea, eb, dc;
-        int mode;
-        tail_masses64((double)za, (double)zb, ea, eb, dc, mode);
-        tail_set64(ea, eb, dc, mode, st);
-    }
+    int ka, kb;
+    float ma = tail_mass_sel<CALLS>(fabsf(za), ka);
+    float mb = tail_mass_sel<CALLS>(fabsf(zb), kb);
+    if (isinf(za)) ma = 0.0f;
+    if (isinf(zb)) mb = 0.0f;
+    const int mode = (zb <= 0.0f) ? 0 : (za >= 0.0f) ? 1 : 2;
+    const int k = (mode == 0) ? kb : (mode == 1) ? ka : 0;
+    st.ea = ldexpf(ma, k - ka);
+    st.eb = ldexpf(mb, k - kb);
+    st.dc = (mode == 0) ? st.eb - st.ea : (mode == 1) ? st.ea - st.eb : (1.0f - st.ea) - st.eb;
+    st.mode = mode;
+    st.k = k;
 }
 
 __device__ __forceinline__ double tail_sample_w(const StateTail<double>& st, double w)
@@ -435,18 +426,7 @@ __device__ __forceinline__ float tail_sample_f(const StateTail<float>& st, float
 
 __device__ __forceinline__ float tail_sample_w(const StateTail<float>& st, double w)
 {
-    const float u_lo = fmaf((float)w, st.dc, st.ea);
-    const float u_hi = fmaf((float)(1.0 - w), st.dc, st.eb);
-    const bool upper = (st.mode == 1) || (st.mode == 2 && u_hi < u_lo);
-    const float t = upper ? u_hi : u_lo;
-    float y;
-    if (st.k != 0 || t < 1e-6f) {
-        const double tt = fmax(ldexp((double)t, -st.k), 1e-300);
-        y = (float)(-kSqrt2 * dbg_erfcinv(2.0 * tt));
-    } else {
-        y = -1.41421356f * dbg_erfcinvf(2.0f * t);
-    }
-    return upper ? -y : y;
+    return tail_sample_f<true>(st, (float)w, (float)(1.0 - w));
 }
 
 template <bool MIXED, int FORM>

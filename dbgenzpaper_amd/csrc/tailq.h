@@ -26,6 +26,19 @@
 
 namespace dbg {
 
+// 1/d: the device's approximate v_rcp_f32 (within 1 ulp), the host's division
+DBG_HD inline float rcp_f32(float d)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(d);
+#elif defined(DBG_RCP_ULP_TEST)
+    // host check of the refinements: the reciprocal off by one ulp
+    return nextafterf(1.0f / d, DBG_RCP_ULP_TEST > 0 ? INFINITY : 0.0f);
+#else
+    return 1.0f / d;
+#endif
+}
+
 // n / d for the positive, well-scaled denominators of the rational
 // approximations below: a reciprocal (the device's approximate v_rcp_f32,
 // within 1 ulp; the host's division) refined by one Newton step on the
@@ -34,14 +47,7 @@ namespace dbg {
 // IEEE division (no special cases: d is never 0, inf or subnormal here).
 DBG_HD inline float div_pos(float n, float d)
 {
-#if defined(__HIP_DEVICE_COMPILE__)
-    const float r = __builtin_amdgcn_rcpf(d);
-#elif defined(DBG_RCP_ULP_TEST)
-    // host check of the refinement: the reciprocal off by DBG_RCP_ULP_TEST ulps
-    const float r = nextafterf(1.0f / d, DBG_RCP_ULP_TEST > 0 ? INFINITY : 0.0f);
-#else
-    const float r = 1.0f / d;
-#endif
+    const float r = rcp_f32(d);
     const float q0 = n * r;
     return fmaf(fmaf(-d, q0, n), r, q0);
 }
@@ -54,6 +60,50 @@ DBG_HD inline float horner_f32(float x, const float (&c)[N])
 #pragma unroll
     for (int i = 1; i < N; ++i) v = fmaf(v, x, c[i]);
     return v;
+}
+
+// Upper normal tail mass erfc(z)/2 (= 1 - Phi(z sqrt2)) for z >= 0 in FP32,
+// returned as m 2^-k with m in [0.02, 0.71] and k >= 0, so that it stays
+// representable far past the FP32 range.  z is capped at 30 (a mass near
+// 2^-1300, beyond the FP64 range too); the caller handles infinite z.
+//
+// Method: erfc(z) = t exp(-z^2 + P(t)), t = 2 / (2 + z) (the form of
+// Numerical Recipes' erfcc, Press et al. 1992, sec. 6.2), with P a degree-11
+// polynomial in u = 2t - 1 = (2 - z) / (2 + z) fitted by least squares on 600
+// Chebyshev nodes of t in [0, 1] to 50-digit values of
+// log(erfc(z) / t) + z^2 (fit error 1.1e-8 over all z >= 0; the script that
+// made it is fit_tail_mass.py beside this file).  u is formed from z directly
+// (a refined quotient), not from a rounded t, since P is steep in u.  z^2 is
+// split exactly as p + e (e = fma(z, z, -p)) and the exponential reduced
+// Cody-Waite style: n = rint((P - p) / ln2),
+// r = ((-p + n ln2_hi) + P - e) + n ln2_lo (the first sum is exact), so that
+// erfc(z)/2 = 2^n exp(r) / (2 + z).  Relative error about 2.5e-7 with exact
+// reciprocal and exp2, 5e-7 with the device's 1-ulp v_rcp_f32 and v_exp_f32.
+DBG_HD inline float tail_mass_f32(float z, int& k)
+{
+    z = fminf(z, 30.0f);
+    const float s = 2.0f + z;
+    const float h = rcp_f32(s);
+    const float w = 2.0f - z;
+    const float q0 = w * h;
+    const float u = fmaf(fmaf(-s, q0, w), h, q0);
+    const float C[12] = {-8.728381363e-05f, 8.000738489e-06f, 5.737663596e-04f, -2.279109613e-04f,
+                         -2.288765274e-03f, 1.794066280e-03f, 8.808825165e-03f, -9.880019352e-03f,
+                         -4.689364135e-02f, 4.734388739e-02f, 6.726431251e-01f, -6.717941165e-01f};
+    const float P = horner_f32(u, C);
+    const float p = z * z;
+    const float e = fmaf(z, z, -p);
+    const float n = rintf((P - p) * 1.44269504088896341f);
+    float r = fmaf(-n, 0.693145751953125f, -p);  // ln2_hi: 16 bits, the sum is exact
+    r = r + P;
+    r = fmaf(-n, 1.428606765330187e-06f, r - e);  // ln2_lo
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float ex = __builtin_amdgcn_exp2f(r * 1.44269504088896341f);
+#else
+    const float ex = exp2f(r * 1.44269504088896341f);
+#endif
+    k = -(int)n;
+    return h * ex;
 }
 
 DBG_HD inline float tail_quantile_f32(float t, int k)

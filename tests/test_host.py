@@ -157,6 +157,34 @@ def test_fp32_tail_quantile_against_mpmath(tmp_path, rcp_ulp):
     assert worst < 4 * np.finfo(np.float32).eps, worst
 
 
+@pytest.mark.parametrize("rcp_ulp", [0, 1, -1])
+def test_fp32_tail_mass_against_mpmath(tmp_path, rcp_ulp):
+    """The mixed path's FP32 tail mass erfc(z)/2 = m 2^-k against 40-digit
+    erfc over z in [0, 30] (masses down to 2^-1300), with the reciprocal exact
+    and one ulp off either way (the device's v_rcp_f32).  Bound 5e-7 relative
+    (about 8 FP32 ulps); m stays in [0.02, 0.71] and z is capped at 30."""
+    mpmath = pytest.importorskip("mpmath")
+    pre = f"#define DBG_RCP_ULP_TEST {rcp_ulp}\n" if rcp_ulp else ""
+    lib = _host_build(tmp_path, pre + '#include "tailq.h"\n'
+                      'extern "C" float tm(float z, int* k) { return dbg::tail_mass_f32(z, *k); }\n', "tm")
+    lib.tm.restype = ctypes.c_float
+    lib.tm.argtypes = [ctypes.c_float, ctypes.POINTER(ctypes.c_int)]
+    mpmath.mp.dps = 40
+    k = ctypes.c_int()
+    worst, lo, hi = 0.0, 1.0, 0.0
+    zs = np.concatenate([np.linspace(0.0, 6.0, 1201), np.linspace(6.0, 30.0, 481), [1e-30, 2.0, 9.0, 26.5]])
+    for z in zs.astype(np.float32):
+        m = lib.tm(float(z), ctypes.byref(k))
+        assert k.value >= 0
+        lo, hi = min(lo, m), max(hi, m)
+        tr = mpmath.erfc(mpmath.mpf(float(z))) / 2
+        worst = max(worst, float(abs(mpmath.mpf(m) * mpmath.mpf(2) ** (-k.value) / tr - 1)))
+    assert worst < 5e-7, worst
+    assert 0.02 <= lo and hi <= 0.71, (lo, hi)
+    kc = ctypes.c_int()
+    assert lib.tm(45.0, ctypes.byref(k)) == lib.tm(30.0, ctypes.byref(kc)) and k.value == kc.value
+
+
 def test_plan_option_validation(dbg):
     """dbg_plan_create rejects invalid options before touching the device:
     the contraction option (DBG_CONTRACT_*) is checked against the path."""
