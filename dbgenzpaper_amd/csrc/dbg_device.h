@@ -692,8 +692,8 @@ template <typename T>
 __device__ __forceinline__ T scaled_pdf(T x, int k)
 {
     if (isinf(x)) return T(0);
-    if constexpr (std::is_same<T, float>::value)
-        return expf(fmaf(-0.5f * x, x, (float)k * 0.693147180559945309f)) * 0.398942280401432678f;
+    if constexpr (std::is_same<T, float>::value)  // 2^(k - x^2 log2(e) / 2) by v_exp_f32 (1 ulp)
+        return __builtin_amdgcn_exp2f(fmaf(-0.721347520444481703f * x, x, (float)k)) * 0.398942280401432678f;
     else
         return exp(-0.5 * x * x + k * 0.69314718055994530942) * 0.39894228040143267794;
 }

@@ -106,44 +106,37 @@ DBG_HD inline float tail_mass_f32(float z, int& k)
     return h * ex;
 }
 
+DBG_HD inline float log2_f32(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_logf(x);
+#else
+    return log2f(x);
+#endif
+}
+
 DBG_HD inline float tail_quantile_f32(float t, int k)
 {
-    const float p = ldexpf(t, -k);
-    const float q = p - 0.5f;
-    if (fabsf(q) <= 0.425f) {
-        const float r = fmaf(-q, q, 0.180625f);
-        const float A[8] = {2.5090809287301226727e+3f, 3.3430575583588128105e+4f, 6.7265770927008700853e+4f,
-                            4.5921953931549871457e+4f, 1.3731693765509461125e+4f, 1.9715909503065514427e+3f,
-                            1.3314166789178437745e+2f, 3.3871328727963666080e+0f};
-        const float B[8] = {5.2264952788528545610e+3f, 2.8729085735721942674e+4f, 3.9307895800092710610e+4f,
-                            2.1213794301586595867e+4f, 5.3941960214247511077e+3f, 6.8718700749205790830e+2f,
-                            4.2313330701600911252e+1f, 1.0f};
-        return q * div_pos(horner_f32(r, A), horner_f32(r, B));
+    const float p = ldexpf(t, -k);  // 0 when far below the FP32 range
+    // w = -log(4 p (1 - p)) = (k - log2(4 t (1 - p))) log 2
+    const float w = (float(k) - log2_f32(4.0f * t * (1.0f - p))) * 0.693147180559945309f;
+    if (w < 6.25f) {  // p > 4.8e-4
+        const float C1[11] = {9.308407933e-10f, -6.508231731e-09f, -3.488968403e-08f, 6.018015029e-07f,
+                              -1.963471277e-06f, -1.963790237e-05f, 2.641592582e-04f, -1.047509722e-03f,
+                              -8.532967418e-03f, 3.396349549e-01f, 2.338620663e+00f};
+        return fmaf(2.0f, p, -1.0f) * horner_f32(w - 3.125f, C1);
     }
-    // -log p = -log t + k log 2
-    const float s = fmaf((float)k, 0.693147180559945309f, -logf(t));
-    float r = sqrtf(s);
-    float val;
-    if (r <= 5.0f) {
-        const float C[8] = {7.74545014278341407640e-4f, 2.27238449892691845833e-2f, 2.41780725177450611770e-1f,
-                            1.27045825245236838258e+0f, 3.64784832476320460504e+0f, 5.76949722146069140550e+0f,
-                            4.63033784615654529590e+0f, 1.42343711074968357734e+0f};
-        const float D[8] = {1.05075007164441684324e-9f, 5.47593808499534494600e-4f, 1.51986665636164571966e-2f,
-                            1.48103976427480074590e-1f, 6.89767334985100004550e-1f, 1.67638483018380384940e+0f,
-                            2.05319162663775882187e+0f, 1.0f};
-        r -= 1.6f;
-        val = div_pos(horner_f32(r, C), horner_f32(r, D));
-    } else {
-        const float E[8] = {2.01033439929228813265e-7f, 2.71155556874348757815e-5f, 1.24266094738807843860e-3f,
-                            2.65321895265761230930e-2f, 2.96560571828504891230e-1f, 1.78482653991729133580e+0f,
-                            5.46378491116411436990e+0f, 6.65790464350110377720e+0f};
-        const float F[8] = {2.04426310338993978564e-15f, 1.42151175831644588870e-7f, 1.84631831751005468180e-5f,
-                            7.86869131145613259100e-4f, 1.48753612908506148525e-2f, 1.36929880922735805310e-1f,
-                            5.99832206555887937690e-1f, 1.0f};
-        r -= 5.0f;
-        val = div_pos(horner_f32(r, E), horner_f32(r, F));
+    const float s = sqrtf(w);
+    if (s < 6.0f) {  // p > 5.8e-17
+        const float C2[10] = {4.801153182e-07f, -1.164755830e-07f, -6.827907328e-06f, 2.369199137e-05f,
+                              -7.816802827e-05f, 2.896706865e-04f, -8.585309261e-04f, 1.008118619e-03f,
+                              1.428325772e+00f, 5.787277699e+00f};
+        return -horner_f32(s - 4.25f, C2);
     }
-    return -val;
+    const float C3[10] = {1.666337969e+05f, -1.617340527e+04f, -1.590158749e+01f, -9.795043945e+01f,
+                          3.148544884e+01f, -6.921745300e+00f, 2.482273579e+00f, -5.426022410e-01f,
+                          -2.322116643e-01f, 1.400314808e+00f};
+    return -s * horner_f32(rcp_f32(s) - 0.0933333333f, C3);
 }
 
 }  // namespace dbg
