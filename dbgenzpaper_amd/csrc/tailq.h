@@ -1,19 +1,14 @@
-// Lower-tail normal quantile in FP32 for a probability mass given as
-// p = t * 2^-k (t > 0 a normal FP32 number, k >= 0 an integer scale), the
-// form in which the mixed-precision integration carries masses that fall
-// below the FP32 range.  Returns Phi^{-1}(p) for p <= 1/2.
-//
-// Method: Wichura's algorithm AS 241 (Applied Statistics 37, 1988), the
-// PPND16 rational approximations rounded to FP32:
-//   |p - 1/2| <= 0.425:  q (A(r) / B(r)),  r = 0.180625 - q^2, q = p - 1/2;
-//   otherwise:           r = sqrt(-log p), C(r - 1.6) / D(r - 1.6) for r <= 5,
-//                        E(r - 5) / F(r - 5) beyond.
-// -log p is formed as -log t + k log 2 in FP32, so masses far below the FP32
-// range need no FP64 arithmetic.  The result is accurate to a few FP32 ulps
-// (the rationals themselves are good to about 1e-16).
+// FP32 normal tail functions of the mixed-precision integration, which
+// carries probability masses as p = t 2^-k (t a normal FP32 number, k >= 0 an
+// integer scale) so that masses far below the FP32 range need no FP64:
+//   tail_mass_f32      erfc(z)/2 for z >= 0, as m 2^-k;
+//   tail_quantile_f32  Phi^{-1}(p) for p = t 2^-k <= 1/2.
+// Both are polynomials fitted for this code (fit_tailq.py beside this file
+// reproduces every coefficient) and are branch-free or nearly so, since the
+// integration evaluates them once per sample and dimension in SIMD.
 //
 // The same source compiles for the host (plain C++) so that the CPU test
-// suite can check it against a high-precision quantile.
+// suite can check both against high-precision values.
 #pragma once
 
 #include <math.h>
@@ -72,7 +67,7 @@ DBG_HD inline float horner_f32(float x, const float (&c)[N])
 // polynomial in u = 2t - 1 = (2 - z) / (2 + z) fitted by least squares on 600
 // Chebyshev nodes of t in [0, 1] to 50-digit values of
 // log(erfc(z) / t) + z^2 (fit error 1.1e-8 over all z >= 0; the script that
-// made it is fit_tail_mass.py beside this file).  u is formed from z directly
+// made it is fit_tailq.py beside this file).  u is formed from z directly
 // (a refined quotient), not from a rounded t, since P is steep in u.  z^2 is
 // split exactly as p + e (e = fma(z, z, -p)) and the exponential reduced
 // Cody-Waite style: n = rint((P - p) / ln2),
@@ -115,6 +110,14 @@ DBG_HD inline float log2_f32(float x)
 #endif
 }
 
+// Phi^{-1}(p), p = t 2^-k <= 1/2, in terms of w = -log(4p(1 - p)) (the
+// variable of Giles, "Approximating the erfinv function", GPU Computing Gems
+// 2010; coefficients fitted here): y / (2p - 1) is a polynomial in w for
+// w < 6.25 (p > 4.8e-4, the common case, so a wavefront of samples seldom
+// takes two branches); -y a polynomial in s = sqrt(w) for s < 6 (p > 5.8e-17);
+// -y / s a polynomial in 1/s beyond (fitted to s = 50, p near 2^-3600).
+// w is formed as (k - log2(4t(1 - p))) log 2, so no step leaves the FP32
+// range.  Within 1.5 FP32 eps of max(|y|, 1) on the host.
 DBG_HD inline float tail_quantile_f32(float t, int k)
 {
     const float p = ldexpf(t, -k);  // 0 when far below the FP32 range

@@ -119,12 +119,12 @@ def _host_build(tmp_path, src, name):
 
 @pytest.mark.parametrize("rcp_ulp", [0, 1, -1])
 def test_fp32_tail_quantile_against_mpmath(tmp_path, rcp_ulp):
-    """The mixed path's FP32 inverse CDF (AS 241 in FP32, -log p formed as
-    -log t + k log 2) against a 40-digit quantile, from p = 0.5 down to
-    p = 2^-3000 given as scaled masses t 2^-k.  Bound: 4 FP32 ulps of
-    max(|y|, 1).  The rational approximations divide by one refined
-    reciprocal; rcp_ulp = +-1 builds it from a reciprocal one ulp off, as the
-    device's approximate reciprocal may be."""
+    """The mixed path's FP32 inverse CDF (three fitted polynomials in
+    w = -log(4p(1-p)), w formed from t and k) against a 40-digit quantile,
+    from p = 0.5 down to p = 2^-3000 given as scaled masses t 2^-k.  Bound:
+    4 FP32 ulps of max(|y|, 1).  The deepest region takes 1/sqrt(w) from a
+    reciprocal; rcp_ulp = +-1 builds it one ulp off, as the device's
+    approximate reciprocal may be."""
     mpmath = pytest.importorskip("mpmath")
     pre = f"#define DBG_RCP_ULP_TEST {rcp_ulp}\n" if rcp_ulp else ""
     lib = _host_build(tmp_path, pre + '#include "tailq.h"\n'
