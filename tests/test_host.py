@@ -144,7 +144,10 @@ def test_fp32_tail_quantile_against_mpmath(tmp_path, rcp_ulp):
         return y
 
     worst = 0.0
-    log2p = np.concatenate([np.linspace(-1.0, -40.0, 157), np.linspace(-40.0, -3000.0, 97)])
+    # plus both sides of the region switches, w = 6.25 (p = 4.83e-4) and sqrt(w) = 6 (p = 5.80e-17)
+    edges = np.log2(np.concatenate([4.8303e-4 * (1 + np.linspace(-1e-3, 1e-3, 21)),
+                                    5.7988e-17 * (1 + np.linspace(-1e-3, 1e-3, 21))]))
+    log2p = np.concatenate([np.linspace(-1.0, -40.0, 157), np.linspace(-40.0, -3000.0, 97), edges])
     for lp in log2p:
         k = max(0, int(-lp) - 100) if lp < -110 else 0
         t = float(np.float32(2.0 ** (lp + k)))
@@ -152,7 +155,7 @@ def test_fp32_tail_quantile_against_mpmath(tmp_path, rcp_ulp):
         tr = truth(p)
         y = lib.tq(t, k)
         worst = max(worst, float(abs(y - tr) / max(abs(tr), 1)))
-    # p = 1/2 exactly and the central/tail switch at |p - 1/2| = 0.425
+    # p = 1/2 gives 0 exactly
     assert lib.tq(0.5, 0) == 0.0
     assert worst < 4 * np.finfo(np.float32).eps, worst
 
