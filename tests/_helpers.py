@@ -95,3 +95,16 @@ def logp_fixed_perm(O, Sigma, a, b, perm, *, m, n_reps, shifts):
     n = Sigma.shape[0]
     r = O.qmc_pnorm(U, ap, bp, m=m, n_reps=n_reps, shifts=shifts, phi_form="tail", bs_i=n - 1, bs_i2=n - 1)
     return r.log_value, U, ap, bp
+
+
+def normal_quantile_mp(p, mpmath):
+    """Phi^{-1}(p) for 0 < p <= 1/2 (an mpmath number) by Newton steps on
+    log Phi(y) = log p, to about 30 digits."""
+    y = mpmath.mpf(-0.5) if p > 0.1 else -mpmath.sqrt(-2 * mpmath.log(p))
+    for _ in range(200):
+        f = mpmath.log(mpmath.ncdf(y)) - mpmath.log(p)
+        yn = y - f * mpmath.ncdf(y) / mpmath.npdf(y)
+        if abs(yn - y) < mpmath.mpf(10) ** -30:
+            return yn
+        y = yn
+    return y
