@@ -134,9 +134,9 @@ int check_block_sizes(int bs1, int bs2, int bs3, int bs4)
 }
 
 // Copy the inputs (host or device memory, hipMemcpyDefault) and factor.
-// pivot: cholesky_genz!, else cholesky_classic!.
+// pivot: 0 cholesky_classic!, 1 cholesky_genz!, 2 cholesky_rowmax!.
 int factor_on_device(CholBuffers& B, int n, const double* M_in, int64_t lda, const double* a_in,
-                     const double* b_in, int bs1, int bs2, int bs3, int bs4, bool pivot,
+                     const double* b_in, int bs1, int bs2, int bs3, int bs4, int pivot,
                      hipStream_t s, int32_t* rank_out)
 {
     int rc = check_block_sizes(bs1, bs2, bs3, bs4);
@@ -168,7 +168,7 @@ int factor_on_device(CholBuffers& B, int n, const double* M_in, int64_t lda, con
 }
 
 int chol_host(double* M, int64_t lda, int32_t n, double* a, double* b, int32_t bs1, int32_t bs2,
-              int32_t bs3, int32_t bs4, int32_t* perm, double* scale, int32_t* rank, bool pivot)
+              int32_t bs3, int32_t bs4, int32_t* perm, double* scale, int32_t* rank, int pivot)
 {
     if (n < 1 || lda < n || !M || !a || !b) return fail(DBG_ERR_ARG, "invalid matrix arguments");
     hipStream_t s;
@@ -304,7 +304,7 @@ int plan_factor(dbg_plan* P)
     hipStream_t s = P->stream;
     const bool pivot = (o.kind == DBG_KIND_DENSE);
     int rc = factor_on_device(P->F, n, P->sig, n, P->a0, P->b0, o.chol_block_size, o.chol_block_size2,
-                              o.chol_block_size3, o.chol_block_size4, pivot, s, &P->rank);
+                              o.chol_block_size3, o.chol_block_size4, pivot ? 1 : 0, s, &P->rank);
     if (rc) return rc;
     if (pivot && P->rank != n)
         return fail(DBG_ERR_RANK, "QMCData requires a full-rank Cholesky factorization.");
@@ -722,14 +722,22 @@ int dbg_chol_genz_f64(double* M, int64_t lda, int32_t n, double* a, double* b, i
                       int32_t* rank)
 {
     g_err.clear();
-    return chol_host(M, lda, n, a, b, bs1, bs2, bs3, bs4, perm, scale, rank, true);
+    return chol_host(M, lda, n, a, b, bs1, bs2, bs3, bs4, perm, scale, rank, 1);
+}
+
+int dbg_chol_rowmax_f64(double* M, int64_t lda, int32_t n, double* a, double* b, int32_t bs1,
+                        int32_t bs2, int32_t bs3, int32_t bs4, int32_t* perm, double* scale,
+                        int32_t* rank)
+{
+    g_err.clear();
+    return chol_host(M, lda, n, a, b, bs1, bs2, bs3, bs4, perm, scale, rank, 2);
 }
 
 int dbg_chol_classic_f64(double* M, int64_t lda, int32_t n, double* a, double* b, int32_t bs1,
                          int32_t bs2, int32_t bs3, int32_t bs4, double* scale)
 {
     g_err.clear();
-    return chol_host(M, lda, n, a, b, bs1, bs2, bs3, bs4, nullptr, scale, nullptr, false);
+    return chol_host(M, lda, n, a, b, bs1, bs2, bs3, bs4, nullptr, scale, nullptr, 0);
 }
 
 int dbg_plan_create(dbg_plan** plan, const dbg_opts* opts, int32_t n, const double* Sigma,

@@ -128,6 +128,43 @@ __device__ __forceinline__ bool better(double v, int i, double bv, int bi)
     return i < bi;
 }
 
+// Pivot modes of the window kernels
+constexpr int kPivotNone = 0;    // cholesky_classic!
+constexpr int kPivotGenz = 1;    // cholesky_genz!: smallest interval mass
+constexpr int kPivotRowMax = 2;  // cholesky_rowmax!: largest Schur diagonal
+
+// argmax key of cholesky_rowmax! (src/cholesky.jl:319-327: a scan from i with
+// a strict >, so the first maximum wins and NaN is never taken after i): a
+// NaN loses to every number, then larger value, then smaller index.  A NaN at
+// i itself keeps i; the caller checks that case.
+__device__ __forceinline__ bool better_max(double v, int i, double bv, int bi)
+{
+    const bool vn = isnan(v), bn = isnan(bv);
+    if (vn != bn) return bn;
+    if (!vn && v != bv) return v > bv;
+    return i < bi;
+}
+
+// pivot selection shared by both window kernels: reduce (bv, bi) over a
+// wavefront by the mode's key
+__device__ __forceinline__ void pivot_wave_reduce(int pivot, double& bv, int& bi)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(bv, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        const bool take = pivot == kPivotRowMax ? better_max(ov, oi, bv, bi) : better(ov, oi, bv, bi);
+        if (take) {
+            bv = ov;
+            bi = oi;
+        }
+    }
+}
+
+__device__ __forceinline__ bool pivot_better(int pivot, double v, int i, double bv, int bi)
+{
+    return pivot == kPivotRowMax ? better_max(v, i, bv, bi) : better(v, i, bv, bi);
+}
+
 // Factor the pivots [ib_begin, ib_end) of the window [w0, w1) (whole inner
 // blocks of nb) with one workgroup of NT threads (window size at most WMAX).
 // With inkernel_update the deferred update of the rest of the window after
@@ -173,26 +210,24 @@ __device__ __forceinline__ void chol_window_body(const CholArgs& C, int w0, int 
             int bi = INT32_MAX;
             double best_order = 0.0;
             if (pivot) {
-                bv = INFINITY;
+                bv = pivot == kPivotRowMax ? -INFINITY : INFINITY;
                 for (int k = i + tid; k < w1; k += kCholThreads) {
-                    double m_kk = MM(k, k) - s_dsum[k - w0];
-                    m_kk = m_kk < 0 ? 0.0 : sqrt(m_kk);
-                    const double a_k = (s_a[k - w0] - s_sv[k - w0]) / m_kk * c2;
-                    const double b_k = (s_b[k - w0] - s_sv[k - w0]) / m_kk * c2;
-                    const double o = erf(b_k) - erf(a_k);
-                    if (better(o, k, bv, bi)) {
+                    double o;
+                    if (pivot == kPivotRowMax) {
+                        o = MM(k, k) - s_dsum[k - w0];
+                    } else {
+                        double m_kk = MM(k, k) - s_dsum[k - w0];
+                        m_kk = m_kk < 0 ? 0.0 : sqrt(m_kk);
+                        const double a_k = (s_a[k - w0] - s_sv[k - w0]) / m_kk * c2;
+                        const double b_k = (s_b[k - w0] - s_sv[k - w0]) / m_kk * c2;
+                        o = erf(b_k) - erf(a_k);
+                    }
+                    if (pivot_better(pivot, o, k, bv, bi)) {
                         bv = o;
                         bi = k;
                     }
                 }
-                for (int off = 32; off > 0; off >>= 1) {
-                    const double ov = __shfl_xor(bv, off, 64);
-                    const int oi = __shfl_xor(bi, off, 64);
-                    if (better(ov, oi, bv, bi)) {
-                        bv = ov;
-                        bi = oi;
-                    }
-                }
+                pivot_wave_reduce(pivot, bv, bi);
                 if ((tid & 63) == 0) {
                     s_rv[tid >> 6] = bv;
                     s_ri[tid >> 6] = bi;
@@ -205,25 +240,33 @@ __device__ __forceinline__ void chol_window_body(const CholArgs& C, int w0, int 
                     bv = s_rv[0];
                     bi = s_ri[0];
                     for (int w = 1; w < kCholThreads / 64; ++w)
-                        if (better(s_rv[w], s_ri[w], bv, bi)) {
+                        if (pivot_better(pivot, s_rv[w], s_ri[w], bv, bi)) {
                             bv = s_rv[w];
                             bi = s_ri[w];
                         }
                     im = bi;
                     best_order = bv;
+                    // rowmax: a NaN at i keeps i (nothing compares greater)
+                    if (pivot == kPivotRowMax && (bi == INT32_MAX || isnan(MM(i, i) - s_dsum[i - w0]))) im = i;
                 }
                 double m_ii = MM(im, im) - s_dsum[im - w0];
+                // rank: genz stops at m_ii <= eps0 after the swap (src/cholesky.jl:518-541),
+                // rowmax at m_ii <= 0 before it (:329-350)
+                const bool stop = (pivot == kPivotGenz && !(m_ii > eps0)) || (pivot == kPivotRowMax && !(m_ii > 0.0));
+                if (stop && pivot == kPivotRowMax) im = i;
                 if (im != i) {
                     const int a0 = i - w0, a1 = im - w0;
                     int tp = s_perm[a0]; s_perm[a0] = s_perm[a1]; s_perm[a1] = tp;
                     double t;
                     t = s_dsum[a0]; s_dsum[a0] = s_dsum[a1]; s_dsum[a1] = t;
                     t = s_sv[a0]; s_sv[a0] = s_sv[a1]; s_sv[a1] = t;
-                    t = s_a[a0]; s_a[a0] = s_a[a1]; s_a[a1] = t;
-                    t = s_b[a0]; s_b[a0] = s_b[a1]; s_b[a1] = t;
+                    if (pivot != kPivotRowMax) {  // rowmax leaves a and b in place
+                        t = s_a[a0]; s_a[a0] = s_a[a1]; s_a[a1] = t;
+                        t = s_b[a0]; s_b[a0] = s_b[a1]; s_b[a1] = t;
+                    }
                 }
                 s_im = im;
-                if (pivot && !(m_ii > eps0)) {
+                if (stop) {
                     s_stop = 1;
                     *C.rank = i;
                 } else {
@@ -231,8 +274,9 @@ __device__ __forceinline__ void chol_window_body(const CholArgs& C, int w0, int 
                     s_uii = uii;
                     const double a_norm = (s_a[i - w0] - s_sv[i - w0]) / uii;
                     const double b_norm = (s_b[i - w0] - s_sv[i - w0]) / uii;
-                    s_yi = pivot ? (exp(-(a_norm * a_norm) / 2) - exp(-(b_norm * b_norm) / 2)) * is2pi / best_order
-                                 : 0.0;
+                    s_yi = pivot == kPivotGenz
+                               ? (exp(-(a_norm * a_norm) / 2) - exp(-(b_norm * b_norm) / 2)) * is2pi / best_order
+                               : 0.0;
                 }
             }
             __syncthreads();
@@ -403,10 +447,11 @@ struct RegShared {
 
 // One pivot i = ib0 + 16 ng + E of the inner block; returns false when the
 // block ends (rank deficiency).  All branches are uniform.
-template <int E>
+template <int PIVOT, int E>
 __device__ __forceinline__ bool reg_pivot(const CholArgs& C, RegShared& S, ColRegs& R, int ng, int ib0, int w1,
-                                          int pivot, double eps0)
+                                          double eps0)
 {
+    constexpr int pivot = PIVOT;
     double* M = C.M;
     const int64_t lda = C.lda;
     const int n = C.n;
@@ -422,26 +467,24 @@ __device__ __forceinline__ bool reg_pivot(const CholArgs& C, RegShared& S, ColRe
     int bi = INT32_MAX;
     double best_order = 0.0;
     if (pivot) {
-        bv = INFINITY;
+        bv = pivot == kPivotRowMax ? -INFINITY : INFINITY;
         if (own && c >= i) {
-            double m_kk = S.dg[tid] - S.dsum[tid];
-            m_kk = m_kk < 0 ? 0.0 : sqrt(m_kk);
-            const double a_k = (S.a[tid] - S.sv[tid]) / m_kk * c2;
-            const double b_k = (S.b[tid] - S.sv[tid]) / m_kk * c2;
-            const double o = erf(b_k) - erf(a_k);
-            if (better(o, c, bv, bi)) {
+            double o;
+            if (pivot == kPivotRowMax) {
+                o = S.dg[tid] - S.dsum[tid];
+            } else {
+                double m_kk = S.dg[tid] - S.dsum[tid];
+                m_kk = m_kk < 0 ? 0.0 : sqrt(m_kk);
+                const double a_k = (S.a[tid] - S.sv[tid]) / m_kk * c2;
+                const double b_k = (S.b[tid] - S.sv[tid]) / m_kk * c2;
+                o = erf(b_k) - erf(a_k);
+            }
+            if (pivot_better(pivot, o, c, bv, bi)) {
                 bv = o;
                 bi = c;
             }
         }
-        for (int off = 32; off > 0; off >>= 1) {
-            const double ov = __shfl_xor(bv, off, 64);
-            const int oi = __shfl_xor(bi, off, 64);
-            if (better(ov, oi, bv, bi)) {
-                bv = ov;
-                bi = oi;
-            }
-        }
+        pivot_wave_reduce(pivot, bv, bi);
         if ((tid & 63) == 0) {
             S.rv[tid >> 6] = bv;
             S.ri[tid >> 6] = bi;
@@ -454,26 +497,35 @@ __device__ __forceinline__ bool reg_pivot(const CholArgs& C, RegShared& S, ColRe
             bv = S.rv[0];
             bi = S.ri[0];
             for (int w = 1; w < kRegCols / 64; ++w)
-                if (better(S.rv[w], S.ri[w], bv, bi)) {
+                if (pivot_better(pivot, S.rv[w], S.ri[w], bv, bi)) {
                     bv = S.rv[w];
                     bi = S.ri[w];
                 }
             im = bi;
             best_order = bv;
+            // rowmax: a NaN at i keeps i (nothing compares greater)
+            if (pivot == kPivotRowMax && (bi == INT32_MAX || isnan(S.dg[i - ib0] - S.dsum[i - ib0]))) im = i;
         }
-        const int a0 = i - ib0, a1 = im - ib0;
-        const double m_ii = S.dg[a1] - S.dsum[a1];  // read before the swap
+        const int a0 = i - ib0;
+        const double m_ii = S.dg[im - ib0] - S.dsum[im - ib0];  // read before the swap
+        // rank: genz stops at m_ii <= eps0 after the swap (src/cholesky.jl:518-541),
+        // rowmax at m_ii <= 0 before it (:329-350)
+        const bool stop = (pivot == kPivotGenz && !(m_ii > eps0)) || (pivot == kPivotRowMax && !(m_ii > 0.0));
+        if (stop && pivot == kPivotRowMax) im = i;
+        const int a1 = im - ib0;
         if (im != i) {
             int tp = S.perm[a0]; S.perm[a0] = S.perm[a1]; S.perm[a1] = tp;
             double t;
             t = S.dsum[a0]; S.dsum[a0] = S.dsum[a1]; S.dsum[a1] = t;
             t = S.sv[a0]; S.sv[a0] = S.sv[a1]; S.sv[a1] = t;
-            t = S.a[a0]; S.a[a0] = S.a[a1]; S.a[a1] = t;
-            t = S.b[a0]; S.b[a0] = S.b[a1]; S.b[a1] = t;
+            if (pivot != kPivotRowMax) {  // rowmax leaves a and b in place
+                t = S.a[a0]; S.a[a0] = S.a[a1]; S.a[a1] = t;
+                t = S.b[a0]; S.b[a0] = S.b[a1]; S.b[a1] = t;
+            }
             t = S.dg[a0]; S.dg[a0] = S.dg[a1]; S.dg[a1] = t;
         }
         S.im = im;
-        if (pivot && !(m_ii > eps0)) {
+        if (stop) {
             S.stop = 1;
             *C.rank = i;
         } else {
@@ -481,7 +533,9 @@ __device__ __forceinline__ bool reg_pivot(const CholArgs& C, RegShared& S, ColRe
             S.uii = uii;
             const double a_norm = (S.a[a0] - S.sv[a0]) / uii;
             const double b_norm = (S.b[a0] - S.sv[a0]) / uii;
-            S.yi = pivot ? (exp(-(a_norm * a_norm) / 2) - exp(-(b_norm * b_norm) / 2)) * is2pi / best_order : 0.0;
+            S.yi = pivot == kPivotGenz
+                       ? (exp(-(a_norm * a_norm) / 2) - exp(-(b_norm * b_norm) / 2)) * is2pi / best_order
+                       : 0.0;
         }
     }
     __syncthreads();
@@ -540,20 +594,22 @@ __device__ __forceinline__ bool reg_pivot(const CholArgs& C, RegShared& S, ColRe
     return true;
 }
 
-template <int E>
+template <int PIVOT, int E>
 __device__ __forceinline__ bool reg_pivots(const CholArgs& C, RegShared& S, ColRegs& R, int ng, int ib0, int ib1,
-                                           int w1, int pivot, double eps0)
+                                           int w1, double eps0)
 {
     if constexpr (E < 16) {
         if (ib0 + 16 * ng + E >= ib1) return false;  // uniform: the block is done
-        if (!reg_pivot<E>(C, S, R, ng, ib0, w1, pivot, eps0)) return false;
-        return reg_pivots<E + 1>(C, S, R, ng, ib0, ib1, w1, pivot, eps0);
+        if (!reg_pivot<PIVOT, E>(C, S, R, ng, ib0, w1, eps0)) return false;
+        return reg_pivots<PIVOT, E + 1>(C, S, R, ng, ib0, ib1, w1, eps0);
     }
     return true;
 }
 
-__global__ __launch_bounds__(kRegCols, 1) void chol_window_reg_kernel(CholArgs C, int w0, int w1, int ib0, int nb,
-                                                                      int pivot)
+// PIVOT: kPivotNone, kPivotGenz or kPivotRowMax (a template parameter, so that
+// each mode's register allocation is its own)
+template <int PIVOT>
+__global__ __launch_bounds__(kRegCols, 1) void chol_window_reg_kernel(CholArgs C, int w0, int w1, int ib0, int nb)
 {
     if (*C.rank < C.n) return;  // an earlier pivot found the matrix rank deficient
     __shared__ RegShared S;
@@ -578,7 +634,7 @@ __global__ __launch_bounds__(kRegCols, 1) void chol_window_reg_kernel(CholArgs C
     for (int e = 0; e < 16; ++e) R.g0[e] = R.g1[e] = R.g2[e] = R.cur[e] = 0.0;
     __syncthreads();
     for (int ng = 0; ng < kRegNb / 16; ++ng) {
-        if (!reg_pivots<0>(C, S, R, ng, ib0, ib1, w1, pivot, eps0)) break;
+        if (!reg_pivots<PIVOT, 0>(C, S, R, ng, ib0, ib1, w1, eps0)) break;
         // the completed group moves to its slot
         if (ng == 0) {
 #pragma unroll
@@ -603,16 +659,23 @@ hipError_t chol_syrk(const CholArgs& C, int k0, int k1, int r0, int r1, hipStrea
 rocblas_handle blas_handle(hipStream_t s);
 void* blas_handle_void(hipStream_t s) { return (void*)blas_handle(s); }
 
-hipError_t chol_window(const CholArgs& C, int w0, int w1, int nb, bool pivot, hipStream_t s)
+hipError_t chol_window(const CholArgs& C, int w0, int w1, int nb, int pivot, hipStream_t s)
 {
     for (int ib0 = w0; ib0 < w1; ib0 += nb) {
         const int ib1 = std::min(w1, ib0 + nb);
-        if (w1 - ib0 <= kRegCols && nb <= kRegNb)
-            hipLaunchKernelGGL(chol_window_reg_kernel, dim3(1), dim3(kRegCols), 0, s, C, w0, w1, ib0, nb,
-                               pivot ? 1 : 0);
-        else
-            hipLaunchKernelGGL(chol_window_kernel, dim3(1), dim3(kCholThreads), 0, s, C, w0, w1, ib0, nb,
-                               pivot ? 1 : 0);
+        if (w1 - ib0 <= kRegCols && nb <= kRegNb) {
+            if (pivot == kPivotGenz)
+                hipLaunchKernelGGL(chol_window_reg_kernel<kPivotGenz>, dim3(1), dim3(kRegCols), 0, s, C, w0, w1, ib0,
+                                   nb);
+            else if (pivot == kPivotRowMax)
+                hipLaunchKernelGGL(chol_window_reg_kernel<kPivotRowMax>, dim3(1), dim3(kRegCols), 0, s, C, w0, w1,
+                                   ib0, nb);
+            else
+                hipLaunchKernelGGL(chol_window_reg_kernel<kPivotNone>, dim3(1), dim3(kRegCols), 0, s, C, w0, w1, ib0,
+                                   nb);
+        } else {
+            hipLaunchKernelGGL(chol_window_kernel, dim3(1), dim3(kCholThreads), 0, s, C, w0, w1, ib0, nb, pivot);
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         // deferred update of the rest of the window by the tiled kernel

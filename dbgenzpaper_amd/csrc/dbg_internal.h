@@ -200,7 +200,7 @@ struct CholArgs {
     int32_t* rank;     // device scalar (n while full rank)
 };
 hipError_t chol_standardize(const CholArgs& C, bool mirror, hipStream_t s);
-hipError_t chol_window(const CholArgs& C, int w0, int w1, int nb, bool pivot, hipStream_t s);
+hipError_t chol_window(const CholArgs& C, int w0, int w1, int nb, int pivot, hipStream_t s);
 hipError_t chol_trsm_syrk(const CholArgs& C, int w0, int w1, hipStream_t s);
 hipError_t chol_finish(const CholArgs& C, hipStream_t s);
 int chol_max_window();

@@ -113,6 +113,33 @@ def cholesky_genz(M, a, b, block_size=2 ** 7, block_size2=2 ** 9, block_size3=2 
     return CholeskyGenz(W, perm.astype(np.int64), int(rank.value), av, bv, block_size, "genz", scale)
 
 
+def cholesky_rowmax(M, a, b, block_size=2 ** 7, block_size2=2 ** 9, block_size3=2 ** 10,
+                    block_size4=2 ** 11) -> CholeskyGenz:
+    """cholesky_rowmax!(M, a, b, bs1, bs2, bs3, bs4)  src/cholesky.jl:266-432.
+
+    The blocked loop nest of cholesky_genz! with diagonal pivoting (the largest
+    remaining Schur diagonal of the pivot window, first index on ties; the
+    factorization stops where it is not positive), in FP64 on the GPU.  Like
+    the reference it overwrites M (with U; strict lower triangle zeroed), a and
+    b (standardized, not permuted) when they are writable float64 arrays."""
+    Mi = _as_matrix(M)
+    n = Mi.shape[0]
+    W = np.array(Mi, dtype=np.float64, order="F", copy=True)
+    av = np.array(a, dtype=np.float64, copy=True).reshape(n)
+    bv = np.array(b, dtype=np.float64, copy=True).reshape(n)
+    perm = np.zeros(n, np.int32)
+    scale = np.zeros(n, np.float64)
+    rank = C.c_int32(0)
+    L.check(L.lib().dbg_chol_rowmax_f64(L.ptr(W, C.c_double), n, n, L.ptr(av, C.c_double),
+                                        L.ptr(bv, C.c_double), block_size, block_size2, block_size3,
+                                        block_size4, L.ptr(perm, C.c_int32), L.ptr(scale, C.c_double),
+                                        C.byref(rank)))
+    _write_back(M, W)
+    _write_back(a, av)
+    _write_back(b, bv)
+    return CholeskyGenz(W, perm.astype(np.int64), int(rank.value), av, bv, block_size, "row_max", scale)
+
+
 def cholesky_classic(M, a, b, block_size=2 ** 7, block_size2=2 ** 9, block_size3=2 ** 10,
                      block_size4=2 ** 11) -> CholeskyGenz:
     """cholesky_classic!(M, a, b, bs1, bs2, bs3, bs4)  src/cholesky.jl:144-242 (GPU, FP64)."""

@@ -114,6 +114,19 @@ int dbg_chol_genz_f64(double* M, int64_t lda, int32_t n, double* a, double* b, i
                       int32_t bs2, int32_t bs3, int32_t bs4, int32_t* perm, double* scale,
                       int32_t* rank);
 
+/* cholesky_rowmax!(M, a, b, bs1, bs2, bs3, bs4)  src/cholesky.jl:266-432
+ * The blocked loop nest of cholesky_genz! with diagonal pivoting: at each
+ * step the largest remaining Schur diagonal of the pivot window (first index
+ * on ties) becomes the pivot; the factorization stops (rank = pivots done)
+ * where that diagonal is not positive.  In place on host arrays: M receives
+ * the standardized factor U in its upper triangle (strict lower triangle
+ * zeroed); a and b are standardized but, as in the reference, not permuted.
+ * perm (0-based), scale (scale_vec) and rank are outputs.  FP64 on the
+ * device. */
+int dbg_chol_rowmax_f64(double* M, int64_t lda, int32_t n, double* a, double* b, int32_t bs1,
+                        int32_t bs2, int32_t bs3, int32_t bs4, int32_t* perm, double* scale,
+                        int32_t* rank);
+
 /* cholesky_classic!(M, a, b, bs1, bs2, bs3, bs4)  src/cholesky.jl:144-242
  * Unpivoted, in place on host arrays, FP64 on the device. */
 int dbg_chol_classic_f64(double* M, int64_t lda, int32_t n, double* a, double* b, int32_t bs1,

@@ -370,6 +370,103 @@ int orc_cholesky_genz(double* M, int n, double* a, double* b, int bs1, int bs2, 
     return 0;
 }
 
+/* cholesky_rowmax!  src/cholesky.jl:266-432.  The loop nest of cholesky_genz!
+ * with diagonal pivoting: at step i the pivot is the largest m_kk = M[k,k] -
+ * d[k] over k in [i, ii1), scanned from i with a strict > (:319-327, so the
+ * first maximum wins and a NaN after i is never taken); the rank check
+ * m_ii > 0 (no eps0) comes before the swap (:329-336); a and b are
+ * standardized (:285-300) but not permuted.  perm is 0-based. */
+int orc_cholesky_rowmax(double* M, int n, double* a, double* b, int bs1, int bs2, int bs3,
+                        int bs4, int* perm, double* scale, int* rank_out)
+{
+    (void)standardize(M, n, a, b, scale, 1);
+    double* d_vec = calloc((size_t)n, sizeof(double));
+    if (!d_vec) return -1;
+    int rank = n;
+    for (int j = 0; j < n; ++j) perm[j] = j;
+
+    for (int iv0 = 0; iv0 < n && rank == n; iv0 += bs4) {
+        const int iv1 = iv0 + bs4 < n ? iv0 + bs4 : n;
+        for (int iii0 = iv0; iii0 < iv1 && rank == n; iii0 += bs3) {
+            const int iii1 = iii0 + bs3 < iv1 ? iii0 + bs3 : iv1;
+            for (int ii0 = iii0; ii0 < iii1 && rank == n; ii0 += bs2) {
+                const int ii1 = ii0 + bs2 < iii1 ? ii0 + bs2 : iii1;
+                for (int i0 = ii0; i0 < ii1 && rank == n; i0 += bs1) {
+                    const int i1 = i0 + bs1 < ii1 ? i0 + bs1 : ii1;
+                    for (int k = i0; k < ii1; ++k) d_vec[k] = 0.0;
+                    for (int i = i0; i < i1; ++i) {
+                        int i_m = i;
+                        double m_ii = MM(i, i) - d_vec[i];
+                        for (int k = i; k < ii1; ++k) {
+                            const double m_kk = MM(k, k) - d_vec[k];
+                            if (m_kk > m_ii) {
+                                m_ii = m_kk;
+                                i_m = k;
+                            }
+                        }
+                        if (m_ii > 0) {
+                            m_ii = sqrt(m_ii);
+                        } else {
+                            rank = i;
+                            break;
+                        }
+                        if (i_m != i) {
+                            int tp = perm[i]; perm[i] = perm[i_m]; perm[i_m] = tp;
+                            double t;
+                            t = d_vec[i]; d_vec[i] = d_vec[i_m]; d_vec[i_m] = t;
+                            for (int k = i0; k < n; ++k) {
+                                t = MM(i, k); MM(i, k) = MM(i_m, k); MM(i_m, k) = t;
+                            }
+                            for (int k = 0; k < n; ++k) {
+                                t = MM(k, i); MM(k, i) = MM(k, i_m); MM(k, i_m) = t;
+                            }
+                        }
+                        MM(i, i) = m_ii;
+                        if (i == i0) {
+                            for (int j = i + 1; j < ii1; ++j) {
+                                const double m_ij = MM(i, j) / m_ii;
+                                MM(i, j) = m_ij;
+                                MM(j, i) = m_ij;
+                                d_vec[j] += m_ij * m_ij;
+                            }
+                        } else if (i < ii1 - 1) {
+                            for (int r = i + 1; r < ii1; ++r) {
+                                double acc = MM(r, i);
+                                for (int l = i0; l < i; ++l) acc -= MM(r, l) * MM(l, i);
+                                MM(r, i) = acc;
+                            }
+                            for (int j = i + 1; j < ii1; ++j) {
+                                const double m_ij = MM(j, i) / m_ii;
+                                MM(i, j) = m_ij;
+                                MM(j, i) = m_ij;
+                                d_vec[j] += m_ij * m_ij;
+                            }
+                        }
+                    }
+                    if (rank < n) break;
+                    if (i1 < ii1) {
+                        for (int c = i1; c < ii1; ++c)
+                            for (int r = i1; r < ii1; ++r) {
+                                double acc = MM(r, c);
+                                for (int l = i0; l < i1; ++l) acc -= MM(r, l) * MM(l, c);
+                                MM(r, c) = acc;
+                            }
+                    }
+                }
+                if (rank < n) break;
+                if (ii1 < iii1) trsm_gemm_update(M, n, ii0, ii1, ii1, iii1);
+            }
+            if (rank < n) break;
+            if (iii1 < iv1) trsm_gemm_update(M, n, iii0, iii1, iii1, iv1);
+        }
+        if (rank < n) break;
+        if (iv1 < n) trsm_gemm_update(M, n, iv0, iv1, iv1, n);
+    }
+    free(d_vec);
+    *rank_out = rank;
+    return 0;
+}
+
 /* cholesky_classic!  src/cholesky.jl:144-242 (no pivoting, no rank check). */
 int orc_cholesky_classic(double* M, int n, double* a, double* b, int bs1, int bs2, int bs3,
                          int bs4, double* scale)

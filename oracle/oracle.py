@@ -47,6 +47,7 @@ def lib():
         L.orc_sobol_block.argtypes = [C.c_int, u32p, C.c_uint64, C.c_int64, u64p]
         L.orc_cholesky_genz.argtypes = [dp, C.c_int, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int,
                                         ip, dp, ip, dp]
+        L.orc_cholesky_rowmax.argtypes = [dp, C.c_int, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int, ip, dp, ip]
         L.orc_cholesky_classic.argtypes = [dp, C.c_int, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int, dp]
         L.orc_truncate_U.argtypes = [dp, C.c_int, C.c_double]
         L.orc_qmc_pnorm.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, i64p, C.POINTER(C.c_int32), dp,
@@ -137,6 +138,22 @@ def cholesky_genz(M, a, b, bs1=2 ** 7, bs2=2 ** 9, bs3=2 ** 10, bs4=2 ** 11) -> 
                                  C.byref(rank), _ptr(gaps, C.c_double))
     assert rc == 0
     return CholeskyResult(M, perm.astype(np.int64), rank.value, a, b, scale, gaps)
+
+
+def cholesky_rowmax(M, a, b, bs1=2 ** 7, bs2=2 ** 9, bs3=2 ** 10, bs4=2 ** 11) -> CholeskyResult:
+    """cholesky_rowmax!  src/cholesky.jl:266-432 (on copies)."""
+    M = np.array(M, np.float64, order="F", copy=True)
+    a = np.array(a, np.float64, copy=True)
+    b = np.array(b, np.float64, copy=True)
+    n = M.shape[0]
+    perm = np.zeros(n, np.int32)
+    scale = np.zeros(n, np.float64)
+    rank = C.c_int(0)
+    rc = lib().orc_cholesky_rowmax(_ptr(M, C.c_double), n, _ptr(a, C.c_double), _ptr(b, C.c_double),
+                                   bs1, bs2, bs3, bs4, _ptr(perm, C.c_int), _ptr(scale, C.c_double),
+                                   C.byref(rank))
+    assert rc == 0
+    return CholeskyResult(M, perm.astype(np.int64), rank.value, a, b, scale)
 
 
 def cholesky_classic(M, a, b, bs1=2 ** 7, bs2=2 ** 9, bs3=2 ** 10, bs4=2 ** 11) -> CholeskyResult:

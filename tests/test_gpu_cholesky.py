@@ -99,3 +99,51 @@ def test_classic_matches_oracle(dbg, oracle, bs):
     Ug = np.triu(Fg.U)
     assert np.abs(standardized(C) - Ug.T @ Ug).max() < 1e-10
     assert np.abs(Ug - np.triu(Fo.U)).max() < 1e-12
+
+
+@pytest.mark.parametrize("n,bs,seed", [
+    (16, (2 ** 7, 2 ** 9, 2 ** 10, 2 ** 11), 31),
+    (100, (8, 32, 64, 128), 32),            # several windows, inner blocks, iii/iv nesting
+    (300, (16, 64, 128, 256), 33),
+    (700, (64, 512, 1024, 2048), 34),       # 512-column windows: the register-window kernel
+    (1024, (64, 1024, 1024, 2048), 35),     # a 1024-column window: the LDS-state kernel first
+])
+def test_rowmax_matches_oracle(dbg, oracle, n, bs, seed):
+    """cholesky_rowmax! (src/cholesky.jl:266-432) on the device against the
+    oracle: the same permutation, factor to 1e-12, a and b standardized and
+    not permuted, the reference's invariant C[p, p] = U'U."""
+    C = random_spd(n, seed)
+    rng = np.random.default_rng(seed)
+    a = -rng.uniform(0.5, 2.0, n)
+    b = rng.uniform(0.5, 2.0, n)
+    print(f"rowmax n={n} bs={bs} seed={seed}", flush=True)
+    Fo = oracle.cholesky_rowmax(C, a, b, *bs)
+    Fg = dbg.cholesky_rowmax(C.copy(order="F"), a.copy(), b.copy(), *bs)
+    assert Fg.rank == n == Fo.rank
+    assert np.array_equal(Fg.perm, Fo.perm)
+    Ug = np.triu(Fg.U)
+    assert np.abs(Ug - np.triu(Fo.U)).max() < 1e-12
+    assert np.array_equal(Fg.a, Fo.a) and np.array_equal(Fg.b, Fo.b)
+    assert np.array_equal(Fg.scale_vec, Fo.scale_vec)
+    assert np.abs(standardized(C)[np.ix_(Fg.perm, Fg.perm)] - Ug.T @ Ug).max() < 1e-10
+
+
+def test_rowmax_rank_stop(dbg, oracle):
+    """Identical variables: every Schur diagonal after the first pivot is 0
+    exactly and the factorization stops at rank 1 with the identity order.
+    Two copies of each of 5 variables: the 5 leading pivots equal the
+    oracle's, one of each pair; the copies' Schur diagonals are then at
+    rounding level (1e-16), where the order the rule picks depends on the
+    summation order and is not compared."""
+    n = 16
+    F = dbg.cholesky_rowmax(np.ones((n, n)), -np.ones(n), np.ones(n), 4, 16, 16, 16)
+    assert F.rank == 1 and list(F.perm) == list(range(n))
+    rng = np.random.default_rng(4)
+    B = rng.standard_normal((5, 5))
+    E = np.kron(B @ B.T + 5 * np.eye(5), np.ones((2, 2)))
+    Fo = oracle.cholesky_rowmax(E, -np.ones(10), np.ones(10), 2, 16, 16, 16)
+    Fg = dbg.cholesky_rowmax(E.copy(order="F"), -np.ones(10), np.ones(10), 2, 16, 16, 16)
+    assert 5 <= Fg.rank <= 10 and 5 <= Fo.rank <= 10
+    assert np.array_equal(Fg.perm[:5], Fo.perm[:5])
+    assert sorted(int(p) // 2 for p in Fg.perm[:5]) == list(range(5))
+    assert np.abs(np.triu(Fg.U)[:5, :5] - np.triu(Fo.U)[:5, :5]).max() < 1e-12

@@ -186,6 +186,95 @@ def test_rank_deficient(oracle):
     assert F.rank == 20
 
 
+def test_rowmax_reference_invariant(oracle):
+    """cholesky_rowmax! (src/cholesky.jl:266-432): C[p, p] = U'U for a full-rank
+    matrix; a and b standardized but not permuted."""
+    n = 300
+    C = random_spd(n, 103)
+    rng = np.random.default_rng(3)
+    a, b = -rng.uniform(0.5, 2, n), rng.uniform(0.5, 2, n)
+    F = oracle.cholesky_rowmax(C, a, b, 16, 64, 128, 256)
+    U = np.triu(F.U)
+    assert F.rank == n and sorted(F.perm) == list(range(n))
+    assert np.abs(standardized(C)[np.ix_(F.perm, F.perm)] - U.T @ U).max() < 1e-10
+    d = np.sqrt(np.diag(C))
+    assert np.array_equal(F.a, a / d) or np.allclose(F.a, a / d, rtol=1e-15, atol=0)
+    assert np.allclose(F.b, b / d, rtol=1e-15, atol=0) and np.array_equal(F.scale_vec, d)
+
+
+@pytest.mark.parametrize("n,bs,seed", [(40, (8, 64, 64, 64), 11), (97, (16, 128, 128, 128), 12),
+                                       (150, (64, 256, 256, 256), 13)])
+def test_rowmax_one_window_equals_lapack_pstrf(oracle, n, bs, seed):
+    """With one pivot window over the whole matrix the rule is plain diagonal
+    pivoting, LAPACK's dpstrf (largest remaining diagonal, first on ties):
+    the same permutation and factor, whatever the inner blocking."""
+    from scipy.linalg import lapack
+
+    C = random_spd(n, seed, jitter=n / 8)
+    F = oracle.cholesky_rowmax(C, -np.ones(n), np.ones(n), *bs)
+    Cs = standardized(C)
+    np.fill_diagonal(Cs, 1.0)  # the reference's standardization writes exact ones (src/cholesky.jl:289)
+    c, piv, rank, info = lapack.dpstrf(Cs, tol=-1.0, lower=0)
+    assert info == 0 and rank == n == F.rank
+    assert np.array_equal(F.perm, piv - 1)
+    assert np.abs(np.triu(F.U) - np.triu(c)).max() < 1e-12
+
+
+def _rowmax_unblocked(C, bs2):
+    """Textbook right-looking diagonal pivoting with the reference's window rule
+    (the pivot is searched inside windows of bs2 columns only); independent of
+    the oracle's four-level blocked structure."""
+    n = C.shape[0]
+    d = np.sqrt(np.diag(C))
+    M = C / np.outer(d, d)
+    np.fill_diagonal(M, 1.0)
+    perm = np.arange(n)
+    U = np.zeros((n, n))
+    for w0 in range(0, n, bs2):
+        w1 = min(n, w0 + bs2)
+        for i in range(w0, w1):
+            im = i + int(np.argmax(np.diag(M)[i:w1]))
+            perm[[i, im]] = perm[[im, i]]
+            M[[i, im], :] = M[[im, i], :]
+            M[:, [i, im]] = M[:, [im, i]]
+            U[:, [i, im]] = U[:, [im, i]]
+            uii = np.sqrt(M[i, i])
+            U[i, i] = uii
+            U[i, i + 1:] = M[i, i + 1:] / uii
+            M[i + 1:, i + 1:] -= np.outer(U[i, i + 1:], U[i, i + 1:])
+    return U, perm
+
+
+@pytest.mark.parametrize("n,bs,seed", [(12, (2, 4, 8, 8), 21), (37, (4, 8, 16, 32), 22),
+                                       (90, (5, 20, 40, 80), 23)])
+def test_rowmax_blocked_equals_unblocked(oracle, n, bs, seed):
+    C = random_spd(n, seed, jitter=n / 4)
+    F = oracle.cholesky_rowmax(C, -np.ones(n), np.ones(n), *bs)
+    U, perm = _rowmax_unblocked(C, bs[1])
+    assert np.array_equal(F.perm, perm)
+    assert np.abs(np.triu(F.U) - U).max() < 1e-12
+
+
+def test_rowmax_exactly_singular(oracle):
+    """Identical variables: after the first pivot every remaining Schur diagonal
+    is exactly 0, and the rank check m_ii > 0 stops at rank 1 (ties keep the
+    first index).  Two copies of each of 5 variables: the first 5 pivots take
+    one of each pair; the copies' Schur diagonals are left at rounding level
+    and, having no eps threshold, the reference's rule keeps factoring while
+    they are positive (pivots of order 1e-8 at most)."""
+    n = 16
+    F = oracle.cholesky_rowmax(np.ones((n, n)), -np.ones(n), np.ones(n), 4, 16, 16, 16)
+    assert F.rank == 1 and list(F.perm) == list(range(n))
+    rng = np.random.default_rng(4)
+    B = rng.standard_normal((5, 5))
+    S = B @ B.T + 5 * np.eye(5)
+    E = np.kron(S, np.ones((2, 2)))  # variables 2k and 2k+1 identical
+    F = oracle.cholesky_rowmax(E, -np.ones(10), np.ones(10), 2, 16, 16, 16)
+    assert 5 <= F.rank <= 10
+    assert sorted(int(p) // 2 for p in F.perm[:5]) == list(range(5))
+    assert np.all(np.diag(F.U)[:5] > 0.1) and np.all(np.abs(np.diag(F.U)[5:F.rank]) < 1e-6)
+
+
 # --- integration -------------------------------------------------------------------
 def _run(oracle, Sigma, a, b, *, m, R, seed=0, phi="erf", kind="dense", gen="Richtmyer", **kw):
     n = Sigma.shape[0]
