@@ -27,6 +27,7 @@ CONTRACT_AUTO, CONTRACT_FP32, CONTRACT_SPLIT16 = 0, 1, 2
 # every exported symbol of include/dbgenz.h
 EXPORTS = (
     "dbg_last_error", "dbg_abi_version", "dbg_opts_default", "dbg_chol_genz_f64", "dbg_chol_rowmax_f64",
+    "dbg_adapt_low_rank_f64",
     "dbg_chol_classic_f64", "dbg_plan_create", "dbg_plan_destroy", "dbg_plan_set_shifts",
     "dbg_plan_set_sobol_directions", "dbg_qmc_pnorm", "dbg_qmc_loop", "dbg_plan_sync",
     "dbg_plan_last_kernel_ms", "dbg_plan_summary", "dbg_plan_get_factor", "dbg_plan_get_reps",
@@ -81,6 +82,7 @@ def lib():
     chol = [dp, C.c_int64, C.c_int32, dp, dp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
     L.dbg_chol_genz_f64.argtypes = chol + [ip32, dp, ip32]
     L.dbg_chol_rowmax_f64.argtypes = chol + [ip32, dp, ip32]
+    L.dbg_adapt_low_rank_f64.argtypes = [dp, C.c_int64, C.c_int32, dp, dp, C.c_int32, ip32, C.c_double, ip32, ip32]
     L.dbg_chol_classic_f64.argtypes = chol + [dp]
     L.dbg_plan_create.argtypes = [C.POINTER(vp), C.POINTER(Opts), C.c_int32, dp, C.c_int64, dp, dp]
     L.dbg_plan_destroy.argtypes = [vp]

@@ -733,6 +733,47 @@ int dbg_chol_rowmax_f64(double* M, int64_t lda, int32_t n, double* a, double* b,
     return chol_host(M, lda, n, a, b, bs1, bs2, bs3, bs4, perm, scale, rank, 2);
 }
 
+int dbg_adapt_low_rank_f64(double* M, int64_t lda, int32_t n, double* a, double* b, int32_t rank, int32_t* perm,
+                           double eps1, int32_t* k_js, int32_t* k_0)
+{
+    g_err.clear();
+    if (n < 1 || lda < n || !M || !a || !b || !perm || !k_0 || rank < 0 || rank > n || (rank > 0 && !k_js))
+        return fail(DBG_ERR_ARG, "invalid adapt_low_rank arguments");
+    hipStream_t s;
+    HIPCHK(hipStreamCreate(&s));
+    const size_t nn = (size_t)n * n;
+    double *dM = nullptr, *da = nullptr, *db = nullptr, *dtmp = nullptr;
+    int32_t *dperm = nullptr, *dend = nullptr, *dk0 = nullptr;
+    hipError_t e = dalloc(&dM, nn * sizeof(double));
+    if (e == hipSuccess) e = dalloc(&da, n * sizeof(double));
+    if (e == hipSuccess) e = dalloc(&db, n * sizeof(double));
+    if (e == hipSuccess) e = dalloc(&dtmp, nn * sizeof(double));
+    if (e == hipSuccess) e = dalloc(&dperm, n * sizeof(int32_t));
+    if (e == hipSuccess) e = dalloc(&dend, (size_t)std::max(rank, 1) * sizeof(int32_t));
+    if (e == hipSuccess) e = dalloc(&dk0, sizeof(int32_t));
+    if (e == hipSuccess)
+        e = hipMemcpy2DAsync(dM, (size_t)n * sizeof(double), M, (size_t)lda * sizeof(double), (size_t)n * sizeof(double),
+                             n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(da, a, n * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(db, b, n * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dperm, perm, n * sizeof(int32_t), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = adapt_low_rank(dM, n, n, da, db, dperm, rank, eps1, dend, dtmp, dk0, s);
+    if (e == hipSuccess)
+        e = hipMemcpy2DAsync(M, (size_t)lda * sizeof(double), dM, (size_t)n * sizeof(double), (size_t)n * sizeof(double),
+                             n, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(a, da, n * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(b, db, n * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(perm, dperm, n * sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && rank > 0) e = hipMemcpyAsync(k_js, dend, rank * sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(k_0, dk0, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    for (void* p : {(void*)dM, (void*)da, (void*)db, (void*)dtmp, (void*)dperm, (void*)dend, (void*)dk0})
+        (void)hipFree(p);
+    (void)hipStreamDestroy(s);
+    if (e != hipSuccess) return fail(DBG_ERR_DEVICE, std::string("adapt_low_rank: ") + hipGetErrorString(e));
+    return DBG_OK;
+}
+
 int dbg_chol_classic_f64(double* M, int64_t lda, int32_t n, double* a, double* b, int32_t bs1,
                          int32_t bs2, int32_t bs3, int32_t bs4, double* scale)
 {

@@ -798,6 +798,134 @@ hipError_t chol_finish(const CholArgs& C, hipStream_t s)
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// adapt_low_rank!  src/cholesky.jl:20-123, one workgroup.  Columns rank .. n-1
+// of the factor (the dependent variables) are taken in turn: the level i_j of
+// column j is its last row below rank with |M[i, j]| > eps1 (a parallel max
+// over the rows); a column without one moves to the end (swap with the last
+// active column, k_0 += 1); otherwise rows 0 .. i_j are scaled by 1/M[i_j, j]
+// (a and b too, exchanged when that entry is negative) and the column is
+// inserted after the block of its level, end[i_j] + 1, by rotating the columns
+// in between right by one through the scratch buffer (two element-parallel
+// passes).  end[i] is the 0-based last column of level i's block (the
+// reference's k_js minus one).  Every branch depends on values broadcast
+// through LDS, so the control flow is uniform.
+// ---------------------------------------------------------------------------
+constexpr int kAlrThreads = 1024;
+
+__global__ __launch_bounds__(kAlrThreads) void adapt_low_rank_kernel(double* __restrict__ M, int64_t lda, int n,
+                                                                    double* __restrict__ a, double* __restrict__ b,
+                                                                    int32_t* __restrict__ perm, int rank,
+                                                                    double eps1, int32_t* __restrict__ end,
+                                                                    double* __restrict__ tmp,
+                                                                    int32_t* __restrict__ k0_out)
+{
+    __shared__ int s_lvl[kAlrThreads / 64];
+    __shared__ int s_ij, s_j, s_nact, s_k0;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < rank; i += kAlrThreads) end[i] = i;
+    if (tid == 0) {
+        s_j = rank;
+        s_nact = n;
+        s_k0 = 0;
+    }
+    __syncthreads();
+    for (;;) {
+        const int j = s_j, nact = s_nact;
+        if (j >= nact) break;  // uniform
+        // 1. the level: last row i < rank with |M[i, j]| > eps1 (-1: none)
+        int lv = -1;
+        for (int i = tid; i < rank; i += kAlrThreads)
+            if (fabs(MM(i, j)) > eps1) lv = i;
+        lv = wave_max_int(lv);
+        if ((tid & 63) == 0) s_lvl[tid >> 6] = lv;
+        __syncthreads();
+        if (tid == 0) {
+            int m = -1;
+            for (int w = 0; w < kAlrThreads / 64; ++w) m = max(m, s_lvl[w]);
+            s_ij = m;
+        }
+        __syncthreads();
+        const int ij = s_ij;
+        if (ij < 0) {
+            // a zero column: to the end of the active range
+            const int last = nact - 1;
+            if (j != last) {
+                for (int r = tid; r < n; r += kAlrThreads) {
+                    const double t = MM(r, last);
+                    MM(r, last) = MM(r, j);
+                    MM(r, j) = t;
+                }
+                if (tid == 0) {
+                    double t = a[j]; a[j] = a[last]; a[last] = t;
+                    t = b[j]; b[j] = b[last]; b[last] = t;
+                    const int32_t p = perm[j]; perm[j] = perm[last]; perm[last] = p;
+                }
+            }
+            if (tid == 0) {
+                s_k0 += 1;
+                s_nact = last;  // j stays: the column swapped in is taken next
+            }
+            __syncthreads();
+            continue;
+        }
+        // 2. scale the column up to its level, and the bounds
+        const double mij = MM(ij, j);
+        const double inv = 1.0 / mij;
+        for (int r = tid; r <= ij; r += kAlrThreads) MM(r, j) *= inv;
+        if (tid == 0) {
+            a[j] *= inv;
+            b[j] *= inv;
+            if (mij < 0.0) {
+                const double t = a[j];
+                a[j] = b[j];
+                b[j] = t;
+            }
+        }
+        const int pos = end[ij] + 1;
+        __syncthreads();
+        // 3. insert column j at pos: columns pos .. j-1 move right by one
+        if (j != pos) {
+            const int ncol = j - pos + 1;
+            const int64_t cnt = (int64_t)ncol * n;
+            for (int64_t e = tid; e < cnt; e += kAlrThreads) {
+                const int c = pos + (int)(e / n), r = (int)(e % n);
+                tmp[e] = MM(r, c);
+            }
+            __syncthreads();
+            for (int64_t e = tid; e < cnt; e += kAlrThreads) {
+                const int c = pos + (int)(e / n), r = (int)(e % n);
+                MM(r, c == j ? pos : c + 1) = tmp[e];
+            }
+            if (tid == 0) {
+                const double va = a[j], vb = b[j];
+                const int32_t vp = perm[j];
+                for (int c = j; c > pos; --c) {
+                    a[c] = a[c - 1];
+                    b[c] = b[c - 1];
+                    perm[c] = perm[c - 1];
+                }
+                a[pos] = va;
+                b[pos] = vb;
+                perm[pos] = vp;
+            }
+        }
+        // 4. the blocks of levels ij .. rank-1 end one column later
+        for (int i = ij + tid; i < rank; i += kAlrThreads) end[i] += 1;
+        if (tid == 0) s_j = j + 1;
+        __syncthreads();
+    }
+    if (tid == 0) *k0_out = s_k0;
+}
+
+hipError_t adapt_low_rank(double* M, int64_t lda, int n, double* a, double* b, int32_t* perm, int rank, double eps1,
+                          int32_t* end, double* tmp, int32_t* k0, hipStream_t s)
+{
+    hipLaunchKernelGGL(adapt_low_rank_kernel, dim3(1), dim3(kAlrThreads), 0, s, M, lda, n, a, b, perm, rank, eps1,
+                       end, tmp, k0);
+    return hipGetLastError();
+}
+
 #undef MM
 }  // namespace dbg
 

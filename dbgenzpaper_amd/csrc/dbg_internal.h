@@ -203,6 +203,10 @@ hipError_t chol_standardize(const CholArgs& C, bool mirror, hipStream_t s);
 hipError_t chol_window(const CholArgs& C, int w0, int w1, int nb, int pivot, hipStream_t s);
 hipError_t chol_trsm_syrk(const CholArgs& C, int w0, int w1, hipStream_t s);
 hipError_t chol_finish(const CholArgs& C, hipStream_t s);
+// adapt_low_rank! on device arrays (M n x n, lda; end: rank entries, out; tmp:
+// n * n doubles of scratch; k0: one entry, out)
+hipError_t adapt_low_rank(double* M, int64_t lda, int n, double* a, double* b, int32_t* perm, int rank, double eps1,
+                          int32_t* end, double* tmp, int32_t* k0, hipStream_t s);
 int chol_max_window();
 
 // Post-factor preparation.

@@ -140,6 +140,37 @@ def cholesky_rowmax(M, a, b, block_size=2 ** 7, block_size2=2 ** 9, block_size3=
     return CholeskyGenz(W, perm.astype(np.int64), int(rank.value), av, bv, block_size, "row_max", scale)
 
 
+def adapt_low_rank(M, a, b, rank, perm, eps1):
+    """adapt_low_rank!(M, a, b, rank, perm, eps1)  src/cholesky.jl:20-123, on the GPU.
+
+    Reorders the dependent columns rank .. n-1 of a rank-deficient factor:
+    each is scaled so that its last entry above row rank with |value| > eps1
+    is 1 (a, b scaled too, exchanged when that entry was negative) and moved
+    to the end of the block of that level; zero columns go to the end.  M, a,
+    b and perm (0-based) are updated in place when they are writable arrays of
+    the right type (float64, float64, float64, integer).  Returns (k_js, k_0):
+    each level's last column, 0-based (the reference's k_js minus one), and the
+    number of zero columns."""
+    Mi = _as_matrix(M)
+    n = Mi.shape[0]
+    if not 0 <= rank <= n:
+        raise ValueError("rank must be in [0, n]")
+    W = np.array(Mi, dtype=np.float64, order="F", copy=True)
+    av = np.array(a, dtype=np.float64, copy=True).reshape(n)
+    bv = np.array(b, dtype=np.float64, copy=True).reshape(n)
+    pv = np.array(perm, dtype=np.int32, copy=True).reshape(n)
+    kjs = np.zeros(max(rank, 1), np.int32)
+    k0 = C.c_int32(0)
+    L.check(L.lib().dbg_adapt_low_rank_f64(L.ptr(W, C.c_double), n, n, L.ptr(av, C.c_double),
+                                           L.ptr(bv, C.c_double), int(rank), L.ptr(pv, C.c_int32), float(eps1),
+                                           L.ptr(kjs, C.c_int32), C.byref(k0)))
+    _write_back(M, W)
+    _write_back(a, av)
+    _write_back(b, bv)
+    _write_back(perm, pv)
+    return kjs[:rank].astype(np.int64), int(k0.value)
+
+
 def cholesky_classic(M, a, b, block_size=2 ** 7, block_size2=2 ** 9, block_size3=2 ** 10,
                      block_size4=2 ** 11) -> CholeskyGenz:
     """cholesky_classic!(M, a, b, bs1, bs2, bs3, bs4)  src/cholesky.jl:144-242 (GPU, FP64)."""

@@ -127,6 +127,18 @@ int dbg_chol_rowmax_f64(double* M, int64_t lda, int32_t n, double* a, double* b,
                         int32_t bs2, int32_t bs3, int32_t bs4, int32_t* perm, double* scale,
                         int32_t* rank);
 
+/* adapt_low_rank!(M, a, b, rank, perm, eps1)  src/cholesky.jl:20-123
+ * Reorders the dependent columns rank .. n-1 of a rank-deficient factor M
+ * (n x n, lda) in place: each column whose last entry above row rank with
+ * |value| > eps1 sits in row i_j is scaled so that entry is 1 (a, b scaled
+ * too and exchanged when it was negative) and moved to the end of the block
+ * of level i_j; columns with no such entry go to the end.  a, b and perm
+ * (0-based here) move with their columns.  k_js (rank entries, out) receives
+ * each level's last column, 0-based (the reference's k_js minus one); k_0
+ * (out) the number of zero columns.  Runs on the device (one workgroup). */
+int dbg_adapt_low_rank_f64(double* M, int64_t lda, int32_t n, double* a, double* b, int32_t rank, int32_t* perm,
+                           double eps1, int32_t* k_js, int32_t* k_0);
+
 /* cholesky_classic!(M, a, b, bs1, bs2, bs3, bs4)  src/cholesky.jl:144-242
  * Unpivoted, in place on host arrays, FP64 on the device. */
 int dbg_chol_classic_f64(double* M, int64_t lda, int32_t n, double* a, double* b, int32_t bs1,

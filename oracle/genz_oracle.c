@@ -467,6 +467,68 @@ int orc_cholesky_rowmax(double* M, int n, double* a, double* b, int bs1, int bs2
     return 0;
 }
 
+/* adapt_low_rank!  src/cholesky.jl:20-123, sequentially as written (perm
+ * 0-based; k_js receives each level's last column, 0-based). */
+int orc_adapt_low_rank(double* M, int n, double* a, double* b, int rank, int* perm, double eps1,
+                       int* k_js, int* k0_out)
+{
+    double* tmp = malloc((size_t)n * sizeof(double));
+    if (!tmp) return -1;
+    for (int i = 0; i < rank; ++i) k_js[i] = i;
+    int k0 = 0, j = rank, n_active = n;
+    while (j < n_active) {
+        int ij = -1;
+        for (int i = rank - 1; i >= 0; --i)
+            if (fabs(MM(i, j)) > eps1) {
+                ij = i;
+                break;
+            }
+        if (ij < 0) {
+            const int last = n_active - 1;
+            if (j != last) {
+                for (int r = 0; r < n; ++r) {
+                    const double t = MM(r, last); MM(r, last) = MM(r, j); MM(r, j) = t;
+                }
+                double t = a[j]; a[j] = a[last]; a[last] = t;
+                t = b[j]; b[j] = b[last]; b[last] = t;
+                const int p = perm[j]; perm[j] = perm[last]; perm[last] = p;
+            }
+            k0 += 1;
+            n_active -= 1;
+            continue;
+        }
+        const double m_ij = MM(ij, j);
+        const double inv_m = 1.0 / m_ij;
+        for (int k = 0; k <= ij; ++k) MM(k, j) *= inv_m;
+        a[j] *= inv_m;
+        b[j] *= inv_m;
+        if (m_ij < 0.0) {
+            const double t = a[j]; a[j] = b[j]; b[j] = t;
+        }
+        const int pos = k_js[ij] + 1;
+        if (j != pos) {
+            for (int r = 0; r < n; ++r) tmp[r] = MM(r, j);
+            const double va = a[j], vb = b[j];
+            const int vp = perm[j];
+            for (int c = j; c > pos; --c) {
+                for (int r = 0; r < n; ++r) MM(r, c) = MM(r, c - 1);
+                a[c] = a[c - 1];
+                b[c] = b[c - 1];
+                perm[c] = perm[c - 1];
+            }
+            for (int r = 0; r < n; ++r) MM(r, pos) = tmp[r];
+            a[pos] = va;
+            b[pos] = vb;
+            perm[pos] = vp;
+        }
+        for (int k = ij; k < rank; ++k) k_js[k] += 1;
+        j += 1;
+    }
+    free(tmp);
+    *k0_out = k0;
+    return 0;
+}
+
 /* cholesky_classic!  src/cholesky.jl:144-242 (no pivoting, no rank check). */
 int orc_cholesky_classic(double* M, int n, double* a, double* b, int bs1, int bs2, int bs3,
                          int bs4, double* scale)

@@ -48,6 +48,7 @@ def lib():
         L.orc_cholesky_genz.argtypes = [dp, C.c_int, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int,
                                         ip, dp, ip, dp]
         L.orc_cholesky_rowmax.argtypes = [dp, C.c_int, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int, ip, dp, ip]
+        L.orc_adapt_low_rank.argtypes = [dp, C.c_int, dp, dp, C.c_int, ip, C.c_double, ip, ip]
         L.orc_cholesky_classic.argtypes = [dp, C.c_int, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int, dp]
         L.orc_truncate_U.argtypes = [dp, C.c_int, C.c_double]
         L.orc_qmc_pnorm.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, i64p, C.POINTER(C.c_int32), dp,
@@ -154,6 +155,22 @@ def cholesky_rowmax(M, a, b, bs1=2 ** 7, bs2=2 ** 9, bs3=2 ** 10, bs4=2 ** 11) -
                                    C.byref(rank))
     assert rc == 0
     return CholeskyResult(M, perm.astype(np.int64), rank.value, a, b, scale)
+
+
+def adapt_low_rank(M, a, b, rank, perm, eps1):
+    """adapt_low_rank!  src/cholesky.jl:20-123 (on copies; perm and k_js 0-based).
+    Returns (M, a, b, perm, k_js, k_0)."""
+    M = np.array(M, np.float64, order="F", copy=True)
+    a = np.array(a, np.float64, copy=True)
+    b = np.array(b, np.float64, copy=True)
+    perm = np.array(perm, np.int32, copy=True)
+    n = M.shape[0]
+    kjs = np.zeros(max(rank, 1), np.int32)
+    k0 = C.c_int(0)
+    rc = lib().orc_adapt_low_rank(_ptr(M, C.c_double), n, _ptr(a, C.c_double), _ptr(b, C.c_double), rank,
+                                  _ptr(perm, C.c_int), eps1, _ptr(kjs, C.c_int), C.byref(k0))
+    assert rc == 0
+    return M, a, b, perm.astype(np.int64), kjs[:rank].astype(np.int64), k0.value
 
 
 def cholesky_classic(M, a, b, bs1=2 ** 7, bs2=2 ** 9, bs3=2 ** 10, bs4=2 ** 11) -> CholeskyResult:

@@ -147,3 +147,34 @@ def test_rowmax_rank_stop(dbg, oracle):
     assert np.array_equal(Fg.perm[:5], Fo.perm[:5])
     assert sorted(int(p) // 2 for p in Fg.perm[:5]) == list(range(5))
     assert np.abs(np.triu(Fg.U)[:5, :5] - np.triu(Fo.U)[:5, :5]).max() < 1e-12
+
+
+@pytest.mark.parametrize("n,rank,seed,zeros", [(30, 20, 1, (24,)), (64, 24, 2, (30, 40, 63)),
+                                               (200, 120, 3, (130, 199)), (700, 300, 4, (305, 350, 699)),
+                                               (40, 40, 5, ()), (16, 0, 6, tuple(range(16)))])
+def test_adapt_low_rank_matches_oracle(dbg, oracle, n, rank, seed, zeros):
+    """adapt_low_rank! (src/cholesky.jl:20-123) on the device against the
+    oracle: the same column moves and scalings, bit for bit (the factor, a, b,
+    perm, k_js and k_0)."""
+    from tests.test_oracle import _low_rank_case
+
+    print(f"adapt_low_rank n={n} rank={rank} seed={seed}", flush=True)
+    M, a, b = _low_rank_case(n, rank, seed, zeros)
+    Mo, ao, bo, po, ko, k0o = oracle.adapt_low_rank(M, a, b, rank, np.arange(n), 1e-10)
+    Mg, ag, bg, pg = M.copy(order="F"), a.copy(), b.copy(), np.arange(n, dtype=np.int32)
+    kg, k0g = dbg.adapt_low_rank(Mg, ag, bg, rank, pg, 1e-10)
+    assert k0g == k0o and np.array_equal(kg, ko) and np.array_equal(pg, po)
+    assert np.array_equal(Mg, Mo) and np.array_equal(ag, ao) and np.array_equal(bg, bo)
+
+
+def test_adapt_low_rank_reference_test(dbg):
+    """test/runtests.jl:61-90 on the device."""
+    n, rank = 30, 20
+    rng = np.random.default_rng(61)
+    M = np.zeros((n, n), order="F")
+    M[np.arange(rank), np.arange(rank)] = 1.0
+    M[:rank, rank:] = rng.random((rank, n - rank))
+    M[:, 24] = 0.0
+    perm = np.arange(n, dtype=np.int32)
+    k_js, k_0 = dbg.adapt_low_rank(M, np.zeros(n), np.ones(n), rank, perm, 1e-10)
+    assert k_0 == 1 and perm[-1] == 24 and np.all(M[:, -1] == 0.0) and len(k_js) == rank

@@ -275,6 +275,74 @@ def test_rowmax_exactly_singular(oracle):
     assert np.all(np.diag(F.U)[:5] > 0.1) and np.all(np.abs(np.diag(F.U)[5:F.rank]) < 1e-6)
 
 
+def _low_rank_case(n, rank, seed, zero_cols=(), neg=True):
+    """A rank-deficient factor in the reference test's shape
+    (test/runtests.jl:61-90): unit columns for the independent variables,
+    dependent columns with random entries above row rank whose last nonzero
+    row (the level) varies, some columns zero."""
+    rng = np.random.default_rng(seed)
+    M = np.zeros((n, n))
+    M[np.arange(rank), np.arange(rank)] = 1.0
+    for j in range(rank if rank > 0 else n, n):
+        lvl = int(rng.integers(0, rank))
+        M[:lvl + 1, j] = rng.uniform(0.1, 1.0, lvl + 1) * (rng.choice([-1.0, 1.0], lvl + 1) if neg else 1.0)
+    for c in zero_cols:
+        M[:, c] = 0.0
+    a = -rng.uniform(0.5, 2.0, n)
+    b = rng.uniform(0.5, 2.0, n)
+    return M, a, b
+
+
+def _level(col, rank, eps1):
+    nz = np.nonzero(np.abs(col[:rank]) > eps1)[0]
+    return int(nz[-1]) if len(nz) else -1
+
+
+def test_adapt_low_rank_reference_test(oracle):
+    """test/runtests.jl:61-90: n = 30, rank 20, column 25 (1-based) zero."""
+    n, rank = 30, 20
+    rng = np.random.default_rng(61)
+    M = np.zeros((n, n))
+    M[np.arange(rank), np.arange(rank)] = 1.0
+    M[:rank, rank:] = rng.random((rank, n - rank))
+    M[:, 24] = 0.0
+    Mo, ao, bo, perm, kjs, k0 = oracle.adapt_low_rank(M, np.zeros(n), np.ones(n), rank, np.arange(n), 1e-10)
+    assert k0 == 1 and perm[-1] == 24 and np.all(Mo[:, -1] == 0.0) and len(kjs) == rank
+
+
+@pytest.mark.parametrize("n,rank,seed,zeros", [(30, 20, 1, (24,)), (64, 24, 2, (30, 40, 63)),
+                                               (200, 120, 3, (130, 199)), (50, 0, 4, tuple(range(50)))])
+def test_adapt_low_rank_invariants(oracle, n, rank, seed, zeros):
+    """What adapt_low_rank! guarantees, checked on its result: every output
+    column is an input column (perm), scaled by 1/m on rows up to its level m;
+    the levels are sorted (zero columns last, k_0 of them); each dependent
+    column's level entry is 1 and its bounds stay ordered (exchanged when m was
+    negative); k_js[i] is the last column of level i."""
+    eps1 = 1e-10
+    M, a, b = _low_rank_case(n, rank, seed, zeros)
+    Mo, ao, bo, perm, kjs, k0 = oracle.adapt_low_rank(M, a, b, rank, np.arange(n), eps1)
+    assert sorted(perm) == list(range(n)) and k0 == len(zeros)
+    lv = [_level(Mo[:, c], rank, eps1) for c in range(n)]
+    active = n - k0
+    assert all(v == -1 for v in lv[active:]) and all(v >= 0 for v in lv[:active]) or rank == 0
+    assert lv[:active] == sorted(lv[:active])
+    for c in range(n):
+        p = int(perm[c])
+        src = M[:, p]
+        m = src[lv[c]] if lv[c] >= 0 and p >= rank else 1.0
+        exp = src.copy()
+        if p >= rank and lv[c] >= 0:
+            exp[:lv[c] + 1] = src[:lv[c] + 1] * (1.0 / m)
+            assert abs(Mo[lv[c], c] - 1.0) <= 2 ** -52
+            lo, hi = sorted((a[p] * (1.0 / m), b[p] * (1.0 / m)))
+            assert ao[c] == lo and bo[c] == hi
+        else:
+            assert ao[c] == a[p] and bo[c] == b[p]
+        assert np.array_equal(Mo[:, c], exp)
+    for i in range(rank):
+        assert lv[kjs[i]] == i and (kjs[i] + 1 >= active or lv[kjs[i] + 1] > i)
+
+
 # --- integration -------------------------------------------------------------------
 def _run(oracle, Sigma, a, b, *, m, R, seed=0, phi="erf", kind="dense", gen="Richtmyer", **kw):
     n = Sigma.shape[0]
