@@ -7,8 +7,8 @@ cholesky_rowmax!, adapt_low_rank!, cholesky_classic!, SparseQMC); all numerical 
 (HIP, gfx950) behind the C ABI of include/dbgenz.h.
 """
 from .qmc import (CholeskyGenz, QMC_opts, QMCData, QMCData_Distributions, QMCDataSparse,
-                  QMCOpts, SparseQMC, adapt_low_rank, cholesky_classic, cholesky_genz, cholesky_rowmax, draw_shifts, qmc_pnorm)
+                  QMCOpts, SparseQMC, adapt_low_rank, cholesky_classic, covariance, cholesky_genz, cholesky_rowmax, draw_shifts, qmc_pnorm)
 
-__all__ = ["CholeskyGenz", "QMC_opts", "adapt_low_rank", "QMCData", "QMCData_Distributions", "QMCDataSparse",
+__all__ = ["CholeskyGenz", "QMC_opts", "adapt_low_rank", "covariance", "QMCData", "QMCData_Distributions", "QMCDataSparse",
            "QMCOpts", "SparseQMC", "cholesky_classic", "cholesky_genz", "cholesky_rowmax", "draw_shifts",
            "qmc_pnorm"]

@@ -23,11 +23,12 @@ GEN_RICHTMYER, GEN_SOBOL = 0, 1
 PHI_ERF, PHI_ERFC, PHI_TAIL = 0, 1, 2
 KIND_DENSE, KIND_SPARSE, KIND_SPARSEQMC = 0, 1, 2
 CONTRACT_AUTO, CONTRACT_FP32, CONTRACT_SPLIT16 = 0, 1, 2
+COV_FIXED_DENSE, COV_FIXED_SPARSE, COV_MATERN, COV_LOWRANK = 0, 1, 2, 3
 
 # every exported symbol of include/dbgenz.h
 EXPORTS = (
     "dbg_last_error", "dbg_abi_version", "dbg_opts_default", "dbg_chol_genz_f64", "dbg_chol_rowmax_f64",
-    "dbg_adapt_low_rank_f64",
+    "dbg_adapt_low_rank_f64", "dbg_cov_build",
     "dbg_chol_classic_f64", "dbg_plan_create", "dbg_plan_destroy", "dbg_plan_set_shifts",
     "dbg_plan_set_sobol_directions", "dbg_qmc_pnorm", "dbg_qmc_loop", "dbg_plan_sync",
     "dbg_plan_last_kernel_ms", "dbg_plan_summary", "dbg_plan_get_factor", "dbg_plan_get_reps",
@@ -83,6 +84,7 @@ def lib():
     L.dbg_chol_genz_f64.argtypes = chol + [ip32, dp, ip32]
     L.dbg_chol_rowmax_f64.argtypes = chol + [ip32, dp, ip32]
     L.dbg_adapt_low_rank_f64.argtypes = [dp, C.c_int64, C.c_int32, dp, dp, C.c_int32, ip32, C.c_double, ip32, ip32]
+    L.dbg_cov_build.argtypes = [C.c_int32, C.c_int32, dp, C.c_int32, vp, vp, C.c_int32, vp, C.c_int64, C.c_int32]
     L.dbg_chol_classic_f64.argtypes = chol + [dp]
     L.dbg_plan_create.argtypes = [C.POINTER(vp), C.POINTER(Opts), C.c_int32, dp, C.c_int64, dp, dp]
     L.dbg_plan_destroy.argtypes = [vp]

@@ -359,8 +359,9 @@ int plan_build(dbg_plan* P, const double* Sigma, int64_t lda, const double* a, c
     HIPCHK(dalloc(&P->sig, (size_t)n * n * sizeof(double)));
     HIPCHK(dalloc(&P->a0, n * sizeof(double)));
     HIPCHK(dalloc(&P->b0, n * sizeof(double)));
+    // Sigma in host or device memory (a covariance built by dbg_cov_build stays on the device)
     HIPCHK(hipMemcpy2DAsync(P->sig, (size_t)n * sizeof(double), Sigma, (size_t)lda * sizeof(double),
-                            (size_t)n * sizeof(double), n, hipMemcpyHostToDevice, s));
+                            (size_t)n * sizeof(double), n, hipMemcpyDefault, s));
     HIPCHK(hipMemcpyAsync(P->a0, a, n * sizeof(double), hipMemcpyHostToDevice, s));
     P->a_inf = true;
     for (int i = 0; i < n; ++i) P->a_inf = P->a_inf && std::isinf(a[i]) && a[i] < 0;
@@ -771,6 +772,56 @@ int dbg_adapt_low_rank_f64(double* M, int64_t lda, int32_t n, double* a, double*
         (void)hipFree(p);
     (void)hipStreamDestroy(s);
     if (e != hipSuccess) return fail(DBG_ERR_DEVICE, std::string("adapt_low_rank: ") + hipGetErrorString(e));
+    return DBG_OK;
+}
+
+int dbg_cov_build(int32_t kind, int32_t n, const double* params, int32_t k, const double* B, const double* psi,
+                  int32_t standardize, double* Sigma, int64_t lds, int32_t sigma_on_device)
+{
+    g_err.clear();
+    if (n < 1 || lds < n || !Sigma || kind < DBG_COV_FIXED_DENSE || kind > DBG_COV_LOWRANK)
+        return fail(DBG_ERR_ARG, "invalid covariance arguments");
+    if (kind != DBG_COV_LOWRANK && !params) return fail(DBG_ERR_ARG, "covariance parameters missing");
+    if (kind == DBG_COV_FIXED_SPARSE && !(params[0] >= 0.0 && params[0] == std::floor(params[0])))
+        return fail(DBG_ERR_ARG, "fixed_sparse: the bandwidth n_sp must be a non-negative integer");
+    if (kind == DBG_COV_MATERN &&
+        (!(params[0] == 0.5 || params[0] == 1.5 || params[0] == 2.5) || !(params[1] > 0.0)))
+        return fail(DBG_ERR_ARG, "Matern: nu must be 0.5, 1.5 or 2.5 and dist positive");
+    if (kind == DBG_COV_LOWRANK && (k < 1 || !B || !psi)) return fail(DBG_ERR_ARG, "low-rank: B (n x k) and psi needed");
+    hipStream_t s;
+    HIPCHK(hipStreamCreate(&s));
+    double *dS = nullptr, *dB = nullptr, *dpsi = nullptr, *dscr = nullptr;
+    const int64_t ld = sigma_on_device ? lds : n;
+    hipError_t e = hipSuccess;
+    if (sigma_on_device) dS = Sigma;
+    else e = dalloc(&dS, (size_t)n * n * sizeof(double));
+    if (e == hipSuccess && standardize) e = dalloc(&dscr, (size_t)n * sizeof(double));
+    int rc = DBG_OK;
+    if (e == hipSuccess && kind == DBG_COV_LOWRANK) {
+        e = dalloc(&dB, (size_t)n * k * sizeof(double));
+        if (e == hipSuccess) e = dalloc(&dpsi, (size_t)n * sizeof(double));
+        if (e == hipSuccess) e = hipMemcpyAsync(dB, B, (size_t)n * k * sizeof(double), hipMemcpyDefault, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(dpsi, psi, (size_t)n * sizeof(double), hipMemcpyDefault, s);
+        if (e == hipSuccess && cov_lowrank(n, k, dB, dpsi, dS, ld, s) != 0)
+            rc = fail(DBG_ERR_DEVICE, "covariance: rocBLAS dsyrk failed");
+    } else if (e == hipSuccess) {
+        const int kk = kind == DBG_COV_FIXED_DENSE ? kCovFixedDense
+                                                   : (kind == DBG_COV_FIXED_SPARSE ? kCovFixedSparse : kCovMatern);
+        e = cov_fill(kk, n, dS, ld, params[0], kind == DBG_COV_MATERN ? params[1] : 0.0,
+                     kind == DBG_COV_MATERN ? params[2] : 0.0, s);
+    }
+    if (e == hipSuccess && !rc && standardize) e = cov_standardize(n, dS, ld, dscr, s);
+    if (e == hipSuccess && !rc && !sigma_on_device)
+        e = hipMemcpy2DAsync(Sigma, (size_t)lds * sizeof(double), dS, (size_t)n * sizeof(double),
+                             (size_t)n * sizeof(double), n, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (!sigma_on_device) (void)hipFree(dS);
+    (void)hipFree(dB);
+    (void)hipFree(dpsi);
+    (void)hipFree(dscr);
+    (void)hipStreamDestroy(s);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(DBG_ERR_DEVICE, std::string("covariance: ") + hipGetErrorString(e));
     return DBG_OK;
 }
 

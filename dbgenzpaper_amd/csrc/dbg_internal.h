@@ -203,6 +203,11 @@ hipError_t chol_standardize(const CholArgs& C, bool mirror, hipStream_t s);
 hipError_t chol_window(const CholArgs& C, int w0, int w1, int nb, int pivot, hipStream_t s);
 hipError_t chol_trsm_syrk(const CholArgs& C, int w0, int w1, hipStream_t s);
 hipError_t chol_finish(const CholArgs& C, hipStream_t s);
+// covariance generators (cov.hip); kinds as DBG_COV_* of the C ABI
+constexpr int kCovFixedDense = 0, kCovFixedSparse = 1, kCovMatern = 2, kCovLowRank = 3;
+hipError_t cov_fill(int kind, int n, double* S, int64_t ld, double p0, double p1, double p2, hipStream_t s);
+int cov_lowrank(int n, int k, const double* B, const double* psi, double* S, int64_t ld, hipStream_t s);
+hipError_t cov_standardize(int n, double* S, int64_t ld, double* scratch, hipStream_t s);
 // adapt_low_rank! on device arrays (M n x n, lda; end: rank entries, out; tmp:
 // n * n doubles of scratch; k0: one entry, out)
 hipError_t adapt_low_rank(double* M, int64_t lda, int n, double* a, double* b, int32_t* perm, int rank, double eps1,

@@ -171,6 +171,38 @@ def adapt_low_rank(M, a, b, rank, perm, eps1):
     return kjs[:rank].astype(np.int64), int(k0.value)
 
 
+def covariance(kind, n, *, jitter=1.0, n_sp=1, nu=1.5, dist=1.0, sigma2=1.0, B=None, psi=None,
+               standardize=False) -> np.ndarray:
+    """The covariance generators of the reference's simulation scripts
+    (scripts/simulations/simulations_functions.jl), built on the GPU and
+    returned as a column-major float64 array:
+      "fixed_dense"   ones(n, n) + jitter I               (:21-24)
+      "fixed_sparse"  fixed_sparse(n, n_sp)               (:33-41)
+      "matern"        mattern_cov(n; nu, dist, sigma2)    (:56-79), nu in {0.5, 1.5, 2.5}
+      "lowrank"       B B' + diag(psi): the factor model; rand_spd(n; jitter) (:13-18)
+                      with B = A (n x n, drawn by the caller) and psi = jitter * n
+    standardize rescales to a correlation matrix.  (The C entry point
+    dbg_cov_build can also leave the matrix in device memory for
+    dbg_plan_create.)"""
+    kinds = {"fixed_dense": L.COV_FIXED_DENSE, "fixed_sparse": L.COV_FIXED_SPARSE, "matern": L.COV_MATERN,
+             "lowrank": L.COV_LOWRANK}
+    if kind not in kinds:
+        raise ValueError(f"unknown covariance kind {kind!r}")
+    S = np.zeros((n, n), np.float64, order="F")
+    params = np.array({"fixed_dense": [jitter, 0, 0], "fixed_sparse": [n_sp, 0, 0], "matern": [nu, dist, sigma2],
+                       "lowrank": [0, 0, 0]}[kind], np.float64)
+    k, Bp, pp = 0, None, None
+    if kind == "lowrank":
+        Bm = np.array(B, np.float64, order="F", copy=True)
+        if Bm.ndim != 2 or Bm.shape[0] != n:
+            raise ValueError("B must be n x k")
+        pv = np.array(psi, np.float64, copy=True).reshape(n)
+        k, Bp, pp = Bm.shape[1], Bm.ctypes.data, pv.ctypes.data
+    L.check(L.lib().dbg_cov_build(kinds[kind], n, L.ptr(params, C.c_double), k, Bp, pp, int(bool(standardize)),
+                                  S.ctypes.data, n, 0))
+    return S
+
+
 def cholesky_classic(M, a, b, block_size=2 ** 7, block_size2=2 ** 9, block_size3=2 ** 10,
                      block_size4=2 ** 11) -> CholeskyGenz:
     """cholesky_classic!(M, a, b, bs1, bs2, bs3, bs4)  src/cholesky.jl:144-242 (GPU, FP64)."""

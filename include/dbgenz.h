@@ -139,6 +139,27 @@ int dbg_chol_rowmax_f64(double* M, int64_t lda, int32_t n, double* a, double* b,
 int dbg_adapt_low_rank_f64(double* M, int64_t lda, int32_t n, double* a, double* b, int32_t rank, int32_t* perm,
                            double eps1, int32_t* k_js, int32_t* k_0);
 
+/* Covariance generators of the reference's simulation scripts
+ * (scripts/simulations/simulations_functions.jl), built on the device:
+ *   DBG_COV_FIXED_DENSE   fixed_dense(n; jitter)   :21-24   params = {jitter}
+ *   DBG_COV_FIXED_SPARSE  fixed_sparse(n, n_sp)    :33-41   params = {n_sp}
+ *   DBG_COV_MATERN        mattern_cov(n; nu, dist, sigma2)  :56-79
+ *                         params = {nu, dist, sigma2}, nu in {0.5, 1.5, 2.5} (closed forms; the
+ *                         reference's presets mattern_cov0/1/2 use nu = 1.5)
+ *   DBG_COV_LOWRANK       B B' + diag(psi), B n x k column-major, psi n: the factor model, and
+ *                         rand_spd(n; jitter) :13-18 with k = n, psi = jitter * n (A drawn by
+ *                         the caller, whose RNG stream the library cannot reproduce)
+ * standardize != 0 rescales to a correlation matrix (S_ij / (s_i s_j), ones on the
+ * diagonal).  Sigma (n x n, lds) is device memory when sigma_on_device != 0 (it can
+ * then go straight to dbg_plan_create), else host memory.  B and psi may be host or
+ * device memory.  No reference counterpart as a library call. */
+#define DBG_COV_FIXED_DENSE 0
+#define DBG_COV_FIXED_SPARSE 1
+#define DBG_COV_MATERN 2
+#define DBG_COV_LOWRANK 3
+int dbg_cov_build(int32_t kind, int32_t n, const double* params, int32_t k, const double* B, const double* psi,
+                  int32_t standardize, double* Sigma, int64_t lds, int32_t sigma_on_device);
+
 /* cholesky_classic!(M, a, b, bs1, bs2, bs3, bs4)  src/cholesky.jl:144-242
  * Unpivoted, in place on host arrays, FP64 on the device. */
 int dbg_chol_classic_f64(double* M, int64_t lda, int32_t n, double* a, double* b, int32_t bs1,
@@ -147,7 +168,8 @@ int dbg_chol_classic_f64(double* M, int64_t lda, int32_t n, double* a, double* b
 /* QMCData(C, a, b, opts, rng, qmc_type)        src/qmc.jl:149-202
  * QMCDataSparse(C, a, b, opts, rng, qmc_type)  src/qmc.jl:225-282
  * SparseQMC.QMCDataSparse(C, a, b; opts)       src/qmc_sparse.jl:58-90
- * Copies Sigma (n x n, lda), a and b to the device, factors (pivoted for
+ * Copies Sigma (n x n, lda; host or device memory), a and b (host) to the
+ * device, factors (pivoted for
  * DBG_KIND_DENSE, unpivoted otherwise), applies truncate_matrixU! and builds
  * the packed / CSC layouts.  The random shifts are set separately (the RNG
  * stays with the caller).  Fails with DBG_ERR_RANK for a rank-deficient

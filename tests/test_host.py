@@ -215,3 +215,37 @@ def test_plan_option_validation(dbg):
     assert create(_lib.PREC_MIXED, n_bits=0) == _lib.DBG_ERR_ARG
     assert create(_lib.PREC_MIXED, m=0) == _lib.DBG_ERR_ARG
     assert not h.value
+
+
+def test_cov_build_validation(dbg):
+    """dbg_cov_build rejects invalid generators before touching the device."""
+    from dbgenzpaper_amd import _lib
+
+    L = _lib.lib()
+    n = 8
+    S = np.zeros((n, n), order="F")
+
+    def build(kind, params, k=0, B=None, psi=None, lds=n):
+        p = np.array(params, np.float64)
+        return L.dbg_cov_build(kind, n, _lib.ptr(p, ctypes.c_double), k, B, psi, 0, S.ctypes.data, lds, 0)
+
+    assert build(7, [1, 0, 0]) == _lib.DBG_ERR_ARG
+    assert build(_lib.COV_MATERN, [1.0, 1.0, 1.0]) == _lib.DBG_ERR_ARG      # nu not half-integer
+    assert b"nu must be" in L.dbg_last_error()
+    assert build(_lib.COV_MATERN, [1.5, 0.0, 1.0]) == _lib.DBG_ERR_ARG      # dist = 0
+    assert build(_lib.COV_FIXED_SPARSE, [2.5, 0, 0]) == _lib.DBG_ERR_ARG    # fractional bandwidth
+    assert build(_lib.COV_LOWRANK, [0, 0, 0]) == _lib.DBG_ERR_ARG           # B and psi missing
+    assert build(_lib.COV_FIXED_DENSE, [1, 0, 0], lds=n - 1) == _lib.DBG_ERR_ARG
+
+
+@pytest.mark.parametrize("nu", [0.5, 1.5, 2.5])
+def test_matern_closed_forms(nu):
+    """The half-integer closed forms the device generator evaluates equal the
+    reference's general formula sigma2 2^(1-nu)/Gamma(nu) z^nu K_nu(z)
+    (scripts/simulations/simulations_functions.jl:56-79) through SciPy's kv."""
+    from scipy.special import gamma, kv
+
+    z = np.concatenate([np.linspace(1e-3, 5, 400), np.linspace(5, 40, 200)])
+    general = 2 ** (1 - nu) / gamma(nu) * z ** nu * kv(nu, z)
+    closed = {0.5: np.exp(-z), 1.5: (1 + z) * np.exp(-z), 2.5: (1 + z + z * z / 3) * np.exp(-z)}[nu]
+    assert np.allclose(closed, general, rtol=1e-13, atol=0)
