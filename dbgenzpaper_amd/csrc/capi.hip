@@ -855,6 +855,10 @@ int dbg_plan_create(dbg_plan** plan, const dbg_opts* opts, int32_t n, const doub
         return fail(DBG_ERR_ARG, "unknown contraction");
     if (o.contraction == DBG_CONTRACT_SPLIT16 && (o.precision != DBG_PREC_MIXED || o.kind != DBG_KIND_DENSE))
         return fail(DBG_ERR_ARG, "the split-FP16 contraction is built for the mixed dense path");
+    // NaN bounds would give the reference a NaN estimate; they are refused here (the
+    // FP32 tail mass clamps its argument with a NaN-dropping min)
+    for (int i = 0; i < n; ++i)
+        if (std::isnan(a[i]) || std::isnan(b[i])) return fail(DBG_ERR_ARG, "the bounds a and b must not be NaN");
     dbg_plan* P = new dbg_plan();
     P->o = o;
     P->n = n;

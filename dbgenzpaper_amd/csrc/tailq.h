@@ -60,7 +60,8 @@ DBG_HD inline float horner_f32(float x, const float (&c)[N])
 // Upper normal tail mass erfc(z)/2 (= 1 - Phi(z sqrt2)) for z >= 0 in FP32,
 // returned as m 2^-k with m in [0.02, 0.71] and k >= 0, so that it stays
 // representable far past the FP32 range.  z is capped at 30 (a mass near
-// 2^-1300, beyond the FP64 range too); the caller handles infinite z.
+// 2^-1300, beyond the FP64 range too); the caller handles infinite z, and z is
+// never NaN (finite factor, bounds checked for NaN when the plan is made).
 //
 // Method: erfc(z) = t exp(-z^2 + P(t)), t = 2 / (2 + z) (the form of
 // Numerical Recipes' erfcc, Press et al. 1992, sec. 6.2), with P a degree-11
@@ -76,7 +77,7 @@ DBG_HD inline float horner_f32(float x, const float (&c)[N])
 // reciprocal and exp2, 5e-7 with the device's 1-ulp v_rcp_f32 and v_exp_f32.
 DBG_HD inline float tail_mass_f32(float z, int& k)
 {
-    z = fminf(z, 30.0f);
+    z = fminf(z, 30.0f);  // one v_min_f32; NaN bounds are rejected where a plan is created
     const float s = 2.0f + z;
     const float h = rcp_f32(s);
     const float w = 2.0f - z;

@@ -173,7 +173,8 @@ int dbg_chol_classic_f64(double* M, int64_t lda, int32_t n, double* a, double* b
  * DBG_KIND_DENSE, unpivoted otherwise), applies truncate_matrixU! and builds
  * the packed / CSC layouts.  The random shifts are set separately (the RNG
  * stays with the caller).  Fails with DBG_ERR_RANK for a rank-deficient
- * pivoted factor. */
+ * pivoted factor and with DBG_ERR_ARG for NaN bounds (where the reference
+ * would return a NaN estimate). */
 int dbg_plan_create(dbg_plan** plan, const dbg_opts* opts, int32_t n, const double* Sigma,
                     int64_t lda, const double* a, const double* b);
 int dbg_plan_destroy(dbg_plan* plan);

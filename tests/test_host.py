@@ -213,6 +213,9 @@ def test_plan_option_validation(dbg):
     assert b"split-FP16" in L.dbg_last_error()
     assert create(_lib.PREC_MIXED, contraction=_lib.CONTRACT_SPLIT16, kind=_lib.KIND_SPARSE) == _lib.DBG_ERR_ARG
     assert create(_lib.PREC_MIXED, n_bits=0) == _lib.DBG_ERR_ARG
+    b[3] = np.nan
+    assert create(_lib.PREC_MIXED) == _lib.DBG_ERR_ARG and b"NaN" in L.dbg_last_error()
+    b[3] = 1.0
     assert create(_lib.PREC_MIXED, m=0) == _lib.DBG_ERR_ARG
     assert not h.value
 
