@@ -178,3 +178,43 @@ def test_adapt_low_rank_reference_test(dbg):
     perm = np.arange(n, dtype=np.int32)
     k_js, k_0 = dbg.adapt_low_rank(M, np.zeros(n), np.ones(n), rank, perm, 1e-10)
     assert k_0 == 1 and perm[-1] == 24 and np.all(M[:, -1] == 0.0) and len(k_js) == rank
+
+
+def test_leading_dimension_padding(dbg, oracle):
+    """dbg_chol_rowmax_f64 and dbg_adapt_low_rank_f64 on a matrix stored with
+    lda > n: the n x n block equals the unpadded results, the padding rows are
+    untouched."""
+    import ctypes
+
+    from dbgenzpaper_amd import _lib
+    from tests.test_oracle import _low_rank_case
+
+    L = _lib.lib()
+    n, pad = 90, 7
+    C = random_spd(n, 77)
+    a, b = -np.ones(n), np.ones(n)
+    W = np.full((n + pad, n), 123.0, order="F")
+    W[:n, :] = C
+    av, bv = a.copy(), b.copy()
+    perm = np.zeros(n, np.int32)
+    scale = np.zeros(n)
+    rank = ctypes.c_int32()
+    dp = lambda x: _lib.ptr(x, ctypes.c_double)  # noqa: E731
+    _lib.check(L.dbg_chol_rowmax_f64(dp(W), n + pad, n, dp(av), dp(bv), 16, 64, 128, 256,
+                                     _lib.ptr(perm, ctypes.c_int32), dp(scale), ctypes.byref(rank)))
+    F = dbg.cholesky_rowmax(C.copy(order="F"), a.copy(), b.copy(), 16, 64, 128, 256)
+    assert rank.value == F.rank == n and np.array_equal(perm, F.perm)
+    assert np.array_equal(W[:n, :], F.U) and np.all(W[n:, :] == 123.0)
+
+    M, a2, b2 = _low_rank_case(n, 40, 8, (50, 89))
+    Wm = np.full((n + pad, n), -5.0, order="F")
+    Wm[:n, :] = M
+    pv = np.arange(n, dtype=np.int32)
+    kjs = np.zeros(40, np.int32)
+    k0 = ctypes.c_int32()
+    a3, b3 = a2.copy(), b2.copy()
+    _lib.check(L.dbg_adapt_low_rank_f64(dp(Wm), n + pad, n, dp(a3), dp(b3), 40, _lib.ptr(pv, ctypes.c_int32), 1e-10,
+                                        _lib.ptr(kjs, ctypes.c_int32), ctypes.byref(k0)))
+    Mo, ao, bo, po, ko, k0o = oracle.adapt_low_rank(M, a2, b2, 40, np.arange(n), 1e-10)
+    assert np.array_equal(Wm[:n, :], Mo) and np.all(Wm[n:, :] == -5.0)
+    assert np.array_equal(pv, po) and np.array_equal(kjs, ko) and k0.value == k0o

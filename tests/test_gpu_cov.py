@@ -111,3 +111,18 @@ def test_plan_from_device_covariance(dbg):
         assert vd == vh and 0 < vd[0] < 1
     finally:
         hip.hipFree(dptr)
+
+
+def test_leading_dimension_padding(dbg):
+    """dbg_cov_build into host memory with lds > n: the n x n block equals the
+    unpadded matrix, the padding rows are untouched."""
+    from dbgenzpaper_amd import _lib
+
+    L = _lib.lib()
+    n, pad = 64, 5
+    S = np.full((n + pad, n), 9.0, order="F")
+    params = np.array([1.5, 4.0, 2.0])
+    _lib.check(L.dbg_cov_build(_lib.COV_MATERN, n, _lib.ptr(params, ctypes.c_double), 0, None, None, 0,
+                               S.ctypes.data, n + pad, 0))
+    assert np.array_equal(S[:n, :], dbg.covariance("matern", n, nu=1.5, dist=4.0, sigma2=2.0))
+    assert np.all(S[n:, :] == 9.0)
