@@ -18,11 +18,17 @@ namespace dbg {
 constexpr int kTileB = 256;
 int batch_tile() { return kTileB; }
 
-__global__ void fill_u64_kernel(unsigned long long* p, unsigned long long v) { *p = v; }
-
-hipError_t fill_u64(unsigned long long* p, unsigned long long v, hipStream_t s)
+// queue head and status word back to zero (a kernel, not a memset node, so
+// that the step can be captured in a graph)
+__global__ void queue_reset_kernel(unsigned long long* head, int* status)
 {
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(1), 0, s, p, v);
+    *head = 0ull;
+    *status = 0;
+}
+
+hipError_t queue_reset(const BatchArgs& B, hipStream_t s)
+{
+    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(1), 0, s, B.counter, B.status);
     return hipGetLastError();
 }
 
@@ -40,12 +46,26 @@ __global__ __launch_bounds__(kTileB) void batch_integrate_kernel(BatchArgs B, co
     const int tid = threadIdx.x;
     T* __restrict__ Ys = Yall + (size_t)blockIdx.x * (size_t)B.y_slot;
 
-    for (;;) {
-        if (tid == 0) s_item = (int64_t)atomicAdd(B.counter, 1ull);
-        __syncthreads();
-        const int64_t it = s_item;
-        __syncthreads();
-        if (it >= B.n_items) break;  // uniform: every wave leaves together
+    // Work-queue loop.  Thread 0 takes the next item in ONE region at the end
+    // of the body (after storing the item's partial sum), closed by a barrier
+    // before the whole workgroup reads the item and tests the loop condition.
+    // A loop written with a thread-0 fetch at the top and a thread-0 store at
+    // the bottom let the compiler join the two regions across the back-edge
+    // into a nested loop that only thread 0 left after an item: the other
+    // threads re-read the old item forever.  Each fetched item is strictly
+    // larger than the previous one; a repeat stops the workgroup and is
+    // reported through the status word instead of spinning.
+    if (tid == 0) s_item = (int64_t)atomicAdd(B.counter, 1ull);
+    __syncthreads();
+    int64_t it = s_item;
+    int64_t prev = -1;
+    while (it < B.n_items) {  // uniform: it comes from one LDS word after a barrier
+        if (it <= prev) {
+            if (tid == 0) atomicOr(B.status, 1);
+            break;
+        }
+        prev = it;
+        __syncthreads();  // every thread has read s_item before thread 0 rewrites it
         const int2 wi = B.items[it];
         const int p = wi.x;
         const int lt = wi.y;
@@ -94,7 +114,10 @@ __global__ __launch_bounds__(kTileB) void batch_integrate_kernel(BatchArgs B, co
         if (tid == 0) {
             part_m[P.part_off + lt] = S;
             part_e[P.part_off + lt] = E;
+            s_item = (int64_t)atomicAdd(B.counter, 1ull);
         }
+        __syncthreads();
+        it = s_item;
     }
 }
 

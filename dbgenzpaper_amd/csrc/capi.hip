@@ -1121,6 +1121,56 @@ struct dbg_batch {
 
 namespace {
 
+// The queue loop's status word after a launch has finished (nonzero: a
+// workgroup was handed an item it had already taken, so it stopped).
+int batch_status(dbg_batch* Bt)
+{
+    int st = 0;
+    HIPCHK(hipMemcpyAsync(&st, Bt->B.status, sizeof(int), hipMemcpyDeviceToHost, Bt->stream));
+    HIPCHK(hipStreamSynchronize(Bt->stream));
+    if (st != 0) return fail(DBG_ERR_DEVICE, "batched work queue: a workgroup fetched an item twice");
+    return DBG_OK;
+}
+
+// Extents of the batch's device buffers (elements), for the host-side check
+// of every offset the kernels derive from the problem table and the items.
+struct BatchExtents {
+    int64_t sig, vec, uk, sh, part, y_slot, nq;
+    int K, R, dmax, tile;
+    int64_t tpr;
+};
+
+// Every access the batched kernels make, bounded on the host before upload:
+// the factor work matrix and its column-major copy (d x d at sig_off), the
+// packed rows (d - 1 rows of ldu; the contraction reads W <= 32 entries from
+// column g0 + 1 with g0 <= d - 2, so ldu >= d + 31), the per-index vectors
+// (d at vec_off), the shifts ((d - 1) x R at shift_off), the tile partials
+// (tpr x R at part_off), the per-workgroup y slot ((d - 1) x tile) and the
+// lattice table (d - 1 entries); each item names a problem and a tile of it.
+int batch_validate(const std::vector<BatchProblem>& probs, const std::vector<int2>& items,
+                   const std::vector<int2>& sample, const BatchExtents& X)
+{
+    auto bad = [](const std::string& what) { return fail(DBG_ERR_ARG, "batched layout: " + what); };
+    for (int k = 0; k < X.K; ++k) {
+        const BatchProblem& P = probs[k];
+        const int64_t d = P.d, nd = std::max<int64_t>(d - 1, 0);
+        if (d < 1 || d > kBatchMaxDim || d > X.dmax) return bad("dimension out of range");
+        if (P.sig_off < 0 || P.sig_off + d * d > X.sig || P.ucol_off != P.sig_off) return bad("factor extent");
+        if (P.vec_off < 0 || P.vec_off + d > X.vec) return bad("vector extent");
+        if (P.ldu < d + 31) return bad("packed row stride");
+        if (P.uk_off < 0 || P.uk_off + nd * P.ldu > X.uk) return bad("packed rows extent");
+        if (P.shift_off < 0 || P.shift_off + nd * X.R > X.sh) return bad("shift extent");
+        if (P.tiles_per_rep != X.tpr || P.part_off < 0 || P.part_off + X.tpr * X.R > X.part)
+            return bad("partials extent");
+        if (nd * X.tile > X.y_slot || nd > X.nq) return bad("workspace extent");
+    }
+    for (const std::vector<int2>* v : {&items, &sample})
+        for (const int2& it : *v)
+            if (it.x < 0 || it.x >= X.K || it.y < 0 || (int64_t)it.y >= X.tpr * X.R) return bad("work item");
+    if ((int64_t)items.size() != (int64_t)X.K * X.tpr * X.R) return bad("item count");
+    return DBG_OK;
+}
+
 int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const double* a,
                 const double* b, const double* shifts)
 {
@@ -1168,6 +1218,23 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
         for (int64_t lt = 0; lt < tpr * R; ++lt) items.push_back(make_int2(k, (int)lt));
     std::vector<int2> sample;
     for (size_t i = 0; i < items.size(); i += 16) sample.push_back(items[i]);
+    {
+        BatchExtents X{};
+        X.sig = sig;
+        X.vec = vec;
+        X.uk = uk;
+        X.sh = sh;
+        X.part = part;
+        X.y_slot = (int64_t)std::max(dmax - 1, 1) * tile;
+        X.nq = std::max(dmax - 1, 1);
+        X.K = K;
+        X.R = R;
+        X.dmax = dmax;
+        X.tile = tile;
+        X.tpr = tpr;
+        const int vrc = batch_validate(probs, items, sample, X);
+        if (vrc) return vrc;
+    }
 
     BatchProblem* dprobs;
     int rc;
@@ -1199,6 +1266,7 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
     BALLOC(items_d, items.size() * sizeof(int2));
     BALLOC(Bt->sample_items, std::max<size_t>(sample.size(), 1) * sizeof(int2));
     BALLOC(B.counter, sizeof(unsigned long long));
+    BALLOC(B.status, sizeof(int));
     const std::vector<double> qh = sqrt_prime_fracs(dmax - 1);
     const std::vector<uint64_t> q64h = fixed_q64(qh);
     const std::vector<uint32_t> s32h = fixed_shift32(shifts, (size_t)sh);
@@ -1256,12 +1324,14 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
     for (int w = 0; w < 3; ++w) {
         float msum = 0.f;
         for (int rep = 0; rep < 2; ++rep) {  // first pass warms up
-            HIPCHK(fill_u64(B.counter, 0ull, s));
+            HIPCHK(queue_reset(B, s));
             HIPCHK(hipEventRecord(Bt->ev0, s));
             HIPCHK(batch_integrate(Bs, Bt->mixed, widths[w], Bt->grid, Bt->Uk, Bt->Ucol, Bt->Y, Bt->part_m,
                                    Bt->part_e, s));
             HIPCHK(hipEventRecord(Bt->ev1, s));
             HIPCHK(hipEventSynchronize(Bt->ev1));
+            rc = batch_status(Bt);
+            if (rc) return rc;
             float ms = 0.f;
             HIPCHK(hipEventElapsedTime(&ms, Bt->ev0, Bt->ev1));
             if (rep == 1) msum = ms;
@@ -1275,7 +1345,7 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
     // ---- the whole job as one graph: factor, queue reset, integration, reductions
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipError_t e = batch_factor(B, Bt->mixed, Bt->Uk, Bt->Ucol, s);
-    if (e == hipSuccess) e = fill_u64(B.counter, 0ull, s);
+    if (e == hipSuccess) e = queue_reset(B, s);
     if (e == hipSuccess)
         e = batch_integrate(B, Bt->mixed, Bt->W, Bt->grid, Bt->Uk, Bt->Ucol, Bt->Y, Bt->part_m, Bt->part_e, s);
     if (e == hipSuccess) e = batch_reduce(B, Bt->part_m, Bt->part_e, s);
@@ -1299,6 +1369,8 @@ int dbg_batch_create(dbg_batch** out, const dbg_opts* opts, int32_t K, const int
     g_err.clear();
     if (!out || !opts || !dims || !sigmas || !a || !b || K < 1) return fail(DBG_ERR_ARG, "invalid argument");
     *out = nullptr;
+    for (int32_t k = 0; k < K; ++k)
+        if (dims[k] > 1 && !shifts) return fail(DBG_ERR_ARG, "shifts must not be NULL when a dimension exceeds 1");
     if (opts->m < 1 || opts->n_reps < 1) return fail(DBG_ERR_ARG, "m and n_reps must be positive");
     if (opts->generator != DBG_GEN_RICHTMYER || opts->kind != DBG_KIND_DENSE || opts->phi_form != DBG_PHI_ERF)
         return fail(DBG_ERR_ARG, "batched problems use the dense Richtmyer erf-form path");
@@ -1335,7 +1407,7 @@ int dbg_batch_run(dbg_batch* Bt, double* results)
         HIPCHK(hipMemcpyAsync(results, Bt->B.out, (size_t)8 * Bt->K * sizeof(double), hipMemcpyDeviceToHost,
                               Bt->stream));
     HIPCHK(hipStreamSynchronize(Bt->stream));
-    return DBG_OK;
+    return batch_status(Bt);
 }
 
 int dbg_batch_last_ms(dbg_batch* Bt, float* ms)

@@ -162,6 +162,7 @@ struct BatchArgs {
     const int2* items;             // (problem, rep * tiles_per_rep + tile), largest d first
     int64_t n_items;
     unsigned long long* counter;   // work-queue head
+    int* status;                   // nonzero: a workgroup fetched an item twice (queue loop stopped)
     const double* q;
     const double* shifts;
     const uint64_t* q64;           // q in 64-bit fixed point (mixed)
@@ -177,7 +178,7 @@ hipError_t batch_factor(const BatchArgs& B, bool mixed, void* Uk, void* Ucol, hi
 hipError_t batch_integrate(const BatchArgs& B, bool mixed, int W, int grid, const void* Uk,
                            const void* Ucol, void* Y, double* part_m, int* part_e, hipStream_t s);
 hipError_t batch_reduce(const BatchArgs& B, const double* part_m, const int* part_e, hipStream_t s);
-hipError_t fill_u64(unsigned long long* p, unsigned long long v, hipStream_t s);
+hipError_t queue_reset(const BatchArgs& B, hipStream_t s);
 int batch_tile();
 
 // Per-shift reduction of the tile partials and running-mean merge

@@ -184,15 +184,11 @@ def test_split16_contraction(dbg, oracle, d, family, gen):
     ls, lf, lo = res["split16"][0], res["fp32"][0], r.log_value
     print(f"  log P split16 {ls:.10f} fp32 {lf:.10f} oracle {lo:.10f}", flush=True)
     assert np.isfinite(ls)
-    # |delta log P| is the relative error of P.  The split carries the operands
-    # to FP32-level precision, so its error against FP64 is of the FP32
-    # contraction's size: within 1e-4 where P is moderate, and no more than
-    # twice the FP32 contraction's own error in the deep lower orthant (log P
-    # about -2e4 at d = 700, where both are 2e-4 .. 5e-4)
-    if family == "random":
-        assert abs(ls - lo) < 1e-4 and abs(lf - lo) < 1e-4, (ls, lf, lo)
-    else:
-        assert abs(ls - lo) < 2 * abs(lf - lo) + 1e-5, (ls, lf, lo)
+    # |delta log P| is the relative error of P, bounded absolutely for both
+    # contractions: 1e-4 where P is moderate, 1e-3 in the deep lower orthant
+    # of the factor model (log P about -2e4 at d = 700 and -3e4 at d = 1100)
+    tol = 1e-4 if family == "random" else 1e-3
+    assert abs(ls - lo) < tol and abs(lf - lo) < tol, (ls, lf, lo)
 
 
 @pytest.mark.parametrize("d,m", [(96, 700), (130, 300), (257, 1000), (400, 513)])
