@@ -34,6 +34,7 @@ EXPORTS = (
     "dbg_plan_last_kernel_ms", "dbg_plan_summary", "dbg_plan_get_factor", "dbg_plan_get_reps",
     "dbg_plan_get_state", "dbg_plan_set_state", "dbg_plan_refactor", "dbg_batch_create",
     "dbg_batch_run", "dbg_batch_last_ms", "dbg_batch_tuning", "dbg_batch_destroy", "dbg_qmc_grad",
+    "dbg_plan_refine_info",
 )
 
 
@@ -52,7 +53,7 @@ class Opts(C.Structure):
         ("n_bits", C.c_int32), ("max_pts", C.c_int64), ("max_abs_err", C.c_double),
         ("precision", C.c_int32), ("generator", C.c_int32), ("phi_form", C.c_int32),
         ("kind", C.c_int32), ("sobol_skip", C.c_int64), ("contraction", C.c_int32),
-        ("reserved", C.c_int32),
+        ("refine_tiles", C.c_int32),
     ]
 
 
@@ -100,6 +101,7 @@ def lib():
     L.dbg_plan_get_state.argtypes = [vp, i64p]
     L.dbg_plan_set_state.argtypes = [vp, C.c_int64]
     L.dbg_plan_refactor.argtypes = [vp]
+    L.dbg_plan_refine_info.argtypes = [vp, ip32, dp, C.POINTER(C.c_float)]
     L.dbg_batch_create.argtypes = [C.POINTER(vp), C.POINTER(Opts), C.c_int32, ip32, dp, dp, dp, dp]
     L.dbg_batch_run.argtypes = [vp, dp]
     L.dbg_batch_last_ms.argtypes = [vp, C.POINTER(C.c_float)]

@@ -249,6 +249,30 @@ struct dbg_plan {
     double* rep_m = nullptr;
     int* rep_e = nullptr;
     double* out = nullptr;
+    // FP64 re-evaluation of the heaviest tiles (mixed dense plans): K tiles per
+    // shift, the FP64 kernel's layouts of the FP64-truncated factor, its tiles
+    // (ratio per mixed tile) and partials, the picks and the unpicked shares
+    struct Refine {
+        int K = 0;
+        int ratio = 1;
+        bool mfma = false;
+        int grid = 0;
+        int64_t ldu = 0, tpr = 0, n_tiles = 0, y_slot = 0;
+        double* ucol = nullptr;   // n x n, column-major (the FP64 factor)
+        double* uk = nullptr;     // packed rows, ldu
+        double* ak = nullptr;
+        double* bk = nullptr;
+        double* invk = nullptr;
+        double* first = nullptr;
+        double* Y = nullptr;      // grid slots of y_slot
+        double* part_m = nullptr; // n_tiles
+        int* part_e = nullptr;
+        int64_t* sel = nullptr;   // n_reps x K
+        int64_t* list = nullptr;  // n_reps x K x ratio
+        double* frac = nullptr;   // n_reps
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
+        std::vector<void*> all() const { return {ucol, uk, ak, bk, invk, first, Y, part_m, part_e, sel, list, frac}; }
+    } R;
     // adjoint buffers (allocated on the first dbg_qmc_grad)
     struct Grad {
         bool ready = false;
@@ -283,6 +307,9 @@ struct dbg_plan {
     void release()
     {
         for (void* p : G.all()) (void)hipFree(p);
+        for (void* p : R.all()) (void)hipFree(p);
+        if (R.ev0) (void)hipEventDestroy(R.ev0);
+        if (R.ev1) (void)hipEventDestroy(R.ev1);
         F.release();
         void* ptrs[] = {sig, a0, b0, Uk, ucol32, uc16, ak, bk, invk, first, colcnt, colptr, rowval, nzval, reach_d, q, shifts_f,
                         q64, shift32, shifts_u, V, Y, part_m, part_e, rep_m, rep_e, out};
@@ -308,7 +335,14 @@ int plan_factor(dbg_plan* P)
     if (rc) return rc;
     if (pivot && P->rank != n)
         return fail(DBG_ERR_RANK, "QMCData requires a full-rank Cholesky factorization.");
-    // truncate_matrixU!: |U_ij| < eps(T) / (9n) -> 0
+    // truncate_matrixU!: |U_ij| < eps(T) / (9n) -> 0.  A refining mixed plan
+    // first keeps the factor truncated as in Float64 for its FP64 tiles.
+    if (P->R.K > 0) {
+        HIPCHK(truncate_upper(P->F.M, n, n, DBL_EPSILON / (double)(n * 9), s));
+        HIPCHK(hipMemcpyAsync(P->R.ucol, P->F.M, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        HIPCHK(pack_dense(P->R.ucol, n, n, P->F.a, P->F.b, false, P->R.uk, P->R.ldu, P->R.ak, P->R.bk, P->R.invk,
+                          P->R.first, DBG_PHI_ERF, s));
+    }
     const double min_val = P->mixed ? (double)(FLT_EPSILON / (float)(n * 9)) : DBL_EPSILON / (double)(n * 9);
     HIPCHK(truncate_upper(P->F.M, n, n, min_val, s));
     const size_t T = P->tsz;
@@ -391,6 +425,39 @@ int plan_build(dbg_plan* P, const double* Sigma, int64_t lda, const double* a, c
     P->split = P->use_mfma && P->mixed && o.contraction != DBG_CONTRACT_FP32;
     if (P->split)
         HIPCHK(dalloc(&P->uc16, (size_t)split16_cols(n) * (size_t)split16_col_stride(n) * sizeof(uint16_t)));
+    // tiles of the mixed kernel (for the refinement's tile arithmetic)
+    const int tile_mixed = P->split ? dense_split_tile() : (P->use_mfma ? dense_mfma_tile(true) : dense_valu_tile());
+    const int64_t tpr_mixed = (o.m + tile_mixed - 1) / tile_mixed;
+    P->R.K = (P->mixed && o.kind == DBG_KIND_DENSE && nd > 0)
+                 ? (int)std::min<int64_t>(o.refine_tiles < 0 ? kDefaultRefineTiles : o.refine_tiles, tpr_mixed)
+                 : 0;
+    if (P->R.K > 0) {
+        auto& R = P->R;
+        R.mfma = n >= kMfmaMinDim;
+        const int t64 = R.mfma ? dense_mfma_tile(false) : dense_valu_tile();
+        if (tile_mixed % t64 != 0) return fail(DBG_ERR_ARG, "refinement: mixed tile not a multiple of the FP64 tile");
+        R.ratio = tile_mixed / t64;
+        R.ldu = (int64_t)n + dense_mfma_panel(false) + 32;
+        R.tpr = (o.m + t64 - 1) / t64;
+        R.n_tiles = R.tpr * o.n_reps;
+        const int64_t nlist = (int64_t)o.n_reps * R.K * R.ratio;
+        R.grid = (int)std::min<int64_t>(nlist, (int64_t)ncu * (R.mfma ? 1 : 2));
+        R.y_slot = (int64_t)nd * t64;
+        HIPCHK(dalloc(&R.ucol, (size_t)n * n * sizeof(double)));
+        HIPCHK(dalloc(&R.uk, (size_t)nd * R.ldu * sizeof(double)));
+        HIPCHK(dalloc(&R.ak, n * sizeof(double)));
+        HIPCHK(dalloc(&R.bk, n * sizeof(double)));
+        HIPCHK(dalloc(&R.invk, n * sizeof(double)));
+        HIPCHK(dalloc(&R.first, 8 * sizeof(double)));
+        HIPCHK(dalloc(&R.Y, (size_t)R.grid * R.y_slot * sizeof(double)));
+        HIPCHK(dalloc(&R.part_m, (size_t)R.n_tiles * sizeof(double)));
+        HIPCHK(dalloc(&R.part_e, (size_t)R.n_tiles * sizeof(int)));
+        HIPCHK(dalloc(&R.sel, (size_t)o.n_reps * R.K * sizeof(int64_t)));
+        HIPCHK(dalloc(&R.list, (size_t)nlist * sizeof(int64_t)));
+        HIPCHK(dalloc(&R.frac, (size_t)o.n_reps * sizeof(double)));
+        HIPCHK(hipEventCreate(&R.ev0));
+        HIPCHK(hipEventCreate(&R.ev1));
+    }
     rc = plan_factor(P);
     if (rc) return rc;
 
@@ -480,6 +547,35 @@ int run_loop(dbg_plan* P, int64_t block, int64_t n_pts0)
             HIPCHK(launch_dense_mfma(A, P->mixed, gen, o.phi_form, P->grid, P->stream));
         else
             HIPCHK(launch_dense_valu(A, P->mixed, gen, o.phi_form, P->grid, P->stream));
+        HIPCHK(hipEventRecord(P->ev1, P->stream));
+        if (P->R.K > 0) {
+            // the heaviest tiles of each shift again, in FP64 (tail form)
+            auto& R = P->R;
+            HIPCHK(hipEventRecord(R.ev0, P->stream));
+            HIPCHK(refine_select(P->part_m, P->part_e, P->tiles_per_rep, o.n_reps, R.K, R.ratio, R.tpr, R.sel, R.list,
+                                 R.frac, P->stream));
+            DenseArgs B = A;
+            B.tiles_per_rep = R.tpr;
+            B.n_tiles = (int64_t)o.n_reps * R.K * R.ratio;
+            B.tile_list = R.list;
+            B.Uk = R.uk;
+            B.ldu = R.ldu;
+            B.Ucol = R.ucol;
+            B.ak = R.ak;
+            B.bk = R.bk;
+            B.invk = R.invk;
+            B.Y = R.Y;
+            B.y_slot = R.y_slot;
+            B.part_m = R.part_m;
+            B.part_e = R.part_e;
+            if (R.mfma)
+                HIPCHK(launch_dense_mfma(B, false, gen, kPhiTail, R.grid, P->stream));
+            else
+                HIPCHK(launch_dense_valu(B, false, gen, kPhiTail, R.grid, P->stream));
+            HIPCHK(refine_merge(R.sel, o.n_reps, R.K, R.ratio, P->tiles_per_rep, R.tpr, R.part_m, R.part_e,
+                                P->part_m, P->part_e, P->stream));
+            HIPCHK(hipEventRecord(R.ev1, P->stream));
+        }
     } else {
         SparseArgs A{};
         A.n = P->n;
@@ -503,8 +599,8 @@ int run_loop(dbg_plan* P, int64_t block, int64_t n_pts0)
         A.part_e = P->part_e;
         const int grid = P->ring > 0 ? (int)std::min<int64_t>(P->n_tiles, P->ring_grid) : P->grid;
         HIPCHK(launch_sparse(A, P->mixed, gen, P->ring, grid, P->stream));
+        HIPCHK(hipEventRecord(P->ev1, P->stream));
     }
-    HIPCHK(hipEventRecord(P->ev1, P->stream));
     HIPCHK(launch_reduce_reps(P->part_m, P->part_e, P->tiles_per_rep, o.n_reps, n_pts0, o.m,
                               o.kind == DBG_KIND_SPARSEQMC ? 1 : 0, P->rep_m, P->rep_e, P->out,
                               P->stream));
@@ -692,7 +788,7 @@ extern "C" {
 
 const char* dbg_last_error(void) { return g_err.c_str(); }
 
-int32_t dbg_abi_version(void) { return 2; }
+int32_t dbg_abi_version(void) { return 3; }
 
 void dbg_opts_default(dbg_opts* o, int32_t precision)
 {
@@ -716,6 +812,7 @@ void dbg_opts_default(dbg_opts* o, int32_t precision)
     o->phi_form = DBG_PHI_ERF;
     o->kind = DBG_KIND_DENSE;
     o->sobol_skip = -1;
+    o->refine_tiles = -1;
 }
 
 int dbg_chol_genz_f64(double* M, int64_t lda, int32_t n, double* a, double* b, int32_t bs1,
@@ -991,6 +1088,23 @@ int dbg_qmc_grad(dbg_plan* P, double* log_value, double* ga, double* gb, double*
     if (gU) HIPCHK(hipMemcpyAsync(gU, G.ub, nn, hipMemcpyDeviceToHost, P->stream));
     HIPCHK(hipStreamSynchronize(P->stream));
     if (log_value) *log_value = norm[2];
+    return DBG_OK;
+}
+
+int dbg_plan_refine_info(dbg_plan* P, int32_t* tiles_per_shift, double* unrefined_share, float* ms)
+{
+    g_err.clear();
+    if (!P) return fail(DBG_ERR_ARG, "null plan");
+    auto& R = P->R;
+    if (tiles_per_shift) *tiles_per_shift = R.K;
+    if (unrefined_share) *unrefined_share = 0.0;
+    if (ms) *ms = 0.0f;
+    if (R.K == 0) return DBG_OK;
+    std::vector<double> f(P->o.n_reps, 0.0);
+    HIPCHK(hipMemcpyAsync(f.data(), R.frac, f.size() * sizeof(double), hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    if (unrefined_share) *unrefined_share = *std::max_element(f.begin(), f.end());
+    if (ms) HIPCHK(hipEventElapsedTime(ms, R.ev0, R.ev1));
     return DBG_OK;
 }
 

@@ -45,6 +45,9 @@ struct DenseArgs {
     int64_t y_slot;            // elements per slot
     double* part_m;            // per-tile partial sums (scaled)
     int* part_e;
+    // optional list of the tiles to run (FP64 integration kernels): entry i of
+    // n_tiles names tile tile_list[i] of the round, entries < 0 are skipped
+    const int64_t* tile_list;
     // adjoint support (matrix-core kernel only)
     int mode;                  // kModeIntegrate, kModeRecord or kModeReverse
     int64_t rs;                // row stride of the records: n_tiles * tile samples
@@ -118,6 +121,8 @@ int dense_split_tile();
 int64_t dense_split_slot(int nd);   // y workspace floats per workgroup slot
 // dimension from which the dense path uses the matrix-core kernel
 constexpr int kMfmaMinDim = 96;
+// FP64 tiles re-evaluated per shift by a mixed dense plan unless its options say otherwise
+constexpr int kDefaultRefineTiles = 8;
 
 // ring: 0 (y rows in the global workspace) or the LDS ring size from sparse_ring
 hipError_t launch_sparse(const SparseArgs& A, bool mixed, int gen, int ring, int grid, hipStream_t s);
@@ -222,6 +227,15 @@ hipError_t pack_dense(const double* U, int64_t lda, int n, const double* a, cons
                       double* first /* c1, dc1, ea1, eb1, d1, mode1 */, int phi_form,
                       hipStream_t s);
 hipError_t convert_f32(const double* in, size_t count, float* out, hipStream_t s);
+// FP64 re-evaluation of the heaviest tiles of a mixed round (refine.hip):
+// per shift, the K tiles with the largest partial sums (sel: global mixed tile
+// indices, list64: the FP64 tiles covering them, ratio per mixed tile; -1
+// where fewer than K tiles are nonzero) and the share of the shift's sum the
+// other tiles hold (frac); then the picked partials from the FP64 tiles.
+hipError_t refine_select(const double* part_m, const int* part_e, int64_t tpr, int n_reps, int K, int ratio,
+                         int64_t tpr64, int64_t* sel, int64_t* list64, double* frac, hipStream_t s);
+hipError_t refine_merge(const int64_t* sel, int n_reps, int K, int ratio, int64_t tpr, int64_t tpr64,
+                        const double* p64_m, const int* p64_e, double* part_m, int* part_e, hipStream_t s);
 hipError_t csc_count(const double* U, int64_t lda, int n, int64_t* colcount, hipStream_t s);
 hipError_t csc_fill(const double* U, int64_t lda, int n, const int64_t* colptr, bool mixed,
                     int32_t* rowval, void* nzval, hipStream_t s);

@@ -135,7 +135,9 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
     const bool walker = tid < TS;
     const PointArgs pts = DBG_WITH_TABLES(A.pts);
 
-    for (int64_t tile = blockIdx.x; tile < A.n_tiles; tile += gridDim.x) {
+    for (int64_t it = blockIdx.x; it < A.n_tiles; it += gridDim.x) {
+        const int64_t tile = A.tile_list ? A.tile_list[it] : it;
+        if (tile < 0) continue;  // uniform: one entry per workgroup
         const int k = (int)(tile / A.tiles_per_rep);
         const int64_t jt = tile - (int64_t)k * A.tiles_per_rep;
         const int64_t j = jt * TS + tid + 1;  // 1-based point index of the walker's sample

@@ -42,7 +42,9 @@ __global__ __launch_bounds__(kTileV) void dense_valu_kernel(DenseArgs A, const T
     const int nd = A.n - 1;
     T* __restrict__ Ys = Yall + (size_t)blockIdx.x * (size_t)A.y_slot;
 
-    for (int64_t tile = blockIdx.x; tile < A.n_tiles; tile += gridDim.x) {
+    for (int64_t it = blockIdx.x; it < A.n_tiles; it += gridDim.x) {
+        const int64_t tile = A.tile_list ? A.tile_list[it] : it;
+        if (tile < 0) continue;  // uniform: one entry per workgroup
         const int k = (int)(tile / A.tiles_per_rep);
         const int64_t jt = tile - (int64_t)k * A.tiles_per_rep;
         const int64_t j = jt * kTileV + tid + 1;  // 1-based point index of the round

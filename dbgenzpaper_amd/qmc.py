@@ -243,7 +243,8 @@ class _QMCBase:
     _phi = L.PHI_ERF
 
     def __init__(self, C_mat, a, b, opts: QMCOpts | None, rng, qmc_type, *, shifts=None,
-                 sobol_skip=None, chol_sizes=None, phi_form=None, contraction=None, _clamp=True):
+                 sobol_skip=None, chol_sizes=None, phi_form=None, contraction=None, refine_tiles=None,
+                 _clamp=True):
         Cm = _as_matrix(C_mat)
         n = Cm.shape[0]
         if opts is None:
@@ -281,6 +282,8 @@ class _QMCBase:
         # arithmetic of the mixed dense contraction: None/"auto", "fp32" or "split16"
         o.contraction = {None: L.CONTRACT_AUTO, "auto": L.CONTRACT_AUTO, "fp32": L.CONTRACT_FP32,
                          "split16": L.CONTRACT_SPLIT16}[contraction]
+        # mixed dense plans: tiles per shift evaluated again in FP64 (None: the library's default)
+        o.refine_tiles = -1 if refine_tiles is None else int(refine_tiles)
         self._opts_c = o
         plan = C.c_void_p()
         rc = L.lib().dbg_plan_create(C.byref(plan), C.byref(o), n, L.ptr(self._Sigma, C.c_double), n,
@@ -395,6 +398,14 @@ class _QMCBase:
         ms = C.c_float()
         L.check(L.lib().dbg_plan_last_kernel_ms(self._plan, C.byref(ms)))
         return float(ms.value)
+
+    def refine_info(self):
+        """(tiles per shift, largest share of a shift's sum left in FP32,
+        device ms) of the last round's FP64 re-evaluation (mixed dense
+        plans; zeros otherwise)."""
+        k, f, ms = C.c_int32(), C.c_double(), C.c_float()
+        L.check(L.lib().dbg_plan_refine_info(self._plan, C.byref(k), C.byref(f), C.byref(ms)))
+        return int(k.value), float(f.value), float(ms.value)
 
     def summary(self):
         r = L.Result()

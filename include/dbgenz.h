@@ -82,7 +82,11 @@ typedef struct dbg_opts {
     int32_t kind;             /* DBG_KIND_* */
     int64_t sobol_skip;       /* next! calls before the first Sobol point; < 0: as skip(s, 2m-1) */
     int32_t contraction;      /* DBG_CONTRACT_* (mixed dense path) */
-    int32_t reserved;         /* zero */
+    int32_t refine_tiles;     /* mixed dense path: after each round, the refine_tiles tiles of 256
+                                 samples with the largest sums of every shift are evaluated again in
+                                 FP64 (the plain FP64 path's arithmetic) and replace their FP32 sums;
+                                 0: none; < 0: the library's default (8).  No reference counterpart:
+                                 the reference's Float32 mode has no FP64 step in the loop */
 } dbg_opts;
 
 /* Result of qmc_pnorm!: (value, err, n_pts) of src/qmc.jl:829,849 plus the
@@ -227,6 +231,14 @@ int dbg_plan_summary(dbg_plan* plan, dbg_result* out);
  * gradient. */
 int dbg_qmc_grad(dbg_plan* plan, double* log_value, double* ga, double* gb, double* gSigma,
                  double* ga_std, double* gb_std, double* gU);
+
+/* The FP64 re-evaluation of the last round of a mixed dense plan (no
+ * reference counterpart): tiles re-evaluated per shift, the largest share of
+ * a shift's sum held by the tiles left in FP32 (the estimate differs from the
+ * plain FP64 path's only through those tiles), and the device time of the
+ * selection, the FP64 tiles and the merge (HIP events on the plan's stream).
+ * Zeros for plans without it. */
+int dbg_plan_refine_info(dbg_plan* plan, int32_t* tiles_per_shift, double* unrefined_share, float* ms);
 
 /* The factor and the shift means held by the plan. */
 int dbg_plan_get_factor(dbg_plan* plan, double* U, int32_t* perm, double* a, double* b,

@@ -32,12 +32,12 @@ def test_abi_version_and_defaults(dbg):
     from dbgenzpaper_amd import _lib
 
     L = _lib.lib()
-    assert L.dbg_abi_version() == 2
+    assert L.dbg_abi_version() == 3
     o = _lib.Opts()
     # the ctypes mirror has the size of the C struct (int32/int64 fields, 8-byte aligned)
     assert ctypes.sizeof(o) == 96
     L.dbg_opts_default(ctypes.byref(o), _lib.PREC_FP64)
-    assert o.contraction == _lib.CONTRACT_AUTO
+    assert o.contraction == _lib.CONTRACT_AUTO and o.refine_tiles == -1
     # QMC_opts defaults, src/qmc.jl:111-125
     assert (o.chol_block_size, o.chol_block_size2, o.m, o.block_size_i, o.block_size_j,
             o.block_size_i2, o.n_reps, o.n_bits, o.max_pts) == (64, 512, 10000, 64, 64, 32, 12, 32, 10000)
@@ -252,3 +252,53 @@ def test_matern_closed_forms(nu):
     general = 2 ** (1 - nu) / gamma(nu) * z ** nu * kv(nu, z)
     closed = {0.5: np.exp(-z), 1.5: (1 + z) * np.exp(-z), 2.5: (1 + z + z * z / 3) * np.exp(-z)}[nu]
     assert np.allclose(closed, general, rtol=1e-13, atol=0)
+
+
+_CTYPE = {"int32_t": "c_int", "int64_t": "c_long", "double": "c_double"}
+_JULIA = {"Int32": "c_int", "Int64": "c_long", "Float64": "c_double"}
+
+
+def _c_struct_fields(src, name):
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), src, flags=re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    return [(f, _CTYPE[t]) for t, f in re.findall(r"(int32_t|int64_t|double)\s+(\w+);", body)]
+
+
+def _julia_struct_fields(md, name):
+    body = re.search(r"struct %s\b[^\n]*\n(.*?)\nend" % name, md, flags=re.S).group(1)
+    body = re.sub(r"#[^\n]*", "", body)
+    return [(f, _JULIA[t]) for f, t in re.findall(r"(\w+)::(Int32|Int64|Float64)", body)]
+
+
+def test_julia_binding_matches_the_header():
+    """INTEGRATION.md's Julia structs have the C structs' fields in order and
+    type, the ctypes mirror too, and the routing function builds DbgOpts with
+    one argument per field."""
+    from dbgenzpaper_amd import _lib
+
+    hdr = open(os.path.join(ROOT, "include", "dbgenz.h")).read()
+    md = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    for cname, jname, mirror in (("dbg_opts", "DbgOpts", _lib.Opts), ("dbg_result", "DbgResult", _lib.Result)):
+        c = _c_struct_fields(hdr, cname)
+        j = _julia_struct_fields(md, jname)
+        py = [(f, t.__name__) for f, t in mirror._fields_]
+        py = [(f, {"c_int32": "c_int", "c_int64": "c_long"}.get(t, t)) for f, t in py]
+        assert c == j, (cname, c, j)
+        assert c == py, (cname, c, py)
+    # DbgOpts(...) in the routing function: one argument per field
+    call = md[md.index("o = DbgOpts(") + len("o = DbgOpts("):]
+    depth, args, cur = 1, [], ""
+    for ch in call:
+        if ch == "(":
+            depth += 1
+        elif ch == ")":
+            depth -= 1
+            if depth == 0:
+                args.append(cur)
+                break
+        if ch == "," and depth == 1:
+            args.append(cur)
+            cur = ""
+        else:
+            cur += ch
+    assert len(args) == len(_c_struct_fields(hdr, "dbg_opts")), args
