@@ -152,15 +152,41 @@ int factor_on_device(CholBuffers& B, int n, const double* M_in, int64_t lda, con
     CholArgs C{B.M, (int64_t)n, n, B.a, B.b, B.scale, B.perm, B.s_vec, B.eps0, B.rank};
     HIPCHK(chol_standardize(C, false, s));
     const int nb = std::max(1, std::min(bs1, 64));
-    for (auto& w : pivot_windows(n, bs2, bs3, bs4)) {
+    for (auto& w : pivot_windows(n, bs2, bs3, bs4))
         if (w.second - w.first > chol_max_window())
             return fail(DBG_ERR_ARG, "Cholesky pivot window (min(chol_block_size2, 1024-aligned blocks)) above " +
                                          std::to_string(chol_max_window()));
-        HIPCHK(chol_window(C, w.first, w.second, nb, pivot, s));
-        HIPCHK(chol_trsm_syrk(C, w.first, w.second, s));
+    // the reference's loop nest iv (bs4) > iii (bs3) > ii (bs2): each window is
+    // factored, then solved and applied up to the end of its iii block; a
+    // finished iii block is solved and applied up to the end of its iv block,
+    // a finished iv block up to n (src/cholesky.jl:482-491,589-624).  A pivoted
+    // factorization that stops for rank leaves all later solves undone, as the
+    // reference's break out of the loop nest does: the rank is read back after
+    // each window.
+    int32_t rank = n;
+    bool stopped = false;
+    for (int iv0 = 0; iv0 < n && !stopped; iv0 += bs4) {
+        const int iv1 = std::min(n, iv0 + bs4);
+        for (int iii0 = iv0; iii0 < iv1 && !stopped; iii0 += bs3) {
+            const int iii1 = std::min(iv1, iii0 + bs3);
+            for (int ii0 = iii0; ii0 < iii1; ii0 += bs2) {
+                const int ii1 = std::min(iii1, ii0 + bs2);
+                HIPCHK(chol_window(C, ii0, ii1, nb, pivot, s));
+                if (pivot) {
+                    HIPCHK(hipMemcpyAsync(&rank, B.rank, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+                    HIPCHK(hipStreamSynchronize(s));
+                    if (rank < n) {
+                        stopped = true;
+                        break;
+                    }
+                }
+                HIPCHK(chol_trsm_syrk(C, ii0, ii1, ii1, iii1, s));
+            }
+            if (!stopped) HIPCHK(chol_trsm_syrk(C, iii0, iii1, iii1, iv1, s));
+        }
+        if (!stopped) HIPCHK(chol_trsm_syrk(C, iv0, iv1, iv1, n, s));
     }
     HIPCHK(chol_finish(C, s));
-    int32_t rank = n;
     HIPCHK(hipMemcpyAsync(&rank, B.rank, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     *rank_out = rank;

@@ -763,14 +763,15 @@ hipError_t chol_syrk(const CholArgs& C, int k0, int k1, int r0, int r1, hipStrea
     return hipGetLastError();
 }
 
-// Rows [w0, w1) of U beyond the window, U_ww^T X = M[w0:w1, w1:n] (a plain
-// triangular solve, left / upper / transposed / non-unit), then the trailing
-// update of [w1, n).  A window that found the matrix rank deficient leaves
-// values that are never read: the caller reports the rank and discards M.
-hipError_t chol_trsm_syrk(const CholArgs& C, int w0, int w1, hipStream_t s)
+// Rows [r0, r1) of U in columns [c0, c1), U_rr^T X = M[r0:r1, c0:c1] with
+// the factored diagonal block U_rr = M[r0:r1, r0:r1] (a plain triangular
+// solve, left / upper / transposed / non-unit), then the trailing update of
+// [c0, c1) by those rows: the ldiv! and mul! of one level of the reference's
+// blocking (src/cholesky.jl:589-624; r1 = c0, c1 the parent block's end).
+hipError_t chol_trsm_syrk(const CholArgs& C, int r0, int r1, int c0, int c1, hipStream_t s)
 {
-    const int rest = C.n - w1;
-    if (rest <= 0) return hipSuccess;
+    const int rest = c1 - c0;
+    if (rest <= 0 || r1 <= r0) return hipSuccess;
     rocblas_handle h = blas_handle(s);
     if (!h) return hipErrorNotInitialized;
     const double one = 1.0;
@@ -778,10 +779,10 @@ hipError_t chol_trsm_syrk(const CholArgs& C, int w0, int w1, hipStream_t s)
     const int64_t lda = C.lda;
     const rocblas_status st =
         rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
-                      rocblas_diagonal_non_unit, w1 - w0, rest, &one, &MM(w0, w0), (rocblas_int)lda,
-                      &MM(w0, w1), (rocblas_int)lda);
+                      rocblas_diagonal_non_unit, r1 - r0, rest, &one, &MM(r0, r0), (rocblas_int)lda,
+                      &MM(r0, c0), (rocblas_int)lda);
     if (st != rocblas_status_success) return hipErrorLaunchFailure;
-    return chol_syrk(C, w0, w1, w1, C.n, s);
+    return chol_syrk(C, r0, r1, c0, c1, s);
 }
 
 __global__ void chol_zero_lower_kernel(double* M, int64_t lda, int n)
@@ -822,6 +823,7 @@ __global__ __launch_bounds__(kAlrThreads) void adapt_low_rank_kernel(double* __r
 {
     __shared__ int s_lvl[kAlrThreads / 64];
     __shared__ int s_ij, s_j, s_nact, s_k0;
+    __shared__ double s_mij;
     const int tid = threadIdx.x;
     for (int i = tid; i < rank; i += kAlrThreads) end[i] = i;
     if (tid == 0) {
@@ -844,6 +846,8 @@ __global__ __launch_bounds__(kAlrThreads) void adapt_low_rank_kernel(double* __r
             int m = -1;
             for (int w = 0; w < kAlrThreads / 64; ++w) m = max(m, s_lvl[w]);
             s_ij = m;
+            // the pivot entry, read once before the scaling below rewrites it
+            s_mij = m >= 0 ? MM(m, j) : 0.0;
         }
         __syncthreads();
         const int ij = s_ij;
@@ -870,7 +874,7 @@ __global__ __launch_bounds__(kAlrThreads) void adapt_low_rank_kernel(double* __r
             continue;
         }
         // 2. scale the column up to its level, and the bounds
-        const double mij = MM(ij, j);
+        const double mij = s_mij;
         const double inv = 1.0 / mij;
         for (int r = tid; r <= ij; r += kAlrThreads) MM(r, j) *= inv;
         if (tid == 0) {
@@ -963,7 +967,9 @@ __global__ __launch_bounds__(256) void batch_factor_kernel(BatchArgs B,
         b[j] = B.b_in[P.vec_off + j];
         sv[j] = 0.0;
         perm[j] = j;
-        const double v = M[(size_t)j * (d + 1)];
+        // the diagonal from the input: the copy into M above is made by other
+        // threads and is not complete before the barrier below
+        const double v = B.sig_in[P.sig_off + (int64_t)j * (d + 1)];
         const double av = fabs(v);
         mx = (isnan(av) || av > mx) ? av : mx;
         scale[j] = v > 0 ? sqrt(v) : 1.0;

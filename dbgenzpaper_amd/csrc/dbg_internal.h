@@ -207,7 +207,7 @@ struct CholArgs {
 };
 hipError_t chol_standardize(const CholArgs& C, bool mirror, hipStream_t s);
 hipError_t chol_window(const CholArgs& C, int w0, int w1, int nb, int pivot, hipStream_t s);
-hipError_t chol_trsm_syrk(const CholArgs& C, int w0, int w1, hipStream_t s);
+hipError_t chol_trsm_syrk(const CholArgs& C, int r0, int r1, int c0, int c1, hipStream_t s);
 hipError_t chol_finish(const CholArgs& C, hipStream_t s);
 // covariance generators (cov.hip); kinds as DBG_COV_* of the C ABI
 constexpr int kCovFixedDense = 0, kCovFixedSparse = 1, kCovMatern = 2, kCovLowRank = 3;

@@ -80,3 +80,29 @@ def test_batch_ragged_edges(oracle):
             continue
         ro = _oracle(oracle, p)
         assert rel(r.value[k], ro.value) < 1e-11, (k, p.d, r.value[k], ro.value)
+
+
+def test_batch_large_d_repeatable():
+    """48 problems of d = 200..256 (long copy loops in the batched
+    constructor): every factor of full rank, replays identical, the FP64
+    batch equal to single-problem plans and the mixed batch finite and close
+    to it."""
+    probs = W.batch_uneven(48, d_min=200, d_max=256, m=512, n_reps=2, seed=35)
+    print("batch large d K=48 m=512 dims", sorted(p.d for p in probs), flush=True)
+    B64 = Batch(*_args(probs), T=np.float64, m=512, n_reps=2)
+    r1, r2 = B64.run(), B64.run()
+    B64.close()
+    assert np.array_equal(r1.rank, [p.d for p in probs])
+    assert np.array_equal(r1.value, r2.value)
+    for k in range(0, 48, 6):
+        p = probs[k]
+        opts = dbg.QMC_opts(np.float64, m=p.m, n_reps=p.n_reps, max_pts=p.m, max_abs_err=0.0)
+        D = dbg.QMCData(p.Sigma, p.a, p.b, opts, None, "Richtmyer", shifts=p.shifts, phi_form="erf")
+        v, _, _ = dbg.qmc_pnorm(D)
+        D.close()
+        assert rel(r1.value[k], v) < 1e-12, (k, p.d, r1.value[k], v)
+    r32 = Batch(*_args(probs), T=np.float32, m=512, n_reps=2).run()
+    assert np.array_equal(r32.rank, r1.rank)
+    err = np.abs(np.expm1(r32.log_value - r1.log_value))
+    print("  mixed worst rel err", err.max(), flush=True)
+    assert np.all(np.isfinite(r32.log_value)) and err.max() < 1e-4

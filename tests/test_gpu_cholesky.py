@@ -218,3 +218,61 @@ def test_leading_dimension_padding(dbg, oracle):
     Mo, ao, bo, po, ko, k0o = oracle.adapt_low_rank(M, a2, b2, 40, np.arange(n), 1e-10)
     assert np.array_equal(Wm[:n, :], Mo) and np.all(Wm[n:, :] == -5.0)
     assert np.array_equal(pv, po) and np.array_equal(kjs, ko) and k0.value == k0o
+
+
+def _copies_case(n, n_orig, seed, corr=True, big=1e6):
+    """n_orig variables (correlated, one with variance `big`, so that
+    eps0 = eps * max(diag) sits far above the rounding-level Schur diagonals
+    of exact copies; or independent with exact unit correlations to their
+    copies) followed by n - n_orig exact copies of them."""
+    rng = np.random.default_rng(seed)
+    if corr:
+        A = rng.standard_normal((n_orig, n_orig))
+        S0 = A @ A.T + n_orig * np.eye(n_orig)
+    else:
+        # variances powers of 4: the standardized copies correlate exactly 1
+        S0 = np.diag(4.0 ** rng.integers(-1, 2, n_orig))
+    S0[0, :] *= np.sqrt(big)
+    S0[:, 0] *= np.sqrt(big)
+    src = np.concatenate([np.arange(n_orig), rng.integers(0, n_orig, n - n_orig)])
+    S = np.asfortranarray(S0[np.ix_(src, src)])
+    sd = np.sqrt(np.diag(S))
+    a = -rng.uniform(0.5, 2.0, n_orig)[src] * sd
+    b = rng.uniform(0.5, 2.0, n_orig)[src] * sd
+    return S, a, b
+
+
+def test_genz_rank_stop_levels(dbg, oracle):
+    """cholesky_genz! stopping for rank inside the second pivot window of the
+    first bs3 block (n = 1500, bs2 = 512, bs3 = 1024): like the reference's
+    break out of the loop nest (src/cholesky.jl:533-538), nothing later is
+    solved -- the first window's rows stay unsolved past column 1024 (its
+    solve reaches the end of its bs3 block only) -- and the whole upper
+    triangle, the permutation and the rank equal the oracle's."""
+    S, a, b = _copies_case(1500, 700, 7)
+    print("genz rank stop n=1500 rank 700", flush=True)
+    bs = (64, 512, 1024, 2048)
+    Fo = oracle.cholesky_genz(S, a, b, *bs)
+    Fg = dbg.cholesky_genz(S.copy(order="F"), a.copy(), b.copy(), *bs)
+    print(f"  rank device {Fg.rank} oracle {Fo.rank}", flush=True)
+    assert Fg.rank == Fo.rank == 700
+    assert np.array_equal(Fg.perm, Fo.perm)
+    du = np.abs(np.triu(Fg.U) - np.triu(Fo.U)).max()
+    print(f"  max |dM| upper {du:.2e}", flush=True)
+    assert du < 1e-12
+
+
+def test_rowmax_rank_stop_windows(dbg, oracle):
+    """cholesky_rowmax! with rank 300 < bs2 = 512 < n = 700 (default blocks):
+    independent variables and exact copies, so the Schur diagonals of the
+    copies are exactly 0 and the stop (m_ii <= 0 before the swap,
+    src/cholesky.jl:329-350) falls at 300 in both; nothing after it is
+    solved.  The whole upper triangle, permutation and rank equal the
+    oracle's."""
+    S, a, b = _copies_case(700, 300, 8, corr=False, big=1.0)
+    print("rowmax rank stop n=700 rank 300", flush=True)
+    Fo = oracle.cholesky_rowmax(S, a, b)
+    Fg = dbg.cholesky_rowmax(S.copy(order="F"), a.copy(), b.copy())
+    assert Fg.rank == Fo.rank == 300
+    assert np.array_equal(Fg.perm, Fo.perm)
+    assert np.abs(np.triu(Fg.U) - np.triu(Fo.U)).max() < 1e-12
