@@ -34,7 +34,7 @@ EXPORTS = (
     "dbg_plan_last_kernel_ms", "dbg_plan_summary", "dbg_plan_get_factor", "dbg_plan_get_reps",
     "dbg_plan_get_state", "dbg_plan_set_state", "dbg_plan_refactor", "dbg_batch_create",
     "dbg_batch_run", "dbg_batch_last_ms", "dbg_batch_tuning", "dbg_batch_destroy", "dbg_qmc_grad",
-    "dbg_plan_refine_info",
+    "dbg_plan_refine_info", "dbg_chol_genz_f64_dev_noalloc",
 )
 
 
@@ -87,6 +87,8 @@ def lib():
     L.dbg_adapt_low_rank_f64.argtypes = [dp, C.c_int64, C.c_int32, dp, dp, C.c_int32, ip32, C.c_double, ip32, ip32]
     L.dbg_cov_build.argtypes = [C.c_int32, C.c_int32, dp, C.c_int32, vp, vp, C.c_int32, vp, C.c_int64, C.c_int32]
     L.dbg_chol_classic_f64.argtypes = chol + [dp]
+    L.dbg_chol_genz_f64_dev_noalloc.argtypes = [vp, C.c_int64, C.c_int32, vp, vp, C.c_int32, C.c_int32, vp, vp, vp,
+                                                vp]
     L.dbg_plan_create.argtypes = [C.POINTER(vp), C.POINTER(Opts), C.c_int32, dp, C.c_int64, dp, dp]
     L.dbg_plan_destroy.argtypes = [vp]
     L.dbg_plan_set_shifts.argtypes = [vp, vp]

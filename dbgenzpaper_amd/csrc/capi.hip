@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <numeric>
 #include <vector>
+#include <mutex>
 
 #include <rocblas/rocblas.h>
 
@@ -216,6 +217,48 @@ int chol_host(double* M, int64_t lda, int32_t n, double* a, double* b, int32_t b
     B.release();
     (void)hipStreamDestroy(s);
     return rc;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Fixed scratch of the allocation-free factorization: the conditional means
+// s_vec of one pivot window and eps0 (kNoAllocMaxN + 1 doubles, 4 KB), one
+// copy per device for the life of the process.  Host code never passes the
+// symbol itself to a kernel: it takes the device address from
+// hipGetSymbolAddress once per device (noalloc_scratch below).
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocMaxN = 511;
+static __device__ __align__(256) double scratch[kNoAllocMaxN + 1];
+
+__global__ void noalloc_init_kernel(double* s_vec, int n, int32_t* perm)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        s_vec[i] = 0.0;
+        perm[i] = i;
+    }
+}
+
+namespace {
+
+constexpr int kNoAllocMaxDev = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kNoAllocMaxDev] = {};
+
+// device address of `scratch` on the current device (looked up on first use)
+int noalloc_scratch(double** out)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDev) return fail(DBG_ERR_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return DBG_OK;
 }
 
 }  // namespace
@@ -945,6 +988,31 @@ int dbg_cov_build(int32_t kind, int32_t n, const double* params, int32_t k, cons
     (void)hipStreamDestroy(s);
     if (rc) return rc;
     if (e != hipSuccess) return fail(DBG_ERR_DEVICE, std::string("covariance: ") + hipGetErrorString(e));
+    return DBG_OK;
+}
+
+int dbg_chol_genz_f64_dev_noalloc(double* M, int64_t lda, int32_t n, double* a, double* b, int32_t bs1, int32_t bs2,
+                                  int32_t* perm, double* scale, int32_t* rank, void* stream)
+{
+    g_err.clear();
+    if (!M || !a || !b || !perm || !scale || !rank || lda < n) return fail(DBG_ERR_ARG, "invalid matrix arguments");
+    if (n < 1 || n > kNoAllocMaxN)
+        return fail(DBG_ERR_ARG, "the allocation-free factorization takes 1 <= n <= " + std::to_string(kNoAllocMaxN));
+    if (bs1 < 1 || bs2 < n)
+        return fail(DBG_ERR_ARG, "the allocation-free factorization needs one pivot window (chol_block_size2 >= n)");
+    double* sc = nullptr;
+    int rc = noalloc_scratch(&sc);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    double* s_vec = sc;
+    double* eps0 = sc + kNoAllocMaxN;
+    // s_vec = 0 and perm = 0 .. n-1 by a kernel (no memset node under capture)
+    hipLaunchKernelGGL(noalloc_init_kernel, dim3((n + 255) / 256), dim3(256), 0, s, s_vec, n, perm);
+    HIPCHK(hipGetLastError());
+    CholArgs C{M, lda, n, a, b, scale, perm, s_vec, eps0, rank};
+    HIPCHK(chol_standardize(C, false, s));
+    HIPCHK(chol_window(C, 0, n, std::max(1, std::min(bs1, 64)), 1, s));
+    HIPCHK(chol_finish(C, s));
     return DBG_OK;
 }
 

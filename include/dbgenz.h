@@ -164,6 +164,19 @@ int dbg_adapt_low_rank_f64(double* M, int64_t lda, int32_t n, double* a, double*
 int dbg_cov_build(int32_t kind, int32_t n, const double* params, int32_t k, const double* B, const double* psi,
                   int32_t standardize, double* Sigma, int64_t lds, int32_t sigma_on_device);
 
+/* No counterpart in the reference: cholesky_genz! (as dbg_chol_genz_f64) of
+ * a matrix already in device memory, for callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops.  It allocates
+ * nothing: M (n x n, lda), a, b, perm, scale and rank are the caller's device
+ * buffers, factored in place on `stream` (a hipStream_t; NULL for the null
+ * stream) without waiting, and the only scratch, the window's conditional
+ * means and eps0, is a fixed array of the library (512 doubles, one per
+ * device, held until the process ends).  Limits: one pivot window,
+ * 1 <= n <= 511 and chol_block_size2 >= n (larger n or several windows:
+ * DBG_ERR_ARG).  Calls that overlap on one device race on that array. */
+int dbg_chol_genz_f64_dev_noalloc(double* M, int64_t lda, int32_t n, double* a, double* b, int32_t bs1,
+                                  int32_t bs2, int32_t* perm, double* scale, int32_t* rank, void* stream);
+
 /* cholesky_classic!(M, a, b, bs1, bs2, bs3, bs4)  src/cholesky.jl:144-242
  * Unpivoted, in place on host arrays, FP64 on the device. */
 int dbg_chol_classic_f64(double* M, int64_t lda, int32_t n, double* a, double* b, int32_t bs1,
