@@ -67,6 +67,10 @@ def main() -> int:
                     help="only run the gradient leg and print its record (for profiling it alone)")
     ap.add_argument("--batch-only", action="store_true",
                     help="only run the batched leg and print its record (for profiling it alone)")
+    ap.add_argument("--only", choices=("mixed", "fp64", "sparse", "chol"), default=None,
+                    help="only run one leg at the headline shape (mixed or fp64 dense job, the banded d = 4000 "
+                         "sparse job in both precisions, or the d = 2500 Cholesky alone) and print its timing, "
+                         "for profiling it alone")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the other configurations (d = 16, d = 500, Sobol at d = 2500, banded d = 4000)")
     ap.add_argument("--contraction", default="auto", choices=("auto", "fp32", "split16"),
@@ -95,6 +99,9 @@ def main() -> int:
         return 0
     if args.batch_only:
         print(json.dumps({"batched": batch_leg(args.steps)}), flush=True)
+        return 0
+    if args.only:
+        print(json.dumps(single_leg(args.only, args.steps, args.warmup, args.d, args.m, args.reps)), flush=True)
         return 0
 
     d, m, R = args.d, args.m, args.reps
@@ -224,6 +231,45 @@ def main() -> int:
 
     print(json.dumps(out), flush=True)
     return 0
+
+
+def single_leg(which, steps, warmup, d, m, R):
+    """One leg alone: the mixed or FP64 dense job (factorization, integration,
+    reductions) of the headline shape, the banded d = 4000 sparse job (mixed
+    then FP64), or the headline shape's device Cholesky (refactor only)."""
+    import numpy as np
+
+    import dbgenzpaper_amd as dbg
+    from dbgenzpaper_amd import workloads as W
+
+    if which == "sparse":
+        P = W.banded(4000, 16, m=m, n_reps=R, seed=2)
+        out = {}
+        for name, T, form in (("mixed", np.float32, "tail"), ("fp64", np.float64, "erf")):
+            opts = dbg.QMC_opts(T, m=P.m, n_reps=P.n_reps, max_pts=P.m, max_abs_err=0.0)
+            D = dbg.QMCDataSparse(P.Sigma, P.a, P.b, opts, None, "Richtmyer", shifts=P.shifts, phi_form=form)
+            step, kms, sm = _time_loop(D, steps)
+            out[name] = {"step_ms": step, "kernel_ms": kms, "log_p": sm.log_value}
+            D.close()
+        return {"leg": "banded d=4000 bw=16", **out}
+    P = W.factor_model(d, 8, m=m, n_reps=R, seed=1)
+    T = np.float64 if which in ("fp64", "chol") else np.float32
+    opts = dbg.QMC_opts(T, m=m, n_reps=R, max_pts=m, max_abs_err=0.0)
+    D = dbg.QMCData(P.Sigma, P.a, P.b, opts, None, "Richtmyer", shifts=P.shifts, phi_form="tail")
+    if which == "chol":
+        for _ in range(warmup):
+            D.refactor()
+        D.sync()
+        t0 = time.perf_counter()
+        for _ in range(max(1, steps)):
+            D.refactor()
+        D.sync()
+        res = {"leg": f"cholesky_genz! d={d} (refactor)", "ms": (time.perf_counter() - t0) / max(1, steps) * 1e3}
+    else:
+        step, kms, sm = _time_loop(D, steps)
+        res = {"leg": f"{which} dense d={d}", "step_ms": step, "kernel_ms": kms, "log_p": sm.log_value}
+    D.close()
+    return res
 
 
 def batch_leg(steps):
@@ -439,20 +485,26 @@ def grad_leg(steps):
 
 
 def copy_calibration(nbytes):
-    """A plain device-to-device copy of nbytes (torch hands it to the HIP
-    runtime's blit kernel), run three times: under rocprofv3 --pmc the summed
-    FETCH_SIZE and WRITE_SIZE of its dispatches give the counters' scale
-    against known bytes."""
-    import torch
+    """A plain device-to-device copy of nbytes by the HIP runtime (its blit
+    kernel), run three times: under rocprofv3 --pmc the summed FETCH_SIZE and
+    WRITE_SIZE of its dispatches give the counters' scale against known
+    bytes."""
+    import ctypes
 
-    n = nbytes // 4
-    x = torch.ones(n, dtype=torch.float32, device="cuda")
-    y = torch.empty_like(x)
-    torch.cuda.synchronize()
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    x, y = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(x), nbytes) == 0 and hip.hipMalloc(ctypes.byref(y), nbytes) == 0
+    assert hip.hipMemset(x, 1, nbytes) == 0
     for _ in range(3):
-        y.copy_(x)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes": n * 4, "check": float(y[-1].item())}), flush=True)
+        assert hip.hipMemcpy(y, x, nbytes, 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(x)
+    hip.hipFree(y)
+    print(json.dumps({"copy_bytes": nbytes, "copies": 3}), flush=True)
     return 0
 
 
