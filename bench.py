@@ -459,7 +459,9 @@ def grad_leg(steps):
         gm.append(D.last_kernel_ms())
     fwd = sum(fw) / len(fw)
     grd = sum(gm) / len(gm)
-    _log(f"adjoint d={d}: forward kernel {fwd:.1f} ms, gradient {grd:.1f} ms, log P {g.log_value:.4f}")
+    chunk, n_chunks, dev_bytes = D.grad_info()
+    _log(f"adjoint d={d}: forward kernel {fwd:.1f} ms, gradient {grd:.1f} ms, log P {g.log_value:.4f}, "
+         f"{n_chunks} chunks of {chunk} samples, {dev_bytes / 2**30:.2f} GiB")
     D.close()
     # the forward with the FP32 matrix-core contraction (the arithmetic of the
     # gradient's own record and reverse sweeps)
@@ -481,6 +483,8 @@ def grad_leg(steps):
             "forward_kernel_ms": fwd, "gradient_ms": grd, "ratio": grd / fwd,
             "adjoint_only_ms": grd - fwd, "adjoint_only_ratio": (grd - fwd) / fwd, "log_p": g.log_value,
             "forward_kernel_ms_fp32_contraction": fwd32, "ratio_vs_fp32_contraction_forward": grd / fwd32,
+            "record_chunk_samples": chunk, "record_chunks": n_chunks, "device_bytes": dev_bytes,
+            "records_unchunked_bytes": 2 * (d - 1) * m * R * 4,
             "grad_b_norm": float(np.linalg.norm(g.b)), "grad_sigma_norm": float(np.linalg.norm(g.Sigma))}
 
 

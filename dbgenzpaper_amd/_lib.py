@@ -34,7 +34,7 @@ EXPORTS = (
     "dbg_plan_last_kernel_ms", "dbg_plan_summary", "dbg_plan_get_factor", "dbg_plan_get_reps",
     "dbg_plan_get_state", "dbg_plan_set_state", "dbg_plan_refactor", "dbg_batch_create",
     "dbg_batch_run", "dbg_batch_last_ms", "dbg_batch_tuning", "dbg_batch_destroy", "dbg_qmc_grad",
-    "dbg_plan_refine_info", "dbg_chol_genz_f64_dev_noalloc",
+    "dbg_plan_refine_info", "dbg_chol_genz_f64_dev_noalloc", "dbg_plan_set_grad_budget", "dbg_plan_grad_info",
 )
 
 
@@ -104,6 +104,8 @@ def lib():
     L.dbg_plan_set_state.argtypes = [vp, C.c_int64]
     L.dbg_plan_refactor.argtypes = [vp]
     L.dbg_plan_refine_info.argtypes = [vp, ip32, dp, C.POINTER(C.c_float)]
+    L.dbg_plan_set_grad_budget.argtypes = [vp, C.c_int64]
+    L.dbg_plan_grad_info.argtypes = [vp, i64p, ip32, i64p]
     L.dbg_batch_create.argtypes = [C.POINTER(vp), C.POINTER(Opts), C.c_int32, ip32, dp, dp, dp, dp]
     L.dbg_batch_run.argtypes = [vp, dp]
     L.dbg_batch_last_ms.argtypes = [vp, C.POINTER(C.c_float)]

@@ -8,7 +8,9 @@
 //                  so the reverse recursion ybar_l = sum_{r>l} U[l, r] sbar_r
 //                  is the forward contraction in flipped index order;
 //   grad_norm      E = max exponent of the tile sums, 1 / sum(p 2^-E) and
-//                  log P of the recorded forward;
+//                  log P of the recorded forward (one chunk of samples);
+//   grad_fold_*    the weights with which a chunk's gradient joins those of
+//                  the chunks before it (each chunk's share of the sum);
 //   ubar_product   Ubar[l, g+1] = sum_s y_l sbar_{g+1} (l <= g), split-K on
 //                  the matrix cores with FP64 sums of the chunk partials;
 //   tril_half      Phi(X): lower triangle, halved diagonal (Cholesky adjoint);
@@ -45,7 +47,7 @@ hipError_t pack_flip(const double* U, int n, bool mixed, void* Ufk, int64_t ldu,
 }
 
 // norm[0] = E, norm[1] = 1 / S with S = sum_t part_m[t] 2^(part_e[t] - E);
-// norm[2] = log P = log(S / total) + E log 2.  One block.
+// norm[2] = log P = log(S / total) + E log 2, norm[3] = S.  One block.
 __global__ __launch_bounds__(256) void grad_norm_kernel(const double* __restrict__ pm, const int* __restrict__ pe,
                                                         int64_t n_tiles, double total, double* norm)
 {
@@ -83,12 +85,66 @@ __global__ __launch_bounds__(256) void grad_norm_kernel(const double* __restrict
         norm[0] = (double)E;
         norm[1] = S > 0.0 ? 1.0 / S : 0.0;
         norm[2] = S > 0.0 ? log(S / total) + E * 0.69314718055994530942 : -INFINITY;
+        norm[3] = S;
     }
 }
 
 hipError_t grad_norm(const double* pm, const int* pe, int64_t n_tiles, double total, double* norm, hipStream_t s)
 {
     hipLaunchKernelGGL(grad_norm_kernel, dim3(1), dim3(256), 0, s, pm, pe, n_tiles, total, norm);
+    return hipGetLastError();
+}
+
+// The gradient of log P over all samples is the weighted mean of the chunks'
+// gradients (each normalised by its own sum), with weights W_c / W, W_c =
+// S_c 2^E_c: with the running sums normalised by W_r of the chunks before,
+// the new running sum is alpha (old) + beta (chunk), alpha = W_r / (W_r + W_c),
+// beta = W_c / (W_r + W_c).  run = {E_r, S_r} (S_r in units of 2^E_r).
+__global__ void grad_fold_norm_kernel(double* norm, double* run, double* fold, int first, int last, double total)
+{
+    if (threadIdx.x != 0) return;
+    const double Ec = norm[0], Sc = norm[3];
+    double Er = first ? Ec : run[0], Sr = first ? 0.0 : run[1];
+    double alpha = 0.0, beta = 0.0;
+    if (Sr <= 0.0) {
+        Er = Ec;
+        Sr = Sc;
+        beta = Sc > 0.0 ? 1.0 : 0.0;
+    } else if (Sc > 0.0) {
+        const double E = fmax(Er, Ec);
+        const double u = ldexp(Sr, (int)(Er - E)), v = ldexp(Sc, (int)(Ec - E));
+        Er = E;
+        Sr = u + v;
+        alpha = u / Sr;
+        beta = v / Sr;
+    } else {
+        alpha = 1.0;
+    }
+    run[0] = Er;
+    run[1] = Sr;
+    fold[0] = alpha;
+    fold[1] = beta;
+    if (last) norm[2] = Sr > 0.0 ? log(Sr / total) + Er * 0.69314718055994530942 : -INFINITY;
+}
+
+hipError_t grad_fold_norm(double* norm, double* run, double* fold, int first, int last, double total,
+                          hipStream_t s)
+{
+    hipLaunchKernelGGL(grad_fold_norm_kernel, dim3(1), dim3(64), 0, s, norm, run, fold, first, last, total);
+    return hipGetLastError();
+}
+
+__global__ void grad_fold_acc_kernel(double* __restrict__ acc, const double* __restrict__ cacc, int len,
+                                     const double* __restrict__ fold)
+{
+    const double alpha = fold[0], beta = fold[1];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < len; i += gridDim.x * blockDim.x)
+        acc[i] = alpha * acc[i] + beta * cacc[i];
+}
+
+hipError_t grad_fold_acc(double* acc, const double* cacc, int len, const double* fold, hipStream_t s)
+{
+    hipLaunchKernelGGL(grad_fold_acc_kernel, dim3((len + 255) / 256), dim3(256), 0, s, acc, cacc, len, fold);
     return hipGetLastError();
 }
 
@@ -465,48 +521,53 @@ __global__ __launch_bounds__(UbSplitCfg<TT>::NT, UbSplitCfg<TT>::OCC) void ubar_
                     tot[i][j][r];
 }
 
-// Ub[l, g + 1] += sum over chunks of the partial tiles (l <= g < nd), FP64
+// Ub[l, g + 1] = fold[0] Ub[l, g + 1] + fold[1] (sum over split-K chunks of
+// the partial tiles), l <= g < nd, FP64; one element per thread
 template <typename T, int TT>
 __global__ __launch_bounds__(256) void ubar_reduce_kernel(const T* __restrict__ part, const int2* __restrict__ pairs,
                                                           int n_pairs, int n_chunks, int nd, double* __restrict__ Ub,
-                                                          int n)
+                                                          int n, const double* __restrict__ fold)
 {
     const int pair = blockIdx.x;
     const int2 tp = pairs[pair];
-    for (int e = blockIdx.y * 256 + threadIdx.x; e < TT * TT; e += gridDim.y * 256) {
-        const int i = e / TT, j = e - i * TT;
-        const int l = tp.x * TT + i, g = tp.y * TT + j;
-        if (l >= nd || g >= nd || l > g) continue;
-        double sum = 0.0;
-        for (int kc = 0; kc < n_chunks; ++kc) sum += (double)part[((size_t)kc * n_pairs + pair) * (TT * TT) + e];
-        Ub[(size_t)l + (size_t)(g + 1) * n] += sum;
-    }
+    const int e = blockIdx.y * 256 + threadIdx.x;
+    if (e >= TT * TT) return;
+    const int i = e / TT, j = e - i * TT;
+    const int l = tp.x * TT + i, g = tp.y * TT + j;
+    if (l >= nd || g >= nd || l > g) return;
+    double sum = 0.0;
+    for (int kc = 0; kc < n_chunks; ++kc) sum += (double)part[((size_t)kc * n_pairs + pair) * (TT * TT) + e];
+    double& u = Ub[(size_t)l + (size_t)(g + 1) * n];
+    u = fold[0] * u + fold[1] * sum;
 }
 
-hipError_t ubar_product(const void* Y, const void* S, int64_t rs, int nd, bool mixed, const int2* pairs,
-                        int n_pairs, int n_chunks, int64_t kchunk, void* part, double* Ub, int n, hipStream_t s,
-                        bool split16)
+hipError_t ubar_product(const void* Y, const void* S, int64_t rs, int64_t kmax, int nd, bool mixed,
+                        const int2* pairs, int n_pairs, int n_chunks, int64_t kchunk, void* part, double* Ub, int n,
+                        const double* fold, hipStream_t s, bool split16)
 {
+    if (kmax > rs) return hipErrorInvalidValue;
     dim3 g1(n_pairs, n_chunks), b(256);
-    dim3 g2(n_pairs, 16);
     if (mixed) {
-        if (kchunk % kUbK != 0 || rs % kUbK != 0) return hipErrorInvalidValue;
+        if (kchunk % kUbK != 0 || rs % kUbK != 0 || kmax % kUbK != 0) return hipErrorInvalidValue;
         if (split16) {
-            hipLaunchKernelGGL(ubar_gemm_split_kernel<kUbTile16>, g1, dim3(UbSplitCfg<kUbTile16>::NT), 0, s,
-                               (const float*)Y, (const float*)S, rs, nd, pairs, n_pairs, kchunk, rs, (float*)part);
-            hipLaunchKernelGGL((ubar_reduce_kernel<float, kUbTile16>), g2, b, 0, s, (const float*)part, pairs,
-                               n_pairs, n_chunks, nd, Ub, n);
+            constexpr int TT = kUbTile16;
+            hipLaunchKernelGGL(ubar_gemm_split_kernel<TT>, g1, dim3(UbSplitCfg<TT>::NT), 0, s, (const float*)Y,
+                               (const float*)S, rs, nd, pairs, n_pairs, kchunk, kmax, (float*)part);
+            hipLaunchKernelGGL((ubar_reduce_kernel<float, TT>), dim3(n_pairs, TT * TT / 256), b, 0, s,
+                               (const float*)part, pairs, n_pairs, n_chunks, nd, Ub, n, fold);
         } else {
+            constexpr int TT = kUbTile;
             hipLaunchKernelGGL(ubar_gemm_lds_kernel, g1, b, 0, s, (const float*)Y, (const float*)S, rs, nd, pairs,
-                               n_pairs, kchunk, rs, (float*)part);
-            hipLaunchKernelGGL((ubar_reduce_kernel<float, kUbTile>), g2, b, 0, s, (const float*)part, pairs,
-                               n_pairs, n_chunks, nd, Ub, n);
+                               n_pairs, kchunk, kmax, (float*)part);
+            hipLaunchKernelGGL((ubar_reduce_kernel<float, TT>), dim3(n_pairs, TT * TT / 256), b, 0, s,
+                               (const float*)part, pairs, n_pairs, n_chunks, nd, Ub, n, fold);
         }
     } else {
+        constexpr int TT = kUbTile;
         hipLaunchKernelGGL(ubar_gemm_kernel<double>, g1, b, 0, s, (const double*)Y, (const double*)S, rs, nd,
-                           pairs, n_pairs, kchunk, rs, (double*)part);
-        hipLaunchKernelGGL((ubar_reduce_kernel<double, kUbTile>), g2, b, 0, s, (const double*)part, pairs, n_pairs,
-                           n_chunks, nd, Ub, n);
+                           pairs, n_pairs, kchunk, kmax, (double*)part);
+        hipLaunchKernelGGL((ubar_reduce_kernel<double, TT>), dim3(n_pairs, TT * TT / 256), b, 0, s,
+                           (const double*)part, pairs, n_pairs, n_chunks, nd, Ub, n, fold);
     }
     return hipGetLastError();
 }

@@ -345,17 +345,25 @@ struct dbg_plan {
     // adjoint buffers (allocated on the first dbg_qmc_grad)
     struct Grad {
         bool ready = false;
+        int64_t budget = kGradRecordBudget;  // bytes of y and s records
+        int64_t bytes = 0;      // device memory held by the buffers below
         int tile = 0, grid = 0;
-        int64_t tiles_per_rep = 0, n_tiles = 0, rs = 0;
-        void* yrec = nullptr;   // nd x rs (T): y of every sample
+        int64_t tiles_per_rep = 0, n_tiles = 0;
+        // the samples run in chunks of chunk_tiles tiles (the last may be
+        // shorter), each recorded, swept back and folded into the sums
+        int64_t chunk_tiles = 0, rs = 0;  // rs = chunk_tiles * tile
+        int n_schunks = 0;
+        void* yrec = nullptr;   // nd x rs (T): y of every sample of a chunk
         void* srec = nullptr;   // nd x rs (T): s_{g+1}, then sbar_{g+1}
         void* wm = nullptr;     // rs (T)
         int* we = nullptr;      // rs
         void* ywork = nullptr;  // grid x nd x tile (T): reverse workspace
-        double* part_m = nullptr;
+        double* part_m = nullptr;  // chunk_tiles
         int* part_e = nullptr;
-        double* norm = nullptr; // E, 1/S, log P
-        double* acc = nullptr;  // 3 n: d log P / d a', b', U_rr
+        double* norm = nullptr; // E, 1/S, log P, S of a chunk (log P of all at the end)
+        double* run = nullptr;  // E, S of the chunks folded so far; alpha, beta
+        double* acc = nullptr;  // 3 n: d log P / d a', b', U_rr (all chunks)
+        double* cacc = nullptr; // 3 n: the same for one chunk
         void* ufk = nullptr;    // flipped factor, packed rows (T)
         void* ufc = nullptr;    // flipped factor, column-major (T)
         void* uc16f = nullptr;  // flipped factor, split FP16 layout (mixed split path)
@@ -369,13 +377,21 @@ struct dbg_plan {
         double* gab = nullptr;  // 2 n: d log P / d a, b
         std::vector<void*> all() const
         {
-            return {yrec, srec, wm, we, ywork, part_m, part_e, norm, acc, ufk, ufc, uc16f, part, pairs, ub, x, g, gab};
+            return {yrec, srec, wm, we, ywork, part_m, part_e, norm, run, acc, cacc, ufk, ufc, uc16f, part, pairs,
+                    ub, x, g, gab};
+        }
+        void free_all()
+        {
+            for (void* p : all()) (void)hipFree(p);
+            const int64_t keep = budget;
+            *this = Grad{};
+            budget = keep;
         }
     } G;
 
     void release()
     {
-        for (void* p : G.all()) (void)hipFree(p);
+        G.free_all();
         for (void* p : R.all()) (void)hipFree(p);
         if (R.ev0) (void)hipEventDestroy(R.ev0);
         if (R.ev1) (void)hipEventDestroy(R.ev1);
@@ -694,27 +710,42 @@ int grad_alloc(dbg_plan* P)
     G.tile = dense_mfma_tile(P->mixed);
     G.tiles_per_rep = (P->o.m + G.tile - 1) / G.tile;
     G.n_tiles = G.tiles_per_rep * P->o.n_reps;
-    G.rs = G.n_tiles * G.tile;
-    G.grid = (int)std::min<int64_t>(G.n_tiles, (int64_t)ncu * 2);
-    const size_t rec = (size_t)nd * G.rs * T;
-    HIPCHK(dalloc(&G.yrec, rec));
-    HIPCHK(dalloc(&G.srec, rec));
-    HIPCHK(dalloc(&G.wm, (size_t)G.rs * T));
-    HIPCHK(dalloc(&G.we, (size_t)G.rs * sizeof(int)));
-    HIPCHK(dalloc(&G.ywork, (size_t)G.grid * nd * G.tile * T));
-    HIPCHK(dalloc(&G.part_m, (size_t)G.n_tiles * sizeof(double)));
-    HIPCHK(dalloc(&G.part_e, (size_t)G.n_tiles * sizeof(int)));
-    HIPCHK(dalloc(&G.norm, 4 * sizeof(double)));
-    HIPCHK(dalloc(&G.acc, (size_t)3 * n * sizeof(double)));
-    HIPCHK(dalloc(&G.ufk, (size_t)nd * P->ldu * T));
-    HIPCHK(dalloc(&G.ufc, (size_t)n * n * T + 256 * T));
-    if (P->split)
-        HIPCHK(dalloc(&G.uc16f, (size_t)split16_cols(n) * (size_t)split16_col_stride(n) * sizeof(uint16_t)));
+    // chunks of equal size (whole tiles) whose y and s records fit the budget
     {
-        // split-K layout of Ubar = Y^T Sbar: upper-triangle pairs of row tiles
-        // (128 rows, two resident workgroups per CU; 256 rows and one for the
-        // split form), chunks in whole rounds of the resident slots (pairs x
-        // chunks a multiple of the slots: no short last round)
+        const int64_t per_tile = 2 * (int64_t)nd * G.tile * (int64_t)T;
+        const int64_t fit = std::max<int64_t>(1, G.budget / per_tile);
+        G.n_schunks = (int)((G.n_tiles + fit - 1) / fit);
+        G.chunk_tiles = (G.n_tiles + G.n_schunks - 1) / G.n_schunks;
+    }
+    G.rs = G.chunk_tiles * G.tile;
+    G.grid = (int)std::min<int64_t>(G.chunk_tiles, (int64_t)ncu * 2);
+    int64_t total = 0;
+    auto alloc = [&](auto** p, size_t bytes) {
+        total += (int64_t)bytes;
+        return dalloc(p, bytes);
+    };
+    const size_t rec = (size_t)nd * G.rs * T;
+    HIPCHK(alloc(&G.yrec, rec));
+    HIPCHK(alloc(&G.srec, rec));
+    HIPCHK(alloc(&G.wm, (size_t)G.rs * T));
+    HIPCHK(alloc(&G.we, (size_t)G.rs * sizeof(int)));
+    if (!P->split) HIPCHK(alloc(&G.ywork, (size_t)G.grid * nd * G.tile * T));
+    HIPCHK(alloc(&G.part_m, (size_t)G.chunk_tiles * sizeof(double)));
+    HIPCHK(alloc(&G.part_e, (size_t)G.chunk_tiles * sizeof(int)));
+    HIPCHK(alloc(&G.norm, 4 * sizeof(double)));
+    HIPCHK(alloc(&G.run, 4 * sizeof(double)));
+    HIPCHK(alloc(&G.acc, (size_t)3 * n * sizeof(double)));
+    HIPCHK(alloc(&G.cacc, (size_t)3 * n * sizeof(double)));
+    HIPCHK(alloc(&G.ufk, (size_t)nd * P->ldu * T));
+    HIPCHK(alloc(&G.ufc, (size_t)n * n * T + 256 * T));
+    if (P->split)
+        HIPCHK(alloc(&G.uc16f, (size_t)split16_cols(n) * (size_t)split16_col_stride(n) * sizeof(uint16_t)));
+    {
+        // split-K layout of Ubar = Y^T Sbar over one chunk: upper-triangle
+        // pairs of row tiles (128 rows, two resident workgroups per CU; 256
+        // rows and one for the split form), split-K chunks in whole rounds of
+        // the resident slots (pairs x chunks a multiple of the slots: no
+        // short last round)
         const int tb = ubar_tile(P->split);
         const int nt = (nd + tb - 1) / tb;
         std::vector<int2> pr;
@@ -730,14 +761,15 @@ int grad_alloc(dbg_plan* P)
         want = (want + step - 1) / step * step;
         G.kchunk = std::max<int64_t>(32, ((G.rs + want - 1) / want + 31) / 32 * 32);
         G.n_chunks = (int)((G.rs + G.kchunk - 1) / G.kchunk);
-        HIPCHK(dalloc(&G.pairs, pr.size() * sizeof(int2)));
+        HIPCHK(alloc(&G.pairs, pr.size() * sizeof(int2)));
         HIPCHK(hipMemcpy(G.pairs, pr.data(), pr.size() * sizeof(int2), hipMemcpyHostToDevice));
-        HIPCHK(dalloc(&G.part, (size_t)G.n_chunks * G.n_pairs * tb * tb * T));
+        HIPCHK(alloc(&G.part, (size_t)G.n_chunks * G.n_pairs * tb * tb * T));
     }
-    HIPCHK(dalloc(&G.ub, (size_t)n * n * sizeof(double)));
-    HIPCHK(dalloc(&G.x, (size_t)n * n * sizeof(double)));
-    HIPCHK(dalloc(&G.g, (size_t)n * n * sizeof(double)));
-    HIPCHK(dalloc(&G.gab, (size_t)2 * n * sizeof(double)));
+    HIPCHK(alloc(&G.ub, (size_t)n * n * sizeof(double)));
+    HIPCHK(alloc(&G.x, (size_t)n * n * sizeof(double)));
+    HIPCHK(alloc(&G.g, (size_t)n * n * sizeof(double)));
+    HIPCHK(alloc(&G.gab, (size_t)2 * n * sizeof(double)));
+    G.bytes = total;
     G.ready = true;
     return DBG_OK;
 }
@@ -759,13 +791,13 @@ int run_grad(dbg_plan* P)
     HIPCHK(hipMemsetAsync(G.acc, 0, (size_t)3 * n * sizeof(double), s));
     HIPCHK(hipMemsetAsync(G.ub, 0, (size_t)n * n * sizeof(double), s));
     HIPCHK(pack_flip(P->F.M, n, P->mixed, G.ufk, P->ldu, G.ufc, s));
+    if (P->split) HIPCHK(pack_split16((const float*)G.ufc, n, G.uc16f, s));
 
     DenseArgs A{};
     A.n = n;
     A.m = o.m;
     A.n_reps = o.n_reps;
     A.tiles_per_rep = G.tiles_per_rep;
-    A.n_tiles = G.n_tiles;
     A.ldu = P->ldu;
     A.ldc = n;
     A.ak = P->ak;
@@ -773,7 +805,6 @@ int run_grad(dbg_plan* P)
     A.invk = P->invk;
     for (int q = 0; q < 6; ++q) A.first[q] = P->first_h[q];
     A.pts = point_args(P, P->state);
-    A.y_slot = (int64_t)nd * G.tile;
     A.part_m = G.part_m;
     A.part_e = G.part_e;
     A.rs = G.rs;
@@ -782,42 +813,52 @@ int run_grad(dbg_plan* P)
     A.wm = G.wm;
     A.we = G.we;
     A.norm = G.norm;
-    A.ga = G.acc;
-    A.gb = G.acc + n;
-    A.gd = G.acc + 2 * n;
-    // forward with records (tail form)
-    A.mode = kModeRecord;
-    A.Uk = P->Uk;
-    A.Ucol = P->mixed ? (const void*)P->ucol32 : (const void*)P->F.M;
-    if (P->split) {
-        // the split-FP16 kernel: its own y workspace for the contraction, FP32 records
-        A.Y = P->Y;
-        A.y_slot = P->y_slot;
-        HIPCHK(launch_dense_split(A, gen, P->uc16, P->grid, s));  // P->Y holds P->grid slots
-        A.y_slot = (int64_t)nd * G.tile;
-    } else {
-        A.Y = G.yrec;
-        HIPCHK(launch_dense_adjoint(A, P->mixed, gen, G.grid, s));
-    }
-    HIPCHK(grad_norm(G.part_m, G.part_e, G.n_tiles, (double)o.m * o.n_reps, G.norm, s));
-    // reverse sweep in flipped index order
-    A.mode = kModeReverse;
+    A.ga = G.cacc;
+    A.gb = G.cacc + n;
+    A.gd = G.cacc + 2 * n;
     A.a_inf = P->a_inf ? 1 : 0;
-    A.Uk = G.ufk;
-    A.Ucol = G.ufc;
-    if (P->split) {
-        // the split-FP16 kernel on the flipped factor
-        HIPCHK(pack_split16((const float*)G.ufc, n, G.uc16f, s));
-        A.Y = P->Y;
-        A.y_slot = P->y_slot;
-        HIPCHK(launch_dense_split(A, gen, G.uc16f, P->grid, s));
-    } else {
-        A.Y = G.ywork;
-        HIPCHK(launch_dense_adjoint(A, P->mixed, gen, G.grid, s));
+    const double total = (double)o.m * o.n_reps;
+    for (int c = 0; c < G.n_schunks; ++c) {
+        A.tile0 = (int64_t)c * G.chunk_tiles;
+        A.n_tiles = std::min<int64_t>(G.chunk_tiles, G.n_tiles - A.tile0);
+        const int grid = (int)std::min<int64_t>(A.n_tiles, G.grid);
+        HIPCHK(hipMemsetAsync(G.cacc, 0, (size_t)3 * n * sizeof(double), s));
+        // forward with records (tail form)
+        A.mode = kModeRecord;
+        A.Uk = P->Uk;
+        A.Ucol = P->mixed ? (const void*)P->ucol32 : (const void*)P->F.M;
+        if (P->split) {
+            // the split-FP16 kernel: its own y workspace for the contraction, FP32 records
+            A.Y = P->Y;
+            A.y_slot = P->y_slot;  // P->Y holds P->grid slots
+            HIPCHK(launch_dense_split(A, gen, P->uc16, std::min(grid, P->grid), s));
+        } else {
+            A.Y = G.yrec;
+            A.y_slot = (int64_t)nd * G.tile;
+            HIPCHK(launch_dense_adjoint(A, P->mixed, gen, grid, s));
+        }
+        HIPCHK(grad_norm(G.part_m, G.part_e, A.n_tiles, total, G.norm, s));
+        HIPCHK(grad_fold_norm(G.norm, G.run, G.run + 2, c == 0, c + 1 == G.n_schunks, total, s));
+        // reverse sweep in flipped index order
+        A.mode = kModeReverse;
+        A.Uk = G.ufk;
+        A.Ucol = G.ufc;
+        if (P->split) {
+            // the split-FP16 kernel on the flipped factor
+            A.Y = P->Y;
+            A.y_slot = P->y_slot;
+            HIPCHK(launch_dense_split(A, gen, G.uc16f, std::min(grid, P->grid), s));
+        } else {
+            A.Y = G.ywork;
+            A.y_slot = (int64_t)nd * G.tile;
+            HIPCHK(launch_dense_adjoint(A, P->mixed, gen, grid, s));
+        }
+        HIPCHK(grad_fold_acc(G.acc, G.cacc, 3 * n, G.run + 2, s));
+        // Ubar[l, g + 1] = sum_samples y_l sbar_{g+1}: split-K on the matrix
+        // cores, FP64 chunk sums, folded into the running Ubar
+        HIPCHK(ubar_product(G.yrec, G.srec, G.rs, A.n_tiles * G.tile, nd, P->mixed, G.pairs, G.n_pairs, G.n_chunks,
+                            G.kchunk, G.part, G.ub, n, G.run + 2, s, P->split));
     }
-    // Ubar[l, g + 1] = sum_samples y_l sbar_{g+1}: split-K on the matrix cores, FP64 chunk sums
-    HIPCHK(ubar_product(G.yrec, G.srec, G.rs, nd, P->mixed, G.pairs, G.n_pairs, G.n_chunks, G.kchunk, G.part,
-                        G.ub, n, s, P->split));
     rocblas_handle h = (rocblas_handle)blas_handle_void(s);
     if (!h) return fail(DBG_ERR_DEVICE, "rocBLAS handle");
     HIPCHK(sigma_chain(G.ub, n, G.acc + 2 * n, nullptr, s, 0, nullptr, nullptr, nullptr, nullptr, nullptr,
@@ -1182,6 +1223,31 @@ int dbg_qmc_grad(dbg_plan* P, double* log_value, double* ga, double* gb, double*
     if (gU) HIPCHK(hipMemcpyAsync(gU, G.ub, nn, hipMemcpyDeviceToHost, P->stream));
     HIPCHK(hipStreamSynchronize(P->stream));
     if (log_value) *log_value = norm[2];
+    return DBG_OK;
+}
+
+int dbg_plan_set_grad_budget(dbg_plan* P, int64_t record_bytes)
+{
+    g_err.clear();
+    if (!P) return fail(DBG_ERR_ARG, "null plan");
+    if (record_bytes < 0) return fail(DBG_ERR_ARG, "the gradient budget is negative");
+    const int64_t b = record_bytes == 0 ? kGradRecordBudget : record_bytes;
+    if (P->G.budget != b) {
+        HIPCHK(hipStreamSynchronize(P->stream));
+        P->G.free_all();
+        P->G.budget = b;
+    }
+    return DBG_OK;
+}
+
+int dbg_plan_grad_info(dbg_plan* P, int64_t* chunk_samples, int32_t* n_chunks, int64_t* device_bytes)
+{
+    g_err.clear();
+    if (!P) return fail(DBG_ERR_ARG, "null plan");
+    const auto& G = P->G;
+    if (chunk_samples) *chunk_samples = G.ready ? G.rs : 0;
+    if (n_chunks) *n_chunks = G.ready ? G.n_schunks : 0;
+    if (device_bytes) *device_bytes = G.ready ? G.bytes : 0;
     return DBG_OK;
 }
 

@@ -48,9 +48,13 @@ struct DenseArgs {
     // optional list of the tiles to run (FP64 integration kernels): entry i of
     // n_tiles names tile tile_list[i] of the round, entries < 0 are skipped
     const int64_t* tile_list;
-    // adjoint support (matrix-core kernel only)
+    // adjoint support (matrix-core kernels only)
     int mode;                  // kModeIntegrate, kModeRecord or kModeReverse
-    int64_t rs;                // row stride of the records: n_tiles * tile samples
+    // record / reverse: the launch runs tiles tile0 .. tile0 + n_tiles - 1 of
+    // the round (one chunk of samples); records, weights and partial sums
+    // are indexed from the chunk's first tile
+    int64_t tile0;
+    int64_t rs;                // row stride of the records: chunk tiles * tile samples
     void* srec;                // record: s_{g+1} per (g, sample); reverse: replaced by sbar_{g+1}
     void* yrec;                // record: y_g per (g, sample) (the workspace itself in record mode)
     void* wm;                  // record: per-sample weight mantissa (T)
@@ -96,10 +100,20 @@ hipError_t launch_dense_mfma(const DenseArgs& A, bool mixed, int gen, int form, 
 hipError_t launch_dense_adjoint(const DenseArgs& A, bool mixed, int gen, int grid, hipStream_t s);
 hipError_t pack_flip(const double* U, int n, bool mixed, void* Ufk, int64_t ldu, void* Ufc, hipStream_t s);
 hipError_t grad_norm(const double* pm, const int* pe, int64_t n_tiles, double total, double* norm, hipStream_t s);
+// folds one chunk of samples into the running gradient: norm (E, 1/S, ., S of
+// the chunk) and run (E, S of the chunks before) give fold = {alpha, beta},
+// the weights of the running sums and of the chunk's; with last != 0 also
+// log P of all chunks into norm[2]
+hipError_t grad_fold_norm(double* norm, double* run, double* fold, int first, int last, double total,
+                          hipStream_t s);
+// acc[i] = fold[0] acc[i] + fold[1] cacc[i], i < len
+hipError_t grad_fold_acc(double* acc, const double* cacc, int len, const double* fold, hipStream_t s);
+// Ub[l, g+1] = fold[0] Ub[l, g+1] + fold[1] sum_s Y[l, s] S[g, s] over the
+// first kmax samples of rows of stride rs (l <= g < nd).
 // split16: the mixed product on FP16 matrix cores (high/low split, per-row scales of S)
-hipError_t ubar_product(const void* Y, const void* S, int64_t rs, int nd, bool mixed, const int2* pairs,
-                        int n_pairs, int n_chunks, int64_t kchunk, void* part, double* Ub, int n, hipStream_t s,
-                        bool split16);
+hipError_t ubar_product(const void* Y, const void* S, int64_t rs, int64_t kmax, int nd, bool mixed,
+                        const int2* pairs, int n_pairs, int n_chunks, int64_t kchunk, void* part, double* Ub, int n,
+                        const double* fold, hipStream_t s, bool split16);
 int ubar_tile(bool split16);        // output tile of the Ubar product
 int ubar_wgs_per_cu(bool split16);  // its resident workgroups per CU
 // step 0: Ubar diagonal; 1: Phi(X); 2: C_bar -> G, ga, gb
@@ -123,6 +137,9 @@ int64_t dense_split_slot(int nd);   // y workspace floats per workgroup slot
 constexpr int kMfmaMinDim = 96;
 // FP64 tiles re-evaluated per shift by a mixed dense plan unless its options say otherwise
 constexpr int kDefaultRefineTiles = 8;
+// default device memory for the adjoint's y and s records (bytes); larger
+// jobs run in chunks of samples
+constexpr int64_t kGradRecordBudget = int64_t(4) << 30;
 
 // ring: 0 (y rows in the global workspace) or the LDS ring size from sparse_ring
 hipError_t launch_sparse(const SparseArgs& A, bool mixed, int gen, int ring, int grid, hipStream_t s);

@@ -136,24 +136,26 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
     const PointArgs pts = DBG_WITH_TABLES(A.pts);
 
     for (int64_t it = blockIdx.x; it < A.n_tiles; it += gridDim.x) {
-        const int64_t tile = A.tile_list ? A.tile_list[it] : it;
+        const int64_t tile = A.tile_list ? A.tile_list[it] : A.tile0 + it;
         if (tile < 0) continue;  // uniform: one entry per workgroup
+        // records and partial sums: chunk-relative in record / reverse mode
+        const int64_t rt = (REC || REV) ? it : tile;
         const int k = (int)(tile / A.tiles_per_rep);
         const int64_t jt = tile - (int64_t)k * A.tiles_per_rep;
         const int64_t j = jt * TS + tid + 1;  // 1-based point index of the walker's sample
         const bool valid = walker && j <= A.m;
         SobolLane sl{};
         if constexpr (SOBT) sl = sobol_lane(pts, jt * TS + 1, j);
-        T* __restrict__ Ys = REC ? Yall + (size_t)tile * TS : Yall + (size_t)blockIdx.x * (size_t)A.y_slot;
-        T* __restrict__ srec = (REC || REV) ? srec_all + (size_t)tile * TS : nullptr;
-        const T* __restrict__ yrec = REV ? yrec_all + (size_t)tile * TS : nullptr;
+        T* __restrict__ Ys = REC ? Yall + (size_t)rt * TS : Yall + (size_t)blockIdx.x * (size_t)A.y_slot;
+        T* __restrict__ srec = (REC || REV) ? srec_all + (size_t)rt * TS : nullptr;
+        const T* __restrict__ yrec = REV ? yrec_all + (size_t)rt * TS : nullptr;
 
         typename Path<MIXED, FORM>::State st;
         Weight<T> w;
         T ybar = T(0), wt = T(0);
         if constexpr (REV) {
             if (valid) {
-                const size_t idx = (size_t)tile * TS + tid;
+                const size_t idx = (size_t)rt * TS + tid;
                 const T pmv = ((const T*)A.wm)[idx];
                 wt = pmv != T(0) ? (T)(ldexp((double)pmv, A.we[idx] - (int)A.norm[0]) * A.norm[1]) : T(0);
             }
@@ -353,16 +355,16 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
             const double pm = valid ? (double)w.pm : 0.0;
             if constexpr (REC) {
                 if (walker) {
-                    ((T*)A.wm)[(size_t)tile * TS + tid] = valid ? w.pm : T(0);
-                    A.we[(size_t)tile * TS + tid] = w.pe;
+                    ((T*)A.wm)[(size_t)rt * TS + tid] = valid ? w.pm : T(0);
+                    A.we[(size_t)rt * TS + tid] = w.pe;
                 }
             }
             double S;
             int E;
             block_scaled_sum<256>(pm, w.pe, S, E);
             if (tid == 0) {
-                part_m[tile] = S;
-                part_e[tile] = E;
+                part_m[rt] = S;
+                part_e[rt] = E;
             }
         }
     }

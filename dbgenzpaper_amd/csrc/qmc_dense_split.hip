@@ -167,7 +167,10 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
     const int64_t ldu = A.ldu;
     const PointArgs pts = DBG_WITH_TABLES(A.pts);
 
-    for (int64_t tile = blockIdx.x; tile < A.n_tiles; tile += gridDim.x) {
+    for (int64_t it = blockIdx.x; it < A.n_tiles; it += gridDim.x) {
+        const int64_t tile = A.tile0 + it;
+        // records and partial sums: chunk-relative in record / reverse mode
+        const int64_t rt = (REC || REV) ? it : tile;
         const int k = (int)(tile / A.tiles_per_rep);
         const int64_t jt = tile - (int64_t)k * A.tiles_per_rep;
         const int64_t j = jt * kTS + tid + 1;  // 1-based point index of this thread's sample
@@ -175,15 +178,15 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
         SobolLane sl{};
         if constexpr (SOBT) sl = sobol_lane(pts, jt * kTS + 1, j);
         _Float16* __restrict__ Ys16 = reinterpret_cast<_Float16*>(Yall + (size_t)blockIdx.x * (size_t)A.y_slot);
-        float* __restrict__ yrec = (REC || REV) ? yrec_all + (size_t)tile * kTS : nullptr;
-        float* __restrict__ srec = (REC || REV) ? srec_all + (size_t)tile * kTS : nullptr;
+        float* __restrict__ yrec = (REC || REV) ? yrec_all + (size_t)rt * kTS : nullptr;
+        float* __restrict__ srec = (REC || REV) ? srec_all + (size_t)rt * kTS : nullptr;
 
         typename Path<true, kPhiTail>::State st;
         Weight<float> w;
         float ybar = 0.0f, wt = 0.0f, unsc = 0.0f;
         if constexpr (REV) {
             if (valid) {
-                const size_t idx = (size_t)tile * kTS + tid;
+                const size_t idx = (size_t)rt * kTS + tid;
                 const float pmv = ((const float*)A.wm)[idx];
                 if (pmv != 0.0f) {
                     const double w64 = ldexp((double)pmv, A.we[idx] - (int)A.norm[0]) * A.norm[1];
@@ -378,15 +381,15 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
         }
         const double pm = valid ? (double)w.pm : 0.0;
         if constexpr (REC) {
-            ((float*)A.wm)[(size_t)tile * kTS + tid] = valid ? w.pm : 0.0f;
-            A.we[(size_t)tile * kTS + tid] = w.pe;
+            ((float*)A.wm)[(size_t)rt * kTS + tid] = valid ? w.pm : 0.0f;
+            A.we[(size_t)rt * kTS + tid] = w.pe;
         }
         double S;
         int E;
         block_scaled_sum<256>(pm, w.pe, S, E);
         if (tid == 0) {
-            part_m[tile] = S;
-            part_e[tile] = E;
+            part_m[rt] = S;
+            part_e[rt] = E;
         }
     }
 }

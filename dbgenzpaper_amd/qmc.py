@@ -381,6 +381,18 @@ class _QMCBase:
                                      L.ptr(gU, C.c_double)))
         return QMCGrad(float(lv.value), ga, gb, gs, ga_s, gb_s, gU)
 
+    def set_grad_budget(self, record_bytes: int = 0) -> None:
+        """Device memory for the gradient's y and s records per chunk of
+        samples (0: the default, 4 GiB); larger rounds run in chunks."""
+        L.check(L.lib().dbg_plan_set_grad_budget(self._plan, int(record_bytes)))
+
+    def grad_info(self):
+        """(samples per chunk, chunks, device bytes of all gradient buffers)
+        of the last grad(); zeros before the first."""
+        cs, nc, nb = C.c_int64(), C.c_int32(), C.c_int64()
+        L.check(L.lib().dbg_plan_grad_info(self._plan, C.byref(cs), C.byref(nc), C.byref(nb)))
+        return int(cs.value), int(nc.value), int(nb.value)
+
     # -- hot path ------------------------------------------------------------
     def loop(self, n_pts0: int = 0) -> None:
         """One qmc_loop! over the current block, queued without waiting."""

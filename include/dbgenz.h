@@ -239,11 +239,24 @@ int dbg_plan_summary(dbg_plan* plan, dbg_result* out);
  *              the pivot order of the factorization is held fixed;
  *   ga_std, gb_std, gU  the same with respect to the plan's standardized,
  *              permuted bounds and its factor U (upper triangle, n x n).
- * Dense plans only.  The records take 2 (n-1) m n_reps values of T on the
- * device; dbg_plan_last_kernel_ms then reports the device time of the whole
- * gradient. */
+ * Dense plans only.  The samples run in chunks: each chunk is recorded (y
+ * and the conditional means, 2 (n-1) values of T per sample), swept back,
+ * and folded into the sums with its share of the estimate, so the records
+ * take at most the plan's gradient budget (dbg_plan_set_grad_budget) however
+ * many points the round has.  dbg_plan_last_kernel_ms then reports the device
+ * time of the whole gradient. */
 int dbg_qmc_grad(dbg_plan* plan, double* log_value, double* ga, double* gb, double* gSigma,
                  double* ga_std, double* gb_std, double* gU);
+
+/* Device memory for the gradient's y and s records (no reference
+ * counterpart): record_bytes per chunk of samples (at least one tile of
+ * samples per chunk), 0 for the default of 4 GiB.  Buffers of an earlier
+ * gradient are released when the budget changes. */
+int dbg_plan_set_grad_budget(dbg_plan* plan, int64_t record_bytes);
+/* The chunking of the last gradient: samples per chunk (whole tiles), the
+ * number of chunks and the device memory held by all of the gradient's
+ * buffers.  Zeros before the first dbg_qmc_grad. */
+int dbg_plan_grad_info(dbg_plan* plan, int64_t* chunk_samples, int32_t* n_chunks, int64_t* device_bytes);
 
 /* The FP64 re-evaluation of the last round of a mixed dense plan (no
  * reference counterpart): tiles re-evaluated per shift, the largest share of

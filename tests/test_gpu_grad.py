@@ -76,6 +76,41 @@ def test_grad_mixed_close_to_oracle(dbg, oracle, n, gen):
     assert _relnorm(np.triu(g.U), np.triu(ro.gU)) < 2e-3
 
 
+@pytest.mark.parametrize("n,T,m,R", [(40, np.float64, 700, 3), (150, np.float64, 1000, 3),
+                                     (64, np.float32, 1000, 3), (200, np.float32, 2000, 3)])
+def test_grad_chunks_equal_one_chunk(dbg, oracle, n, T, m, R):
+    """A record budget that splits the round into chunks of samples (the last
+    one shorter, m not a multiple of the tile) gives the gradient of one
+    chunk: FP64 to rounding, mixed within the FP32 rounding of the chunk-
+    relative weights.  FP64 also against the oracle."""
+    print(f"grad chunks n={n} {np.dtype(T).name} m={m} R={R}", flush=True)
+    P = W.factor_model(n, 4, m=m, n_reps=R, seed=n + 1, u_range=(0.05, 0.3))
+    D = _plan(dbg, P.Sigma, P.a, P.b, T, m, R)
+    D.set_shifts(P.shifts)
+    g1 = D.grad()
+    rs1, nc1, bytes1 = D.grad_info()
+    assert nc1 == 1 and rs1 >= m * R and bytes1 > 0
+    rec_total = 2 * (n - 1) * rs1 * np.dtype(T).itemsize
+    D.set_grad_budget(rec_total // 4)
+    gc = D.grad()
+    rs, nc, nbytes = D.grad_info()
+    print(f"  chunks {nc} of {rs} samples, {nbytes / 2**20:.1f} MiB (one chunk: {bytes1 / 2**20:.1f} MiB)",
+          flush=True)
+    assert nc >= 3 and rs * nc >= m * R and nbytes < bytes1
+    tol = 1e-11 if T == np.float64 else 1e-4
+    assert abs(gc.log_value - g1.log_value) <= tol * abs(g1.log_value)
+    assert _relnorm(gc.b_std, g1.b_std) < tol
+    assert _relnorm(np.triu(gc.U), np.triu(g1.U)) < tol
+    assert _relnorm(gc.Sigma, g1.Sigma) < tol
+    if T == np.float64:
+        ro = _oracle_grad(oracle, D, "Richtmyer")
+        assert _relnorm(gc.b_std, ro.gb) < 1e-9
+        assert _relnorm(np.triu(gc.U), np.triu(ro.gU)) < 1e-9
+    D.set_grad_budget(0)
+    assert D.grad_info() == (0, 0, 0)
+    D.close()
+
+
 def test_grad_repeatable_and_state(dbg):
     """Two calls on one plan give the same gradient; the forward estimate is
     unchanged by them."""
