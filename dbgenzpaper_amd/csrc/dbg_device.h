@@ -595,8 +595,8 @@ struct StepOps {
     }
 };
 
-template <bool MIXED, int GEN, int FORM, int TS, int SXS, int SB, int CC, bool YST = true, typename T,
-          typename State>
+template <bool MIXED, int GEN, int FORM, int TS, int SXS, int SB, int CC, bool YST = true, bool SX = false,
+          typename T, typename State>
 __device__ __forceinline__ void walk_steps(const PointArgs& pts, int gb, int cnt, int k, int64_t j, int t,
                                            State& st, Weight<T>& w, T (&yv)[SB], StepOps<MIXED, GEN, SB, T>& cur,
                                            const T* __restrict__ s_x, T* __restrict__ s_y, T* __restrict__ Ys,
@@ -627,11 +627,15 @@ __device__ __forceinline__ void walk_steps(const PointArgs& pts, int gb, int cnt
 #pragma unroll
             for (int l = 0; l <= CC; ++l) acc = fma(yv[l], cur.u[l], acc);
             if (srec) srec[(size_t)g * ys + t] = acc;
+            // SX: the conditional mean back into the hand-off slot it came from
+            // (read only by this thread), for the caller to record
+            if constexpr (SX) const_cast<T*>(s_x)[t * SXS + CC] = acc;
             PT::phi(st, acc, cur.a, cur.b, cur.inv);
             PT::weigh(w, st);
             if constexpr (CC + 1 < SB)
-                walk_steps<MIXED, GEN, FORM, TS, SXS, SB, CC + 1, YST>(pts, gb, cnt, k, j, t, st, w, yv, nxt, s_x, s_y,
-                                                                   Ys, ys, Ucol, ldc, ak, bk, invk, srec, s_sob, sl);
+                walk_steps<MIXED, GEN, FORM, TS, SXS, SB, CC + 1, YST, SX>(pts, gb, cnt, k, j, t, st, w, yv, nxt, s_x,
+                                                                       s_y, Ys, ys, Ucol, ldc, ak, bk, invk, srec,
+                                                                       s_sob, sl);
         }
     }
 }
@@ -639,7 +643,8 @@ __device__ __forceinline__ void walk_steps(const PointArgs& pts, int gb, int cnt
 // The walk of one sub-block; with YST = false the y values are not stored
 // to the workspace row by row but left in yv for the caller (which stores
 // the whole sub-block at once in its own layout).
-template <bool MIXED, int GEN, int FORM, int TS, int SXS, int SB, bool YST, typename T, typename State>
+template <bool MIXED, int GEN, int FORM, int TS, int SXS, int SB, bool YST, bool SX = false, typename T,
+          typename State>
 __device__ __forceinline__ void walk_block_yv(T (&yv)[SB], const PointArgs& pts, int gb, int cnt, int k,
                                               int64_t j, int t, State& st, Weight<T>& w,
                                               const T* __restrict__ s_x, T* __restrict__ s_y,
@@ -650,7 +655,7 @@ __device__ __forceinline__ void walk_block_yv(T (&yv)[SB], const PointArgs& pts,
 {
     StepOps<MIXED, GEN, SB, T> first;
     if (cnt > 0) first.template fetch<0>(pts, gb, k, Ucol, ldc, ak, bk, invk);
-    walk_steps<MIXED, GEN, FORM, TS, SXS, SB, 0, YST>(pts, gb, cnt, k, j, t, st, w, yv, first, s_x, s_y, Ys, ys,
+    walk_steps<MIXED, GEN, FORM, TS, SXS, SB, 0, YST, SX>(pts, gb, cnt, k, j, t, st, w, yv, first, s_x, s_y, Ys, ys,
                                                       Ucol, ldc, ak, bk, invk, srec, s_sob, sl);
 }
 
@@ -775,8 +780,9 @@ struct RevOps {
 // unsc: the factor that undoes a per-sample scale of wt (and so of every
 // sbar and ybar of the sample) where values leave the sample: the recorded
 // sbar and the per-index sums (1 without such a scale).  YST = false leaves
-// the sub-block's sbar in sv for the caller instead of storing them to Ys.
-template <bool MIXED, int GEN, int TS, int SXS, int SB, int CC, bool YST = true, typename T>
+// the sub-block's sbar in sv for the caller instead of storing them to Ys;
+// SREC = false leaves the sbar records to the caller as well.
+template <bool MIXED, int GEN, int TS, int SXS, int SB, int CC, bool YST = true, bool SREC = true, typename T>
 __device__ __forceinline__ void rev_steps(const PointArgs& pts, int n, int gb, int cnt, int k, int64_t j, int t,
                                           T& ybar, T wt, const T* __restrict__ s_x, T* __restrict__ s_y,
                                           T* __restrict__ Ys, const T* __restrict__ Uf, int64_t ldc,
@@ -805,7 +811,7 @@ __device__ __forceinline__ void rev_steps(const PointArgs& pts, int n, int gb, i
             sv[CC] = sbar;
             s_y[CC * TS + t] = sbar;
             if constexpr (YST) Ys[(size_t)g * TS + t] = sbar;
-            srec[(size_t)(r - 1) * rs + t] = sbar * unsc;
+            if constexpr (SREC) srec[(size_t)(r - 1) * rs + t] = sbar * unsc;
             // per-index sums over the wavefront: FP32 partials of 64 FP32
             // terms on the mixed path, FP64 on the FP64 path
             double a0 = 0.0, b0, d0;
@@ -828,9 +834,9 @@ __device__ __forceinline__ void rev_steps(const PointArgs& pts, int n, int gb, i
 #pragma unroll
             for (int l = 0; l <= CC; ++l) acc = fma(sv[l], cur.u[l], acc);
             ybar = acc;
-            rev_steps<MIXED, GEN, TS, SXS, SB, CC + 1, YST>(pts, n, gb, cnt, k, j, t, ybar, wt, s_x, s_y, Ys, Uf, ldc,
-                                                            ak, bk, invk, srec, yrec, rs, red, sv, s_nx, y_nx, ops,
-                                                            a_inf, unsc);
+            rev_steps<MIXED, GEN, TS, SXS, SB, CC + 1, YST, SREC>(pts, n, gb, cnt, k, j, t, ybar, wt, s_x, s_y, Ys,
+                                                                  Uf, ldc, ak, bk, invk, srec, yrec, rs, red, sv,
+                                                                  s_nx, y_nx, ops, a_inf, unsc);
         }
     }
 }
@@ -843,7 +849,12 @@ __device__ __forceinline__ void rev_steps(const PointArgs& pts, int n, int gb, i
 // to the flipped workspace (s_y, Ys) for the contraction and to the record
 // srec[(r - 1) * rs + t] in place of s_r for the gradient of U.  Per index,
 // the three sums over the workgroup's samples are left in red[wave][cc][3].
-template <bool MIXED, int GEN, int TS, int SXS, int SB, bool YST, typename T>
+// The records of a step are read one step ahead (they stream from HBM): on
+// entry s_nx, y_nx hold those of the block's first step (record_first_step);
+// step cc overwrites record row r - 1 only after reading it, and the next
+// step reads row r - 2.  The scalar operands are fetched one step ahead as
+// well.
+template <bool MIXED, int GEN, int TS, int SXS, int SB, bool YST, bool SREC = true, typename T>
 __device__ __forceinline__ void walk_block_rev_sv(T (&sv)[SB], const PointArgs& pts, int n, int gb, int cnt, int k,
                                                   int64_t j, int t, T& ybar, T wt, const T* __restrict__ s_x,
                                                   T* __restrict__ s_y, T* __restrict__ Ys,
@@ -851,22 +862,24 @@ __device__ __forceinline__ void walk_block_rev_sv(T (&sv)[SB], const PointArgs& 
                                                   const T* __restrict__ ak, const T* __restrict__ bk,
                                                   const T* __restrict__ invk, T* __restrict__ srec,
                                                   const T* __restrict__ yrec, int64_t rs,
-                                                  double* __restrict__ red, bool a_inf, T unsc)
+                                                  double* __restrict__ red, bool a_inf, T unsc, T s_nx, T y_nx)
 {
-    // the records of a step are read one step ahead (they stream from HBM);
-    // step cc overwrites record row r - 1 only after reading it, and the
-    // next step reads row r - 2.  The scalar operands are fetched one step
-    // ahead as well.
-    T s_nx = T(0), y_nx = T(0);
     RevOps<MIXED, GEN, SB, T> ops[2];
-    if (cnt > 0) {
-        const int r0 = n - 1 - gb;
-        s_nx = srec[(size_t)(r0 - 1) * rs + t];  // r >= 1 here
-        y_nx = (r0 <= n - 2) ? yrec[(size_t)r0 * rs + t] : T(0);
-        ops[0].template fetch<0>(pts, n, gb, k, Uf, ldc, ak, bk, invk);
-    }
-    rev_steps<MIXED, GEN, TS, SXS, SB, 0, YST>(pts, n, gb, cnt, k, j, t, ybar, wt, s_x, s_y, Ys, Uf, ldc, ak, bk,
-                                               invk, srec, yrec, rs, red, sv, s_nx, y_nx, ops, a_inf, unsc);
+    if (cnt > 0) ops[0].template fetch<0>(pts, n, gb, k, Uf, ldc, ak, bk, invk);
+    rev_steps<MIXED, GEN, TS, SXS, SB, 0, YST, SREC>(pts, n, gb, cnt, k, j, t, ybar, wt, s_x, s_y, Ys, Uf, ldc, ak,
+                                                     bk, invk, srec, yrec, rs, red, sv, s_nx, y_nx, ops, a_inf, unsc);
+}
+
+// The records of the first step of the reverse block at flipped index gb
+// (r0 = n - 1 - gb >= 1): s_r0 from row r0 - 1, y_r0 from row r0 (none for
+// r0 = n - 1).
+template <typename T>
+__device__ __forceinline__ void record_first_step(int n, int gb, int t, const T* __restrict__ srec,
+                                                  const T* __restrict__ yrec, int64_t rs, T& s_nx, T& y_nx)
+{
+    const int r0 = n - 1 - gb;
+    s_nx = srec[(size_t)(r0 - 1) * rs + t];
+    y_nx = (r0 <= n - 2) ? yrec[(size_t)r0 * rs + t] : T(0);
 }
 
 template <bool MIXED, int GEN, int TS, int SXS, int SB, typename T>
@@ -880,8 +893,10 @@ __device__ __forceinline__ void walk_block_rev(const PointArgs& pts, int n, int 
                                                double* __restrict__ red, bool a_inf)
 {
     T sv[SB];
+    T s_nx = T(0), y_nx = T(0);
+    if (cnt > 0) record_first_step(n, gb, t, srec, yrec, rs, s_nx, y_nx);
     walk_block_rev_sv<MIXED, GEN, TS, SXS, SB, true>(sv, pts, n, gb, cnt, k, j, t, ybar, wt, s_x, s_y, Ys, Uf, ldc,
-                                                      ak, bk, invk, srec, yrec, rs, red, a_inf, T(1));
+                                                      ak, bk, invk, srec, yrec, rs, red, a_inf, T(1), s_nx, y_nx);
 }
 
 // ---------------------------------------------------------------------------

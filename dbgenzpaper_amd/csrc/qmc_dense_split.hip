@@ -184,6 +184,10 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
         typename Path<true, kPhiTail>::State st;
         Weight<float> w;
         float ybar = 0.0f, wt = 0.0f, unsc = 0.0f;
+        // reverse: the records of the next block's first step, read ahead
+        // (after each block's closing barrier; for the first block here)
+        float s_nx = 0.0f, y_nx = 0.0f;
+        if constexpr (REV) record_first_step(n, 0, tid, srec, yrec, A.rs, s_nx, y_nx);
         if constexpr (REV) {
             if (valid) {
                 const size_t idx = (size_t)rt * kTS + tid;
@@ -319,20 +323,41 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
                 }
                 const int cnt = min(kSB, nd - gb);
                 float yv[kSB];
+                // the records (record: y from s_y, s left in s_x; reverse: sbar
+                // from s_y) are stored after the wait below, which would
+                // otherwise wait for them as well: they drain during the next walk
                 if constexpr (REV)
-                    walk_block_rev_sv<true, GEN, kTS, kSXS, kSB, false>(yv, pts, n, gb, cnt, k, j, tid, ybar, wt, s_x,
-                                                                        s_y, (float*)nullptr, Ucol, A.ldc, ak, bk,
-                                                                        invk, srec, yrec, A.rs, s_red,
-                                                                        A.a_inf != 0, unsc);
+                    walk_block_rev_sv<true, GEN, kTS, kSXS, kSB, false, false>(
+                        yv, pts, n, gb, cnt, k, j, tid, ybar, wt, s_x, s_y, (float*)nullptr, Ucol, A.ldc, ak, bk,
+                        invk, srec, yrec, A.rs, s_red, A.a_inf != 0, unsc, s_nx, y_nx);
                 else
-                    walk_block_yv<true, GEN, kPhiTail, kTS, kSXS, kSB, REC>(yv, pts, gb, cnt, k, j, tid, st, w, s_x,
-                                                                          s_y, yrec, REC ? A.rs : 0, Ucol, A.ldc, ak,
-                                                                          bk, invk, srec, SOBT ? s_sob : nullptr, sl);
+                    walk_block_yv<true, GEN, kPhiTail, kTS, kSXS, kSB, false, REC>(
+                        yv, pts, gb, cnt, k, j, tid, st, w, s_x, s_y, (float*)nullptr, 0, Ucol, A.ldc, ak, bk, invk,
+                        (float*)nullptr, SOBT ? s_sob : nullptr, sl);
                 store_y16(Ys16, gb, tid, yv, cnt, kOpScale);
                 // the direct loads above write LDS without the compiler tracking them
                 if (later) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
                 __syncthreads();
+                if constexpr (REC) {
+#pragma unroll
+                    for (int c = 0; c < kSB; ++c) {
+                        if (c < cnt) {  // uniform
+                            yrec[(size_t)(gb + c) * A.rs + tid] = s_y[c * kTS + tid];
+                            srec[(size_t)(gb + c) * A.rs + tid] = s_x[tid * kSXS + c];
+                        }
+                    }
+                }
                 if constexpr (REV) {
+                    // the next block's first records, in flight through the
+                    // in-panel update (or the next panel's contraction)
+                    if (gb + kSB < nd) record_first_step(n, gb + kSB, tid, srec, yrec, A.rs, s_nx, y_nx);
+                    // sbar of flipped index gb + c replaces s in record row r - 1,
+                    // r = n - 1 - (gb + c) (every record of the block was read in the walk)
+#pragma unroll
+                    for (int c = 0; c < kSB; ++c) {
+                        if (c < cnt)  // uniform
+                            srec[(size_t)(n - 2 - (gb + c)) * A.rs + tid] = s_y[c * kTS + tid] * unsc;
+                    }
                     // per-index sums of the four waves -> the global accumulators
                     if (tid < 3 * cnt) {
                         const int cc = tid / 3, q = tid - 3 * cc;
