@@ -375,9 +375,11 @@ int orc_cholesky_genz(double* M, int n, double* a, double* b, int bs1, int bs2, 
  * d[k] over k in [i, ii1), scanned from i with a strict > (:319-327, so the
  * first maximum wins and a NaN after i is never taken); the rank check
  * m_ii > 0 (no eps0) comes before the swap (:329-336); a and b are
- * standardized (:285-300) but not permuted.  perm is 0-based. */
+ * standardized (:285-300) but not permuted.  perm is 0-based.  tol: the
+ * rank check is m_ii > tol (0 for the reference; the rank-deficient route
+ * of the plans uses n * DBL_EPSILON, LAPACK dpstrf's default). */
 int orc_cholesky_rowmax(double* M, int n, double* a, double* b, int bs1, int bs2, int bs3,
-                        int bs4, int* perm, double* scale, int* rank_out)
+                        int bs4, int* perm, double* scale, int* rank_out, double tol)
 {
     (void)standardize(M, n, a, b, scale, 1);
     double* d_vec = calloc((size_t)n, sizeof(double));
@@ -404,7 +406,7 @@ int orc_cholesky_rowmax(double* M, int n, double* a, double* b, int bs1, int bs2
                                 i_m = k;
                             }
                         }
-                        if (m_ii > 0) {
+                        if (m_ii > tol) {
                             m_ii = sqrt(m_ii);
                         } else {
                             rank = i;
@@ -702,13 +704,11 @@ static inline double rand_point(const orc_desc* D, const orc_points* P, int i /*
     return fma(2.0, u, -1.0);
 }
 
-/* c, dc of the next dimension from the conditional mean s.  src/qmc.jl:405-429
- * (erf form) and :579-593 (erfc form). */
-static inline void phi_update(int phi_form, double s_val, double a_i, double b_i, double inv,
-                              double* c_out, double* dc_out)
+/* c, dc of the next dimension from the erf arguments of its interval,
+ * c_val = (a - s) / (U_ii sqrt 2) and dc_val = (b - s) / (U_ii sqrt 2) for the
+ * conditional mean s.  src/qmc.jl:405-429 (erf form) and :579-593 (erfc form). */
+static inline void phi_z_update(int phi_form, double c_val, double dc_val, double* c_out, double* dc_out)
 {
-    double c_val = (a_i - s_val) * inv;
-    double dc_val = (b_i - s_val) * inv;
     c_val = fmax(fmin(c_val, 9.0), -9.0);
     dc_val = fmax(fmin(dc_val, 9.0), -9.0);
     if (phi_form == 0) {
@@ -735,14 +735,20 @@ static inline double draw_y(const orc_desc* D, const orc_points* P, int i, int k
     return sqrt(2.0) * orc_erfinv(rand_point(D, P, i, k, j, c, dc));
 }
 
+/* the state of an interval given by its erf arguments za <= zb */
+static inline void phi_state_z(int phi_form, double za, double zb, double* c_out, double* dc_out,
+                               double* eb_out, int* mode_out)
+{
+    if (phi_form == 2)
+        tail_masses(za, zb, c_out, eb_out, dc_out, mode_out);
+    else
+        phi_z_update(phi_form, za, zb, c_out, dc_out);
+}
+
 static inline void phi_state(int phi_form, double s_val, double a_i, double b_i, double inv,
                              double* c_out, double* dc_out, double* eb_out, int* mode_out)
 {
-    if (phi_form == 2) {
-        tail_masses((a_i - s_val) * inv, (b_i - s_val) * inv, c_out, eb_out, dc_out, mode_out);
-    } else {
-        phi_update(phi_form, s_val, a_i, b_i, inv, c_out, dc_out);
-    }
+    phi_state_z(phi_form, (a_i - s_val) * inv, (b_i - s_val) * inv, c_out, dc_out, eb_out, mode_out);
 }
 
 /* Sum of p_t * 2^{e_t} over a batch, as a scaled number. */
@@ -904,6 +910,131 @@ int orc_qmc_loop_dense(const orc_desc* D, const orc_points* P, const double* U, 
     return 0;
 }
 
+/* The first dimension's state, analytic (src/qmc.jl:800-809): erf arguments
+ * arg_a = a_1 / (U_11 sqrt 2), arg_b likewise, unclamped. */
+static void first_state(int phi_form, int kind, double arg_a, double arg_b, double* c_1, double* dc_1,
+                        double* eb_1, int* mode_1)
+{
+    if (phi_form == 2 && kind != 2) {
+        /* tail form: masses of the two tails */
+        tail_masses(arg_a, arg_b, c_1, eb_1, dc_1, mode_1);
+        if (*mode_1 == 2) *dc_1 = 0.5 * (erf(arg_b) - erf(arg_a));
+    } else if (phi_form == 1 && kind == 0) {
+        *c_1 = 0.5 * erfc(-arg_a);
+        *dc_1 = 0.5 * (erfc(-arg_b) - erfc(-arg_a));
+    } else {
+        *c_1 = 0.5 * (1.0 + erf(arg_a));
+        *dc_1 = 0.5 * (erf(arg_b) - erf(arg_a));
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Rank-deficient Sigma (no reference counterpart: QMCData throws for it,   */
+/* src/qmc.jl:195-197)                                                      */
+/* ------------------------------------------------------------------------ */
+/* After cholesky_rowmax! with a rank tolerance (rank r), the rows 0 .. r-1
+ * of the dependent columns completed from Sigma, and adapt_low_rank!
+ * (src/cholesky.jl:20-123), column block i = beg[i] .. beg[i+1]-1 (i < r)
+ * holds latent variable i in its first column (U[i, beg[i]] = U_ii) and the
+ * dependent variables whose last coefficient above row r is on latent i
+ * (scaled so that it is about 1, bounds exchanged where it was negative);
+ * columns beg[r] .. n-1 have no coefficient (zero variance).  One sample's
+ * integrand is prod_i dc_i, the interval of latent i being the intersection
+ * over its block of
+ *     [(a_c - s_c) inv_c, (b_c - s_c) inv_c],  inv_c = 1 / (U[i, c] sqrt 2),
+ *     s_c = sum_{l < i} U[l, c] y_l            (fma, l ascending),
+ * in erf-argument units (an empty intersection gives dc_i = 0), and y_g is
+ * drawn from latent g's interval with point dimension g + 1 (1-based), as in
+ * the dense loop (which this is when every block has one column).  The zero-
+ * variance columns contribute the factor [a_c <= 0 <= b_c] to latent 0's
+ * analytic interval (lowrank_first). */
+static void lowrank_first(int phi_form, const double* U, int n, const double* a, const double* b,
+                          const int* beg, int r, double* c_1, double* dc_1, double* eb_1, int* mode_1)
+{
+    const double sq2 = sqrt(2.0);
+    double za = -INFINITY, zb = INFINITY;
+    for (int c = beg[0]; c < beg[1]; ++c) {
+        const double u = U[(size_t)c * n]; /* row 0 */
+        za = fmax(za, a[c] / (u * sq2));
+        zb = fmin(zb, b[c] / (u * sq2));
+    }
+    if (!(za <= zb)) zb = za;
+    *eb_1 = 0.0;
+    *mode_1 = 0;
+    first_state(phi_form, 0, za, zb, c_1, dc_1, eb_1, mode_1);
+    for (int c = beg[r]; c < n; ++c)
+        if (!(a[c] <= 0.0 && 0.0 <= b[c])) *dc_1 = 0.0;
+}
+
+int orc_qmc_loop_lowrank(const orc_desc* D, const orc_points* P, const double* U, const double* a,
+                         const double* b, const double* inv, const int* beg, int r, int64_t n_pts0,
+                         double c_1, double dc_1, double eb_1, int mode_1, double* rep_m, int* rep_e)
+{
+    const int n = D->n;
+    const int64_t m = D->m;
+    const int bs_j = D->bs_j;
+    const int64_t n_batches = (m + bs_j - 1) / bs_j;
+    scaled_t* bsum = malloc(sizeof(scaled_t) * (size_t)n_batches);
+    if (!bsum) return -1;
+    for (int k = 0; k < D->n_reps; ++k) {
+#pragma omp parallel num_threads(D->threads > 0 ? D->threads : omp_get_max_threads())
+        {
+            double* Y = malloc(sizeof(double) * (size_t)bs_j * (size_t)(r > 1 ? r - 1 : 1));
+            double* cv = malloc(sizeof(double) * (size_t)bs_j);
+            double* dcv = malloc(sizeof(double) * (size_t)bs_j);
+            double* ebv = malloc(sizeof(double) * (size_t)bs_j);
+            int* mdv = malloc(sizeof(int) * (size_t)bs_j);
+            double* pm = malloc(sizeof(double) * (size_t)bs_j);
+            int* pe = malloc(sizeof(int) * (size_t)bs_j);
+#pragma omp for schedule(static)
+            for (int64_t bi = 0; bi < n_batches; ++bi) {
+                const int64_t j1 = bi * bs_j + 1;
+                const int64_t j2 = j1 + bs_j - 1 < m ? j1 + bs_j - 1 : m;
+                const int len = (int)(j2 - j1 + 1);
+                const scaled_t p0 = sc_make(dc_1);
+                for (int t = 0; t < len; ++t) {
+                    cv[t] = c_1;
+                    dcv[t] = dc_1;
+                    ebv[t] = eb_1;
+                    mdv[t] = mode_1;
+                    pm[t] = p0.m;
+                    pe[t] = p0.e;
+                }
+                for (int g = 0; g < r - 1; ++g) {
+                    const int i = g + 1;
+                    for (int t = 0; t < len; ++t) {
+                        double* Yt = Y + (size_t)t * (size_t)(r - 1);
+                        Yt[g] = draw_y(D, P, g + 1, k, j1 + t, cv[t], dcv[t], ebv[t], mdv[t]);
+                        double za = -INFINITY, zb = INFINITY;
+                        for (int c = beg[i]; c < beg[i + 1]; ++c) {
+                            const double* uc = U + (size_t)c * n;
+                            double sc = 0.0;
+                            for (int l = 0; l < i; ++l) sc = fma(Yt[l], uc[l], sc);
+                            za = fmax(za, (a[c] - sc) * inv[c]);
+                            zb = fmin(zb, (b[c] - sc) * inv[c]);
+                        }
+                        if (!(za <= zb)) zb = za;
+                        phi_state_z(D->phi_form, za, zb, &cv[t], &dcv[t], &ebv[t], &mdv[t]);
+                        int e;
+                        pm[t] = frexp(pm[t] * dcv[t], &e);
+                        pe[t] += e;
+                    }
+                }
+                bsum[bi] = batch_sum(pm, pe, len);
+            }
+            free(Y); free(cv); free(dcv); free(ebv); free(mdv); free(pm); free(pe);
+        }
+        scaled_t tot = {0.0, 0};
+        for (int64_t bi = 0; bi < n_batches; ++bi) tot = sc_add(tot, bsum[bi]);
+        scaled_t rep = {rep_m[k], rep_e[k]};
+        rep = rep_merge(rep, tot, n_pts0, m);
+        rep_m[k] = rep.m;
+        rep_e[k] = rep.e;
+    }
+    free(bsum);
+    return 0;
+}
+
 /* Sparse qmc_loop!  src/qmc.jl:618-718.  colptr/rowval/nzval: CSC of
  * triu(U, 1) with 0-based pointers and rows (rows ascending per column);
  * Ud: diagonal of U.  tent = 1 follows src/qmc.jl; tent = 0 with Sobol
@@ -1041,6 +1172,20 @@ static void reps_summary(const double* rep_m, const int* rep_e, int n_reps, doub
  * current block (Richtmyer n_base = r*m; Sobol first index = skip + r*m);
  * next_points! increments it.  SparseQMC: number of blocks generated so far
  * (round 0 always re-uses block 0, copyto!(D.X, D.X0)). */
+/* the level structure of a rank-deficient plan (kind 3) */
+typedef struct {
+    const double* inv;
+    const int* beg;
+    int r;
+} lowrank_t;
+
+static int pnorm_rounds(const orc_desc* D, int kind, const double* U, const double* a, const double* b,
+                        const int64_t* colptr, const int32_t* rowval, const double* nzval, const lowrank_t* LR,
+                        double c_1, double dc_1, double eb_1, int mode_1, const double* q,
+                        const double* shifts_f, const uint32_t* V, const uint64_t* shifts_u,
+                        uint64_t sobol_skip, int64_t max_pts, double max_abs_err, int64_t* gen_state,
+                        double* rep_m, int* rep_e, orc_result* out);
+
 int orc_qmc_pnorm(const orc_desc* D, int kind, const double* U, const double* a, const double* b,
                   const int64_t* colptr, const int32_t* rowval, const double* nzval,
                   const double* q, const double* shifts_f, const uint32_t* V,
@@ -1048,26 +1193,43 @@ int orc_qmc_pnorm(const orc_desc* D, int kind, const double* U, const double* a,
                   double max_abs_err, int64_t* gen_state, double* rep_m, int* rep_e,
                   orc_result* out)
 {
-    const int n = D->n;
     const double u11 = U[0];
     const double sq2 = sqrt(2.0);
     const double arg_a = a[0] / (u11 * sq2);
     const double arg_b = b[0] / (u11 * sq2);
     double c_1, dc_1, eb_1 = 0.0;
     int mode_1 = 0;
-    if (D->phi_form == 2 && kind != 2) {
-        /* tail form of src/qmc.jl:800-809: unclamped arguments */
-        tail_masses(arg_a, arg_b, &c_1, &eb_1, &dc_1, &mode_1);
-        if (mode_1 == 2) dc_1 = 0.5 * (erf(arg_b) - erf(arg_a));
-    } else if (D->phi_form == 1 && kind == 0) {
-        c_1 = 0.5 * erfc(-arg_a);
-        dc_1 = 0.5 * (erfc(-arg_b) - erfc(-arg_a));
-    } else {
-        c_1 = 0.5 * (1.0 + erf(arg_a));
-        dc_1 = 0.5 * (erf(arg_b) - erf(arg_a));
-    }
+    first_state(D->phi_form, kind, arg_a, arg_b, &c_1, &dc_1, &eb_1, &mode_1);
+    return pnorm_rounds(D, kind, U, a, b, colptr, rowval, nzval, NULL, c_1, dc_1, eb_1, mode_1, q, shifts_f, V,
+                        shifts_u, sobol_skip, max_pts, max_abs_err, gen_state, rep_m, rep_e, out);
+}
+
+/* qmc_pnorm! (src/qmc.jl:783-850) over the level structure of a rank-
+ * deficient factor (orc_qmc_loop_lowrank): U, a, b, inv, beg as there,
+ * r >= 1 latent variables, r - 1 sampled dimensions. */
+int orc_qmc_pnorm_lowrank(const orc_desc* D, const double* U, const double* a, const double* b,
+                          const double* inv, const int* beg, int r, const double* q, const double* shifts_f,
+                          const uint32_t* V, const uint64_t* shifts_u, uint64_t sobol_skip, int64_t max_pts,
+                          double max_abs_err, int64_t* gen_state, double* rep_m, int* rep_e, orc_result* out)
+{
+    double c_1, dc_1, eb_1;
+    int mode_1;
+    lowrank_first(D->phi_form, U, D->n, a, b, beg, r, &c_1, &dc_1, &eb_1, &mode_1);
+    const lowrank_t LR = {inv, beg, r};
+    return pnorm_rounds(D, 3, U, a, b, NULL, NULL, NULL, &LR, c_1, dc_1, eb_1, mode_1, q, shifts_f, V, shifts_u,
+                        sobol_skip, max_pts, max_abs_err, gen_state, rep_m, rep_e, out);
+}
+
+static int pnorm_rounds(const orc_desc* D, int kind, const double* U, const double* a, const double* b,
+                        const int64_t* colptr, const int32_t* rowval, const double* nzval, const lowrank_t* LR,
+                        double c_1, double dc_1, double eb_1, int mode_1, const double* q,
+                        const double* shifts_f, const uint32_t* V, const uint64_t* shifts_u,
+                        uint64_t sobol_skip, int64_t max_pts, double max_abs_err, int64_t* gen_state,
+                        double* rep_m, int* rep_e, orc_result* out)
+{
+    const int n = D->n;
     double* Ud = NULL;
-    if (kind != 0) {
+    if (kind == 1 || kind == 2) {
         Ud = malloc(sizeof(double) * (size_t)n);
         for (int i = 0; i < n; ++i) Ud[i] = U[(size_t)i + (size_t)i * n];
     }
@@ -1093,7 +1255,7 @@ int orc_qmc_pnorm(const orc_desc* D, int kind, const double* U, const double* a,
             block = *gen_state;
         P.n_base = block * D->m;
         P.sobol_first = sobol_skip + (uint64_t)block * (uint64_t)D->m;
-        if (n == 1) {
+        if (n == 1 || (kind == 3 && LR->r == 1)) {
             /* no sampled dimension: every point has p = dc_1 */
             for (int k = 0; k < D->n_reps; ++k) {
                 scaled_t pz = sc_make(dc_1);
@@ -1106,6 +1268,9 @@ int orc_qmc_pnorm(const orc_desc* D, int kind, const double* U, const double* a,
             }
         } else if (kind == 0) {
             orc_qmc_loop_dense(D, &P, U, a, b, n_pts, c_1, dc_1, eb_1, mode_1, rep_m, rep_e);
+        } else if (kind == 3) {
+            orc_qmc_loop_lowrank(D, &P, U, a, b, LR->inv, LR->beg, LR->r, n_pts, c_1, dc_1, eb_1, mode_1, rep_m,
+                                 rep_e);
         } else {
             orc_qmc_loop_sparse(D, &P, colptr, rowval, nzval, Ud, a, b, n_pts, c_1, dc_1, eb_1,
                                 mode_1, kind == 1, rep_m, rep_e);

@@ -47,13 +47,16 @@ def lib():
         L.orc_sobol_block.argtypes = [C.c_int, u32p, C.c_uint64, C.c_int64, u64p]
         L.orc_cholesky_genz.argtypes = [dp, C.c_int, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int,
                                         ip, dp, ip, dp]
-        L.orc_cholesky_rowmax.argtypes = [dp, C.c_int, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int, ip, dp, ip]
+        L.orc_cholesky_rowmax.argtypes = [dp, C.c_int, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int, ip, dp, ip,
+                                          C.c_double]
         L.orc_adapt_low_rank.argtypes = [dp, C.c_int, dp, dp, C.c_int, ip, C.c_double, ip, ip]
         L.orc_cholesky_classic.argtypes = [dp, C.c_int, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int, dp]
         L.orc_truncate_U.argtypes = [dp, C.c_int, C.c_double]
         L.orc_qmc_pnorm.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, i64p, C.POINTER(C.c_int32), dp,
                                     dp, dp, u32p, u64p, C.c_uint64, C.c_int64, C.c_double, i64p, dp,
                                     ip, C.c_void_p]
+        L.orc_qmc_pnorm_lowrank.argtypes = [C.c_void_p, dp, dp, dp, dp, ip, C.c_int, dp, dp, u32p, u64p,
+                                            C.c_uint64, C.c_int64, C.c_double, i64p, dp, ip, C.c_void_p]
         L.orc_qmc_grad_dense.argtypes = [C.c_void_p, dp, dp, dp, dp, dp, u32p, u64p, C.c_uint64, dp, dp,
                                          dp, dp]
         L.orc_num_threads.restype = C.c_int
@@ -141,8 +144,9 @@ def cholesky_genz(M, a, b, bs1=2 ** 7, bs2=2 ** 9, bs3=2 ** 10, bs4=2 ** 11) -> 
     return CholeskyResult(M, perm.astype(np.int64), rank.value, a, b, scale, gaps)
 
 
-def cholesky_rowmax(M, a, b, bs1=2 ** 7, bs2=2 ** 9, bs3=2 ** 10, bs4=2 ** 11) -> CholeskyResult:
-    """cholesky_rowmax!  src/cholesky.jl:266-432 (on copies)."""
+def cholesky_rowmax(M, a, b, bs1=2 ** 7, bs2=2 ** 9, bs3=2 ** 10, bs4=2 ** 11, tol=0.0) -> CholeskyResult:
+    """cholesky_rowmax!  src/cholesky.jl:266-432 (on copies).  tol: the rank
+    check is m_ii > tol (0: the reference's)."""
     M = np.array(M, np.float64, order="F", copy=True)
     a = np.array(a, np.float64, copy=True)
     b = np.array(b, np.float64, copy=True)
@@ -152,7 +156,7 @@ def cholesky_rowmax(M, a, b, bs1=2 ** 7, bs2=2 ** 9, bs3=2 ** 10, bs4=2 ** 11) -
     rank = C.c_int(0)
     rc = lib().orc_cholesky_rowmax(_ptr(M, C.c_double), n, _ptr(a, C.c_double), _ptr(b, C.c_double),
                                    bs1, bs2, bs3, bs4, _ptr(perm, C.c_int), _ptr(scale, C.c_double),
-                                   C.byref(rank))
+                                   C.byref(rank), float(tol))
     assert rc == 0
     return CholeskyResult(M, perm.astype(np.int64), rank.value, a, b, scale)
 
@@ -239,7 +243,6 @@ def qmc_pnorm(U, a, b, *, m, n_reps, gen="Richtmyer", phi_form="erf", kind="dens
     a = np.ascontiguousarray(a, np.float64)
     b = np.ascontiguousarray(b, np.float64)
     n = U.shape[0]
-    nd = max(n - 1, 1)
     kind_i = {"dense": 0, "sparse": 1, "sparseqmc": 2}[kind]
     gen_i = {"Richtmyer": 0, "Sobol": 1}[gen]
     if kind_i == 2:
@@ -248,19 +251,7 @@ def qmc_pnorm(U, a, b, *, m, n_reps, gen="Richtmyer", phi_form="erf", kind="dens
                  bs_j, n_bits, threads)
     if max_pts is None:
         max_pts = m
-    if q is None:
-        q = sqrt_primes(nd)
-    q = np.ascontiguousarray(q, np.float64)
-    if gen_i == 0:
-        shifts_f = np.asfortranarray(shifts, np.float64)
-        assert shifts_f.shape == (n - 1, n_reps) or n == 1
-        shifts_u = np.zeros(1, np.uint64)
-        V = np.zeros((1, 32), np.uint32)
-    else:
-        shifts_u = np.asfortranarray(shifts, np.uint64)
-        shifts_f = np.zeros(1, np.float64)
-        V = np.ascontiguousarray(V, np.uint32)
-        assert V.shape[0] >= n - 1
+    q, shifts_f, shifts_u, V = _point_args(n, n_reps, gen_i, q, shifts, V)
     if sobol_skip is None:
         sobol_skip = sobol_skip_count(2 * m - 1)
     if kind_i != 0:
@@ -277,6 +268,109 @@ def qmc_pnorm(U, a, b, *, m, n_reps, gen="Richtmyer", phi_form="erf", kind="dens
                              _ptr(V, C.c_uint32), _ptr(shifts_u, C.c_uint64), sobol_skip, max_pts,
                              max_abs_err, C.byref(state), _ptr(rep_m, C.c_double),
                              _ptr(rep_e, C.c_int), C.byref(res))
+    assert rc == 0
+    return PnormResult(res.value, res.log_value, res.err3, res.rel_err3, res.n_pts, res.rounds,
+                       np.ldexp(rep_m, rep_e), rep_m, rep_e.astype(np.int64), state.value)
+
+
+def _point_args(n, n_reps, gen_i, q, shifts, V):
+    """Richtmyer sqrt-prime fractions and shifts ((n-1) x n_reps), or Sobol
+    direction integers and digital shifts, as the C drivers take them."""
+    nd = max(n - 1, 1)
+    q = np.ascontiguousarray(sqrt_primes(nd) if q is None else q, np.float64)
+    if gen_i == 0:
+        shifts_f = np.asfortranarray(shifts, np.float64)
+        assert shifts_f.shape == (n - 1, n_reps) or n == 1
+        shifts_u = np.zeros(1, np.uint64)
+        V = np.zeros((1, 32), np.uint32)
+    else:
+        shifts_u = np.asfortranarray(shifts, np.uint64)
+        shifts_f = np.zeros(1, np.float64)
+        V = np.ascontiguousarray(V, np.uint32)
+        assert V.shape[0] >= n - 1
+    return q, shifts_f, shifts_u, V
+
+
+@dataclass
+class LowRankFactor:
+    """The level structure of a rank-deficient factor (no reference
+    counterpart; see prepare_lowrank)."""
+    U: np.ndarray          # n x n column-major: rows 0 .. rank-1 meaningful
+    a: np.ndarray          # bounds of the columns (standardized, scaled by adapt_low_rank!)
+    b: np.ndarray
+    perm: np.ndarray       # original variable of each column, 0-based
+    rank: int
+    beg: np.ndarray        # rank + 1 column-block starts; beg[rank] = n - k0
+    k0: int                # zero-variance columns at the end
+    inv: np.ndarray        # 1 / (U[i, c] sqrt 2) for column c of block i
+    scale_vec: np.ndarray
+
+
+def prepare_lowrank(Sigma, a, b, *, chol_block_size=2 ** 6, chol_block_size2=2 ** 9, tol=None, eps1=None):
+    """The rank-deficient route of the QMCData constructor (no reference
+    counterpart: src/qmc.jl:195-197 throws).  cholesky_rowmax!
+    (src/cholesky.jl:266-432) with the rank check m_ii > tol (default
+    n * eps, LAPACK dpstrf's); rows 0 .. r-1 of the dependent columns r .. n-1
+    completed from the standardized, permuted Sigma, U_r^T X = C_p[0:r, r:n];
+    the standardized bounds permuted; adapt_low_rank! (src/cholesky.jl:20-123)
+    with eps1 (default sqrt(eps)); the column blocks and 1/(U[i, c] sqrt 2)."""
+    from scipy.linalg import solve_triangular
+
+    S = np.array(Sigma, np.float64, order="F")
+    n = S.shape[0]
+    eps = np.finfo(np.float64).eps
+    tol = n * eps if tol is None else tol
+    eps1 = np.sqrt(eps) if eps1 is None else eps1
+    F = cholesky_rowmax(S, a, b, chol_block_size, chol_block_size2, tol=tol)
+    r = F.rank
+    if r == n:
+        raise ValueError("the factor has full rank")
+    p = F.perm
+    s = F.scale_vec
+    U = np.zeros((n, n), order="F")
+    U[:r, :r] = np.triu(F.U[:r, :r])
+    Cp = S[np.ix_(p[:r], p[r:])] * ((1.0 / s[p[r:]])[None, :] / s[p[:r]][:, None])
+    U[:r, r:] = solve_triangular(U[:r, :r], Cp, trans="T", lower=False)
+    U, aa, bb, perm, kjs, k0 = adapt_low_rank(U, F.a[p], F.b[p], r, p, eps1)
+    beg = np.zeros(r + 1, np.int32)
+    beg[1:] = kjs + 1
+    assert beg[r] == n - k0
+    inv = np.zeros(n)
+    for i in range(r):
+        for c in range(beg[i], beg[i + 1]):
+            inv[c] = 1.0 / (U[i, c] * np.sqrt(2.0))
+    return LowRankFactor(U, aa, bb, perm, r, beg, k0, inv, s)
+
+
+def qmc_pnorm_lowrank(LF: LowRankFactor, *, m, n_reps, gen="Richtmyer", phi_form="erf", bs_j=2 ** 6, n_bits=32,
+                      max_pts=None, max_abs_err=1e-6, q=None, shifts=None, V=None, sobol_skip=None,
+                      gen_state=None, threads=0) -> PnormResult:
+    """qmc_pnorm! (src/qmc.jl:783-850) over the level structure of a
+    rank-deficient factor: per sample, the interval of latent variable i is the
+    intersection over its column block, and rank - 1 dimensions are sampled
+    (the first rank - 1 rows of the (n-1) x n_reps shifts are used)."""
+    U = np.asfortranarray(LF.U, np.float64)
+    a = np.ascontiguousarray(LF.a, np.float64)
+    b = np.ascontiguousarray(LF.b, np.float64)
+    inv = np.ascontiguousarray(LF.inv, np.float64)
+    beg = np.ascontiguousarray(LF.beg, np.int32)
+    n = U.shape[0]
+    gen_i = {"Richtmyer": 0, "Sobol": 1}[gen]
+    desc = _Desc(n, m, n_reps, gen_i, {"erf": 0, "erfc": 1, "tail": 2}[phi_form], 1, 1, bs_j, n_bits, threads)
+    if max_pts is None:
+        max_pts = m
+    q, shifts_f, shifts_u, V = _point_args(n, n_reps, gen_i, q, shifts, V)
+    if sobol_skip is None:
+        sobol_skip = sobol_skip_count(2 * m - 1)
+    state = C.c_int64(0 if gen_state is None else gen_state)
+    rep_m = np.zeros(n_reps, np.float64)
+    rep_e = np.zeros(n_reps, np.int32)
+    res = _Result()
+    rc = lib().orc_qmc_pnorm_lowrank(C.byref(desc), _ptr(U, C.c_double), _ptr(a, C.c_double), _ptr(b, C.c_double),
+                                     _ptr(inv, C.c_double), _ptr(beg, C.c_int), LF.rank, _ptr(q, C.c_double),
+                                     _ptr(shifts_f, C.c_double), _ptr(V, C.c_uint32), _ptr(shifts_u, C.c_uint64),
+                                     sobol_skip, max_pts, max_abs_err, C.byref(state), _ptr(rep_m, C.c_double),
+                                     _ptr(rep_e, C.c_int), C.byref(res))
     assert rc == 0
     return PnormResult(res.value, res.log_value, res.err3, res.rel_err3, res.n_pts, res.rounds,
                        np.ldexp(rep_m, rep_e), rep_m, rep_e.astype(np.int64), state.value)
