@@ -35,6 +35,7 @@ EXPORTS = (
     "dbg_plan_get_state", "dbg_plan_set_state", "dbg_plan_refactor", "dbg_batch_create",
     "dbg_batch_run", "dbg_batch_last_ms", "dbg_batch_tuning", "dbg_batch_destroy", "dbg_qmc_grad",
     "dbg_plan_refine_info", "dbg_chol_genz_f64_dev_noalloc", "dbg_plan_set_grad_budget", "dbg_plan_grad_info",
+    "dbg_plan_lowrank_info",
 )
 
 
@@ -106,6 +107,7 @@ def lib():
     L.dbg_plan_refine_info.argtypes = [vp, ip32, dp, C.POINTER(C.c_float)]
     L.dbg_plan_set_grad_budget.argtypes = [vp, C.c_int64]
     L.dbg_plan_grad_info.argtypes = [vp, i64p, ip32, i64p]
+    L.dbg_plan_lowrank_info.argtypes = [vp, ip32, ip32, ip32]
     L.dbg_batch_create.argtypes = [C.POINTER(vp), C.POINTER(Opts), C.c_int32, ip32, dp, dp, dp, dp]
     L.dbg_batch_run.argtypes = [vp, dp]
     L.dbg_batch_last_ms.argtypes = [vp, C.POINTER(C.c_float)]

@@ -20,7 +20,8 @@ LIB = os.path.join(HERE, "libdbgenz.so")
 ARCH = os.environ.get("DBG_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["capi.hip", "chol.hip", "prep.hip", "qmc_dense_valu.hip", "qmc_dense_mfma.hip", "qmc_dense_split.hip",
-           "qmc_sparse.hip", "reduce.hip", "batch.hip", "adjoint.hip", "cov.hip", "refine.hip"]
+           "qmc_sparse.hip", "reduce.hip", "batch.hip", "adjoint.hip", "cov.hip", "refine.hip",
+           "qmc_lowrank.hip"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 CFLAGS = [

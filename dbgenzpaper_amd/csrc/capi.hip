@@ -136,9 +136,11 @@ int check_block_sizes(int bs1, int bs2, int bs3, int bs4)
 
 // Copy the inputs (host or device memory, hipMemcpyDefault) and factor.
 // pivot: 0 cholesky_classic!, 1 cholesky_genz!, 2 cholesky_rowmax!.
+// rowmax_tol: the rank check of the row-max pivoting is m_ii > rowmax_tol
+// (0 for cholesky_rowmax! as the reference has it)
 int factor_on_device(CholBuffers& B, int n, const double* M_in, int64_t lda, const double* a_in,
                      const double* b_in, int bs1, int bs2, int bs3, int bs4, int pivot,
-                     hipStream_t s, int32_t* rank_out)
+                     hipStream_t s, int32_t* rank_out, double rowmax_tol = 0.0)
 {
     int rc = check_block_sizes(bs1, bs2, bs3, bs4);
     if (rc) return rc;
@@ -152,6 +154,11 @@ int factor_on_device(CholBuffers& B, int n, const double* M_in, int64_t lda, con
     HIPCHK(hipMemsetAsync(B.s_vec, 0, n * sizeof(double), s));
     CholArgs C{B.M, (int64_t)n, n, B.a, B.b, B.scale, B.perm, B.s_vec, B.eps0, B.rank};
     HIPCHK(chol_standardize(C, false, s));
+    if (pivot == 2) {
+        // the window kernels compare the row-max pivot with the eps0 slot
+        HIPCHK(hipMemcpyAsync(B.eps0, &rowmax_tol, sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
     const int nb = std::max(1, std::min(bs1, 64));
     for (auto& w : pivot_windows(n, bs2, bs3, bs4))
         if (w.second - w.first > chol_max_window())
@@ -275,6 +282,23 @@ struct dbg_plan {
     double* b0 = nullptr;
     CholBuffers F;
     int32_t rank = 0;
+    // rank-deficient dense plans (qmc_lowrank.hip): F holds the reordered
+    // factor of adapt_low_rank!, these its level structure
+    struct LowRank {
+        bool on = false;
+        int r = 0, k0 = 0, grid = 0;
+        std::vector<int32_t> kjs;     // last column of each level (0-based)
+        int32_t* beg = nullptr;       // r + 1 block starts
+        double* a = nullptr;          // bounds of the columns (standardized, scaled)
+        double* b = nullptr;
+        double* inv = nullptr;        // 1 / (U[i, c] sqrt 2)
+        int32_t* end = nullptr;       // adapt_low_rank! outputs and scratch
+        int32_t* k0d = nullptr;
+        double* tmp = nullptr;
+        double* Y = nullptr;          // grid slots of (r - 1) x tile values
+        int64_t slot = 0, tpr = 0, tiles = 0, y_cap = 0;
+        std::vector<void*> all() const { return {beg, a, b, inv, end, k0d, tmp, Y}; }
+    } L;
     // packed layouts
     void* Uk = nullptr;
     float* ucol32 = nullptr;  // FP32 copy of the factor for the mixed walk
@@ -393,6 +417,7 @@ struct dbg_plan {
     {
         G.free_all();
         for (void* p : R.all()) (void)hipFree(p);
+        for (void* p : L.all()) (void)hipFree(p);
         if (R.ev0) (void)hipEventDestroy(R.ev0);
         if (R.ev1) (void)hipEventDestroy(R.ev1);
         F.release();
@@ -407,6 +432,90 @@ struct dbg_plan {
 
 namespace {
 
+#define BLASCHK(expr)                                                                          \
+    do {                                                                                       \
+        rocblas_status st_ = (expr);                                                           \
+        if (st_ != rocblas_status_success)                                                     \
+            return fail(DBG_ERR_DEVICE, std::string("rocBLAS: ") + rocblas_status_to_string(st_)); \
+    } while (0)
+
+// The route of a dense plan whose Genz factorization stops short of full
+// rank (no reference counterpart: QMCData throws, src/qmc.jl:195-197):
+// cholesky_rowmax! with the rank check m_ii > n eps (LAPACK dpstrf's default
+// tolerance), rows 0 .. r-1 of the dependent columns completed from Sigma
+// (U_r^T X = C_p[0:r, r:n]), the standardized bounds permuted, and
+// adapt_low_rank! with eps1 = sqrt(eps): its blocks of columns, one per
+// latent variable, are integrated in FP64 by the level loop of
+// qmc_lowrank.hip.  A factor of full rank under the row-max pivoting runs the
+// same loop with one column per block.
+int lowrank_factor(dbg_plan* P)
+{
+    const dbg_opts& o = P->o;
+    const int n = P->n;
+    hipStream_t s = P->stream;
+    auto& L = P->L;
+    int32_t r = n;
+    int rc = factor_on_device(P->F, n, P->sig, n, P->a0, P->b0, o.chol_block_size, o.chol_block_size2,
+                              o.chol_block_size3, o.chol_block_size4, 2, s, &r, n * DBL_EPSILON);
+    if (rc) return rc;
+    if (r < 1) return fail(DBG_ERR_RANK, "the covariance matrix has rank 0");
+    if (L.beg == nullptr) {
+        HIPCHK(dalloc(&L.beg, (size_t)(n + 1) * sizeof(int32_t)));
+        HIPCHK(dalloc(&L.a, (size_t)n * sizeof(double)));
+        HIPCHK(dalloc(&L.b, (size_t)n * sizeof(double)));
+        HIPCHK(dalloc(&L.inv, (size_t)n * sizeof(double)));
+        HIPCHK(dalloc(&L.end, (size_t)n * sizeof(int32_t)));
+        HIPCHK(dalloc(&L.k0d, sizeof(int32_t)));
+        HIPCHK(dalloc(&L.tmp, (size_t)n * n * sizeof(double)));
+    }
+    // rows 0 .. r-1 of the dependent columns: C_p[0:r, r:n], then U_r^{-T} of it
+    HIPCHK(lowrank_fill(P->F.M, n, r, P->sig, P->F.perm, P->F.scale, s));
+    if (r < n) {
+        rocblas_handle h = (rocblas_handle)blas_handle_void(s);
+        if (!h) return fail(DBG_ERR_DEVICE, "rocBLAS handle");
+        const double one = 1.0;
+        BLASCHK(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
+                              rocblas_diagonal_non_unit, r, n - r, &one, P->F.M, n, P->F.M + (size_t)r * n, n));
+    }
+    HIPCHK(lowrank_bounds(P->F.a, P->F.b, P->F.perm, n, L.a, L.b, s));
+    HIPCHK(adapt_low_rank(P->F.M, n, n, L.a, L.b, P->F.perm, r, std::sqrt(DBL_EPSILON), L.end, L.tmp, L.k0d, s));
+    L.kjs.assign(r, 0);
+    int32_t k0 = 0;
+    HIPCHK(hipMemcpyAsync(L.kjs.data(), L.end, (size_t)r * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&k0, L.k0d, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::vector<int32_t> beg(r + 1, 0);
+    for (int i = 0; i < r; ++i) beg[i + 1] = L.kjs[i] + 1;
+    for (int i = 0; i < r; ++i)
+        if (beg[i + 1] <= beg[i]) return fail(DBG_ERR_DEVICE, "adapt_low_rank: empty level block");
+    if (beg[r] != n - k0) return fail(DBG_ERR_DEVICE, "adapt_low_rank: level blocks do not cover the columns");
+    HIPCHK(hipMemcpyAsync(L.beg, beg.data(), (size_t)(r + 1) * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    HIPCHK(lowrank_prep(P->F.M, n, L.beg, r, L.a, L.b, L.inv, P->first, o.phi_form, s));
+    HIPCHK(hipMemcpyAsync(P->first_h, P->first, 6 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    // integration layout: tiles of lowrank_tile() points, (r - 1) y per point
+    const int tile = lowrank_tile();
+    L.tpr = (o.m + tile - 1) / tile;
+    L.tiles = L.tpr * o.n_reps;
+    // (at a refactor: the plan's partial sums were sized by its dense tiles, never longer)
+    if (P->n_tiles > 0 && L.tiles > P->n_tiles)
+        return fail(DBG_ERR_DEVICE, "rank-deficient route: partial sums do not fit");
+    L.grid = (int)std::min<int64_t>(L.tiles, (int64_t)P->ncu * 4);
+    L.slot = (int64_t)std::max(r - 1, 1) * tile;
+    const int64_t need = (int64_t)L.grid * L.slot;
+    if (need > L.y_cap) {
+        (void)hipFree(L.Y);
+        L.Y = nullptr;
+        HIPCHK(dalloc(&L.Y, (size_t)need * sizeof(double)));
+        L.y_cap = need;
+    }
+    L.r = r;
+    L.k0 = k0;
+    L.on = true;
+    P->rank = r;
+    return DBG_OK;
+}
+
 // Factor the device-resident inputs and rebuild the kernel layouts (the
 // QMCData constructor's work, src/qmc.jl:182-199).
 int plan_factor(dbg_plan* P)
@@ -415,11 +524,11 @@ int plan_factor(dbg_plan* P)
     const int n = P->n;
     hipStream_t s = P->stream;
     const bool pivot = (o.kind == DBG_KIND_DENSE);
+    P->L.on = false;
     int rc = factor_on_device(P->F, n, P->sig, n, P->a0, P->b0, o.chol_block_size, o.chol_block_size2,
                               o.chol_block_size3, o.chol_block_size4, pivot ? 1 : 0, s, &P->rank);
     if (rc) return rc;
-    if (pivot && P->rank != n)
-        return fail(DBG_ERR_RANK, "QMCData requires a full-rank Cholesky factorization.");
+    if (pivot && P->rank != n) return lowrank_factor(P);
     // truncate_matrixU!: |U_ij| < eps(T) / (9n) -> 0.  A refining mixed plan
     // first keeps the factor truncated as in Float64 for its FP64 tiles.
     if (P->R.K > 0) {
@@ -566,8 +675,9 @@ int plan_build(dbg_plan* P, const double* Sigma, int64_t lda, const double* a, c
     P->grid = (int)std::min<int64_t>(P->n_tiles, (int64_t)ncu * 2);
     P->y_slot = P->split ? dense_split_slot(nd) : (int64_t)std::max(nd, 1) * P->tile;
     HIPCHK(dalloc(&P->Y, (size_t)P->grid * P->y_slot * T));
-    HIPCHK(dalloc(&P->part_m, (size_t)P->n_tiles * sizeof(double)));
-    HIPCHK(dalloc(&P->part_e, (size_t)P->n_tiles * sizeof(int)));
+    const int64_t n_part = std::max<int64_t>(P->n_tiles, P->L.on ? P->L.tiles : 0);
+    HIPCHK(dalloc(&P->part_m, (size_t)n_part * sizeof(double)));
+    HIPCHK(dalloc(&P->part_e, (size_t)n_part * sizeof(int)));
     HIPCHK(dalloc(&P->rep_m, (size_t)o.n_reps * sizeof(double)));
     HIPCHK(dalloc(&P->rep_e, (size_t)o.n_reps * sizeof(int)));
     HIPCHK(dalloc(&P->out, 8 * sizeof(double)));
@@ -606,7 +716,32 @@ int run_loop(dbg_plan* P, int64_t block, int64_t n_pts0)
         return fail(DBG_ERR_STATE, "Sobol direction integers not set");
     const int gen = (o.kind == DBG_KIND_SPARSEQMC) ? kGenSobol : o.generator;
     HIPCHK(hipEventRecord(P->ev0, P->stream));
-    if (o.kind == DBG_KIND_DENSE) {
+    if (P->L.on) {
+        // rank-deficient route: the level loop in FP64
+        const auto& L = P->L;
+        DenseArgs A{};
+        A.n = P->n;
+        A.m = o.m;
+        A.n_reps = o.n_reps;
+        A.tiles_per_rep = L.tpr;
+        A.n_tiles = L.tiles;
+        A.Ucol = P->F.M;
+        A.ldc = P->n;
+        A.ak = L.a;
+        A.bk = L.b;
+        A.invk = L.inv;
+        for (int q = 0; q < 6; ++q) A.first[q] = P->first_h[q];
+        A.pts = point_args(P, block);
+        A.Y = L.Y;
+        A.y_slot = L.slot;
+        A.part_m = P->part_m;
+        A.part_e = P->part_e;
+        A.lr_beg = L.beg;
+        A.lr_rank = L.r;
+        const int form = o.phi_form == DBG_PHI_TAIL ? kPhiTail : (o.phi_form == DBG_PHI_ERFC ? kPhiErfc : kPhiErf);
+        HIPCHK(launch_lowrank(A, gen, form, L.grid, P->stream));
+        HIPCHK(hipEventRecord(P->ev1, P->stream));
+    } else if (o.kind == DBG_KIND_DENSE) {
         DenseArgs A{};
         A.n = P->n;
         A.m = o.m;
@@ -686,7 +821,7 @@ int run_loop(dbg_plan* P, int64_t block, int64_t n_pts0)
         HIPCHK(launch_sparse(A, P->mixed, gen, P->ring, grid, P->stream));
         HIPCHK(hipEventRecord(P->ev1, P->stream));
     }
-    HIPCHK(launch_reduce_reps(P->part_m, P->part_e, P->tiles_per_rep, o.n_reps, n_pts0, o.m,
+    HIPCHK(launch_reduce_reps(P->part_m, P->part_e, P->L.on ? P->L.tpr : P->tiles_per_rep, o.n_reps, n_pts0, o.m,
                               o.kind == DBG_KIND_SPARSEQMC ? 1 : 0, P->rep_m, P->rep_e, P->out,
                               P->stream));
     return DBG_OK;
@@ -773,13 +908,6 @@ int grad_alloc(dbg_plan* P)
     G.ready = true;
     return DBG_OK;
 }
-
-#define BLASCHK(expr)                                                                          \
-    do {                                                                                       \
-        rocblas_status st_ = (expr);                                                           \
-        if (st_ != rocblas_status_success)                                                     \
-            return fail(DBG_ERR_DEVICE, std::string("rocBLAS: ") + rocblas_status_to_string(st_)); \
-    } while (0)
 
 int run_grad(dbg_plan* P)
 {
@@ -1202,6 +1330,7 @@ int dbg_qmc_grad(dbg_plan* P, double* log_value, double* ga, double* gb, double*
     const dbg_opts& o = P->o;
     const int n = P->n;
     if (o.kind != DBG_KIND_DENSE) return fail(DBG_ERR_ARG, "the gradient is defined for the dense (QMCData) plan");
+    if (P->L.on) return fail(DBG_ERR_ARG, "the gradient is defined for plans of full rank");
     if (n < 2) return fail(DBG_ERR_ARG, "the gradient needs n >= 2");
     if (P->nd > 0 && !P->shifts_set) return fail(DBG_ERR_STATE, "shifts not set");
     if (o.generator == DBG_GEN_SOBOL && !P->dirs_set) return fail(DBG_ERR_STATE, "Sobol direction integers not set");
@@ -1223,6 +1352,22 @@ int dbg_qmc_grad(dbg_plan* P, double* log_value, double* ga, double* gb, double*
     if (gU) HIPCHK(hipMemcpyAsync(gU, G.ub, nn, hipMemcpyDeviceToHost, P->stream));
     HIPCHK(hipStreamSynchronize(P->stream));
     if (log_value) *log_value = norm[2];
+    return DBG_OK;
+}
+
+int dbg_plan_lowrank_info(dbg_plan* P, int32_t* rank, int32_t* k_js, int32_t* k_0)
+{
+    g_err.clear();
+    if (!P) return fail(DBG_ERR_ARG, "null plan");
+    const auto& L = P->L;
+    if (rank) *rank = L.on ? L.r : P->rank;
+    if (k_0) *k_0 = L.on ? L.k0 : 0;
+    if (k_js) {
+        if (L.on)
+            std::copy(L.kjs.begin(), L.kjs.end(), k_js);
+        else
+            for (int i = 0; i < P->rank; ++i) k_js[i] = i;
+    }
     return DBG_OK;
 }
 
@@ -1256,10 +1401,10 @@ int dbg_plan_refine_info(dbg_plan* P, int32_t* tiles_per_shift, double* unrefine
     g_err.clear();
     if (!P) return fail(DBG_ERR_ARG, "null plan");
     auto& R = P->R;
-    if (tiles_per_shift) *tiles_per_shift = R.K;
+    if (tiles_per_shift) *tiles_per_shift = P->L.on ? 0 : R.K;
     if (unrefined_share) *unrefined_share = 0.0;
     if (ms) *ms = 0.0f;
-    if (R.K == 0) return DBG_OK;
+    if (R.K == 0 || P->L.on) return DBG_OK;
     std::vector<double> f(P->o.n_reps, 0.0);
     HIPCHK(hipMemcpyAsync(f.data(), R.frac, f.size() * sizeof(double), hipMemcpyDeviceToHost, P->stream));
     HIPCHK(hipStreamSynchronize(P->stream));
@@ -1317,8 +1462,9 @@ int dbg_plan_get_factor(dbg_plan* P, double* U, int32_t* perm, double* a, double
     HIPCHK(hipStreamSynchronize(P->stream));
     if (U) HIPCHK(hipMemcpy(U, P->F.M, (size_t)n * n * sizeof(double), hipMemcpyDeviceToHost));
     if (perm) HIPCHK(hipMemcpy(perm, P->F.perm, n * sizeof(int32_t), hipMemcpyDeviceToHost));
-    if (a) HIPCHK(hipMemcpy(a, P->F.a, n * sizeof(double), hipMemcpyDeviceToHost));
-    if (b) HIPCHK(hipMemcpy(b, P->F.b, n * sizeof(double), hipMemcpyDeviceToHost));
+    // rank-deficient plans: the bounds of the reordered columns
+    if (a) HIPCHK(hipMemcpy(a, P->L.on ? P->L.a : P->F.a, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (b) HIPCHK(hipMemcpy(b, P->L.on ? P->L.b : P->F.b, n * sizeof(double), hipMemcpyDeviceToHost));
     if (scale) HIPCHK(hipMemcpy(scale, P->F.scale, n * sizeof(double), hipMemcpyDeviceToHost));
     if (rank) *rank = P->rank;
     return DBG_OK;

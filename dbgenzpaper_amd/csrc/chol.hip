@@ -251,8 +251,9 @@ __device__ __forceinline__ void chol_window_body(const CholArgs& C, int w0, int 
                 }
                 double m_ii = MM(im, im) - s_dsum[im - w0];
                 // rank: genz stops at m_ii <= eps0 after the swap (src/cholesky.jl:518-541),
-                // rowmax at m_ii <= 0 before it (:329-350)
-                const bool stop = (pivot == kPivotGenz && !(m_ii > eps0)) || (pivot == kPivotRowMax && !(m_ii > 0.0));
+                // rowmax at m_ii <= eps0 before it, eps0 then being the rank tolerance
+                // (0 for the reference's m_ii <= 0, :329-350)
+                const bool stop = !(m_ii > eps0) && (pivot == kPivotGenz || pivot == kPivotRowMax);
                 if (stop && pivot == kPivotRowMax) im = i;
                 if (im != i) {
                     const int a0 = i - w0, a1 = im - w0;
@@ -509,8 +510,9 @@ __device__ __forceinline__ bool reg_pivot(const CholArgs& C, RegShared& S, ColRe
         const int a0 = i - ib0;
         const double m_ii = S.dg[im - ib0] - S.dsum[im - ib0];  // read before the swap
         // rank: genz stops at m_ii <= eps0 after the swap (src/cholesky.jl:518-541),
-        // rowmax at m_ii <= 0 before it (:329-350)
-        const bool stop = (pivot == kPivotGenz && !(m_ii > eps0)) || (pivot == kPivotRowMax && !(m_ii > 0.0));
+        // rowmax at m_ii <= eps0 before it, eps0 then being the rank tolerance
+        // (0 for the reference's m_ii <= 0, :329-350)
+        const bool stop = !(m_ii > eps0) && (pivot == kPivotGenz || pivot == kPivotRowMax);
         if (stop && pivot == kPivotRowMax) im = i;
         const int a1 = im - ib0;
         if (im != i) {

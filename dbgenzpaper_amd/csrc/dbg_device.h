@@ -454,8 +454,11 @@ struct Path<false, FORM> {
     }
     static __device__ __forceinline__ void phi(State& st, T s, T a, T b, T inv)
     {
-        double x = __dmul_rn(__dsub_rn(a, s), inv);
-        double xb = __dmul_rn(__dsub_rn(b, s), inv);
+        phi_z(st, __dmul_rn(__dsub_rn(a, s), inv), __dmul_rn(__dsub_rn(b, s), inv));
+    }
+    // the state of the interval with erf arguments x <= xb
+    static __device__ __forceinline__ void phi_z(State& st, double x, double xb)
+    {
         x = fmax(fmin(x, 9.0), -9.0);
         xb = fmax(fmin(xb, 9.0), -9.0);
         if constexpr (FORM == kPhiErf) {
@@ -485,6 +488,7 @@ struct Path<false, kPhiTail> {
     {
         tail_from_z(st, (a - s) * inv, (b - s) * inv);
     }
+    static __device__ __forceinline__ void phi_z(State& st, double za, double zb) { tail_from_z(st, za, zb); }
     static __device__ __forceinline__ void weigh(Weight<T>& w, const State& st) { w.mul(st.dc); }
 };
 
@@ -510,6 +514,43 @@ struct Path<true, FORM> {
         w.pe -= st.k;
     }
 };
+
+// The first dimension's interval, analytic (src/qmc.jl:800-809), from its
+// erf arguments arg_a = a_1 / (U_11 sqrt 2), arg_b likewise (unclamped):
+// first = {c1, dc1} of the erf or erfc form, then {ea, eb, d, mode} of the
+// tail form.
+__device__ __forceinline__ void first_from_args(double arg_a, double arg_b, int phi_form, double* first)
+{
+    double c1, dc1;
+    if (phi_form == kPhiErfc) {
+        c1 = 0.5 * erfc(-arg_a);
+        dc1 = 0.5 * (erfc(-arg_b) - erfc(-arg_a));
+    } else {
+        c1 = 0.5 * (1.0 + erf(arg_a));
+        dc1 = 0.5 * (erf(arg_b) - erf(arg_a));
+    }
+    first[0] = c1;
+    first[1] = dc1;
+    // mixed form of the same interval (tail masses, complementary width)
+    const double ea = 0.5 * erfc(fabs(arg_a));
+    const double eb = 0.5 * erfc(fabs(arg_b));
+    double d;
+    int mode;
+    if (arg_b <= 0.0) {
+        mode = 0;
+        d = eb - ea;
+    } else if (arg_a >= 0.0) {
+        mode = 1;
+        d = ea - eb;
+    } else {
+        mode = 2;
+        d = 0.5 * (erf(arg_b) - erf(arg_a));
+    }
+    first[2] = ea;
+    first[3] = eb;
+    first[4] = d;
+    first[5] = (double)mode;
+}
 
 template <bool MIXED, int FORM>
 __device__ __forceinline__ typename Path<MIXED, FORM>::T path_sample(const typename Path<MIXED, FORM>::State& st,

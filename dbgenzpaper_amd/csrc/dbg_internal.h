@@ -48,6 +48,10 @@ struct DenseArgs {
     // optional list of the tiles to run (FP64 integration kernels): entry i of
     // n_tiles names tile tile_list[i] of the round, entries < 0 are skipped
     const int64_t* tile_list;
+    // rank-deficient plans (qmc_lowrank.hip): rank + 1 column-block starts
+    // of the reordered factor (Ucol, ldc) and the rank r (r - 1 sampled dimensions)
+    const int32_t* lr_beg;
+    int lr_rank;
     // adjoint support (matrix-core kernels only)
     int mode;                  // kModeIntegrate, kModeRecord or kModeReverse
     // record / reverse: the launch runs tiles tile0 .. tile0 + n_tiles - 1 of
@@ -96,6 +100,15 @@ int dense_valu_tile();
 int dense_valu_width();
 hipError_t launch_dense_mfma(const DenseArgs& A, bool mixed, int gen, int form, int grid,
                              hipStream_t s);
+// the rank-deficient route (qmc_lowrank.hip)
+int lowrank_tile();
+hipError_t lowrank_fill(double* U, int n, int r, const double* Sig, const int32_t* perm, const double* scale,
+                        hipStream_t s);
+hipError_t lowrank_bounds(const double* a, const double* b, const int32_t* perm, int n, double* ap, double* bp,
+                          hipStream_t s);
+hipError_t lowrank_prep(const double* U, int n, const int32_t* beg, int r, const double* a, const double* b,
+                        double* inv, double* first, int phi_form, hipStream_t s);
+hipError_t launch_lowrank(const DenseArgs& A, int gen, int form, int grid, hipStream_t s);
 // record (forward keeping y, s and the weights) / reverse sweep, tail form
 hipError_t launch_dense_adjoint(const DenseArgs& A, bool mixed, int gen, int grid, hipStream_t s);
 hipError_t pack_flip(const double* U, int n, bool mixed, void* Ufk, int64_t ldu, void* Ufc, hipStream_t s);

@@ -54,37 +54,7 @@ __global__ void pack_vec_kernel(const double* __restrict__ U, int64_t lda, int n
     }
     if (j == 0) {
         const double u11 = U[0];
-        const double arg_a = a[0] / __dmul_rn(u11, kSqrt2);
-        const double arg_b = b[0] / __dmul_rn(u11, kSqrt2);
-        double c1, dc1;
-        if (phi_form == kPhiErfc) {
-            c1 = 0.5 * erfc(-arg_a);
-            dc1 = 0.5 * (erfc(-arg_b) - erfc(-arg_a));
-        } else {
-            c1 = 0.5 * (1.0 + erf(arg_a));
-            dc1 = 0.5 * (erf(arg_b) - erf(arg_a));
-        }
-        first[0] = c1;
-        first[1] = dc1;
-        // mixed form of the same interval (tail masses, complementary width)
-        const double ea = 0.5 * erfc(fabs(arg_a));
-        const double eb = 0.5 * erfc(fabs(arg_b));
-        double d;
-        int mode;
-        if (arg_b <= 0.0) {
-            mode = 0;
-            d = eb - ea;
-        } else if (arg_a >= 0.0) {
-            mode = 1;
-            d = ea - eb;
-        } else {
-            mode = 2;
-            d = 0.5 * (erf(arg_b) - erf(arg_a));
-        }
-        first[2] = ea;
-        first[3] = eb;
-        first[4] = d;
-        first[5] = (double)mode;
+        first_from_args(a[0] / __dmul_rn(u11, kSqrt2), b[0] / __dmul_rn(u11, kSqrt2), phi_form, first);
     }
 }
 

@@ -336,10 +336,15 @@ class _QMCBase:
             L.check(L.lib().dbg_plan_get_factor(self._plan, L.ptr(U, C.c_double), L.ptr(perm, C.c_int32),
                                                 L.ptr(a, C.c_double), L.ptr(b, C.c_double),
                                                 L.ptr(scale, C.c_double), C.byref(rank)))
-            method = "genz" if self._kind == L.KIND_DENSE else "none"
+            method = ("none" if self._kind != L.KIND_DENSE
+                      else ("genz" if int(rank.value) == n and not self._lowrank_route() else "row_max"))
             self._C = CholeskyGenz(U, perm.astype(np.int64), int(rank.value), a, b,
                                    self.qmc_opts.chol_block_size, method, scale)
         return self._C
+
+    def _lowrank_route(self) -> bool:
+        r, kjs, _ = self.lowrank_info()
+        return r < self.n or not np.array_equal(kjs, np.arange(r))
 
     @property
     def a(self):
@@ -380,6 +385,16 @@ class _QMCBase:
                                      L.ptr(gs, C.c_double), L.ptr(ga_s, C.c_double), L.ptr(gb_s, C.c_double),
                                      L.ptr(gU, C.c_double)))
         return QMCGrad(float(lv.value), ga, gb, gs, ga_s, gb_s, gU)
+
+    def lowrank_info(self):
+        """(rank, last column of each level (0-based), zero-variance columns)
+        of the plan's factor: a Sigma of rank r < n takes the rank-deficient
+        route (row-max factor, adapt_low_rank!, FP64 level loop) where the
+        reference's QMCData throws; (n, 0 .. n-1, 0) at full rank."""
+        r, k0 = C.c_int32(), C.c_int32()
+        kjs = np.zeros(self.n, np.int32)
+        L.check(L.lib().dbg_plan_lowrank_info(self._plan, C.byref(r), L.ptr(kjs, C.c_int32), C.byref(k0)))
+        return int(r.value), kjs[:r.value].astype(np.int64), int(k0.value)
 
     def set_grad_budget(self, record_bytes: int = 0) -> None:
         """Device memory for the gradient's y and s records per chunk of

@@ -189,9 +189,19 @@ int dbg_chol_classic_f64(double* M, int64_t lda, int32_t n, double* a, double* b
  * device, factors (pivoted for
  * DBG_KIND_DENSE, unpivoted otherwise), applies truncate_matrixU! and builds
  * the packed / CSC layouts.  The random shifts are set separately (the RNG
- * stays with the caller).  Fails with DBG_ERR_RANK for a rank-deficient
- * pivoted factor and with DBG_ERR_ARG for NaN bounds (where the reference
- * would return a NaN estimate). */
+ * stays with the caller).  Fails with DBG_ERR_ARG for NaN bounds (where the
+ * reference would return a NaN estimate).
+ * Where the reference throws for a rank-deficient factor (src/qmc.jl:195-197)
+ * a dense plan instead takes the rank-deficient route (no reference
+ * counterpart): cholesky_rowmax! (src/cholesky.jl:266-432) with the rank check
+ * m_ii > n eps, the dependent columns' rows completed from Sigma,
+ * adapt_low_rank! (src/cholesky.jl:20-123) with eps1 = sqrt(eps), and an FP64
+ * integration over the rank r latent variables in which each latent
+ * variable's interval is the intersection of the constraints of its level
+ * (r - 1 sampled dimensions: the first r - 1 rows of the shifts are used).
+ * dbg_plan_get_factor then returns the reordered factor, its columns' bounds
+ * and variables and the rank; dbg_plan_lowrank_info the levels.  DBG_ERR_RANK
+ * remains for a matrix of rank 0. */
 int dbg_plan_create(dbg_plan** plan, const dbg_opts* opts, int32_t n, const double* Sigma,
                     int64_t lda, const double* a, const double* b);
 int dbg_plan_destroy(dbg_plan* plan);
@@ -247,6 +257,13 @@ int dbg_plan_summary(dbg_plan* plan, dbg_result* out);
  * time of the whole gradient. */
 int dbg_qmc_grad(dbg_plan* plan, double* log_value, double* ga, double* gb, double* gSigma,
                  double* ga_std, double* gb_std, double* gU);
+
+/* The level structure of a dense plan (no reference counterpart): its rank r
+ * (n for a plan of full rank), the last column of each level k_js (r entries,
+ * 0-based: adapt_low_rank!'s k_js minus one; 0 .. n-1 at full rank) and the
+ * number of zero-variance columns k_0 at the end.  Any output may be NULL.
+ * The gradient (dbg_qmc_grad) is defined for plans of full rank only. */
+int dbg_plan_lowrank_info(dbg_plan* plan, int32_t* rank, int32_t* k_js, int32_t* k_0);
 
 /* Device memory for the gradient's y and s records (no reference
  * counterpart): record_bytes per chunk of samples (at least one tile of

@@ -81,11 +81,19 @@ def test_genz_rank_deficient(dbg, oracle):
     assert Fg.rank == Fo.rank < n
 
 
-def test_qmcdata_rank_error(dbg):
+def test_qmcdata_rank_deficient_route(dbg):
+    """Where the reference's QMCData throws (src/qmc.jl:195-197), the plan
+    takes the rank-deficient route: ten identical variables are one, and
+    P(-1 <= X <= 1) = Phi(1) - Phi(-1) with nothing sampled."""
+    from scipy.stats import norm
+
     n = 10
     C = np.ones((n, n))
-    with pytest.raises(ValueError, match="full-rank"):
-        dbg.QMCData(C, -np.ones(n), np.ones(n), dbg.QMC_opts(m=256, max_pts=256), 0)
+    D = dbg.QMCData(C, -np.ones(n), np.ones(n), dbg.QMC_opts(m=256, max_pts=256), 0)
+    assert D.lowrank_info()[0] == 1 and D.C.rank == 1
+    v, e, _ = dbg.qmc_pnorm(D)
+    D.close()
+    assert abs(v - (norm.cdf(1.0) - norm.cdf(-1.0))) < 1e-15 and e < 1e-15
 
 
 @pytest.mark.parametrize("bs", [8, 16, 32, 64])
