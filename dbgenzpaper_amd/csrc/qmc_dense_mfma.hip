@@ -43,8 +43,10 @@ struct MfmaCfg<true> {   // FP32: 256 samples x 128 targets, 128 accumulators pe
     static constexpr bool ULDS = true;
 };
 template <>
-struct MfmaCfg<false> {  // FP64: 128 samples x 128 targets, 128 accumulator registers per lane
-    static constexpr int TS = 128, PW = 128, PFY = 4, PFU = 4, OCC = 1;
+struct MfmaCfg<false> {  // FP64: 256 samples x 64 targets, 128 accumulator registers per lane
+    // (one sample per thread: every wavefront walks; the narrower panel costs
+    // twice the y re-reads of 128 targets, which the walk-bound kernel hides)
+    static constexpr int TS = 256, PW = 64, PFY = 4, PFU = 4, OCC = 1;
     static constexpr bool ULDS = false;
 };
 constexpr int kUC = 16;  // U rows per LDS chunk (four K-steps)
@@ -61,7 +63,7 @@ struct alignas(16) YVec {
 template <typename T, int NSB>
 __device__ __forceinline__ void load_y(T (&av)[NSB], const T* __restrict__ ya, int64_t ys, int k0)
 {
-    static_assert(sizeof(YVec<T, NSB>) == 16, "one 16-byte load per lane and K-step");
+    static_assert(sizeof(YVec<T, NSB>) % 16 == 0, "16-byte loads per lane and K-step");
     const YVec<T, NSB> v = *(const YVec<T, NSB>*)(ya + (size_t)k0 * ys);
 #pragma unroll
     for (int sb = 0; sb < NSB; ++sb) av[sb] = v.v[sb];
