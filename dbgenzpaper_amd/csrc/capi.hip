@@ -628,10 +628,10 @@ int plan_build(dbg_plan* P, const double* Sigma, int64_t lda, const double* a, c
     if (P->R.K > 0) {
         auto& R = P->R;
         R.mfma = n >= kMfmaMinDim;
-        const int t64 = R.mfma ? dense_mfma_tile(false) : dense_valu_tile();
+        const int t64 = R.mfma ? dense_mfma_refine_tile() : dense_valu_tile();
         if (tile_mixed % t64 != 0) return fail(DBG_ERR_ARG, "refinement: mixed tile not a multiple of the FP64 tile");
         R.ratio = tile_mixed / t64;
-        R.ldu = (int64_t)n + dense_mfma_panel(false) + 32;
+        R.ldu = (int64_t)n + dense_mfma_refine_panel() + 32;
         R.tpr = (o.m + t64 - 1) / t64;
         R.n_tiles = R.tpr * o.n_reps;
         const int64_t nlist = (int64_t)o.n_reps * R.K * R.ratio;
@@ -789,7 +789,7 @@ int run_loop(dbg_plan* P, int64_t block, int64_t n_pts0)
             B.part_m = R.part_m;
             B.part_e = R.part_e;
             if (R.mfma)
-                HIPCHK(launch_dense_mfma(B, false, gen, kPhiTail, R.grid, P->stream));
+                HIPCHK(launch_dense_mfma_refine(B, gen, R.grid, P->stream));
             else
                 HIPCHK(launch_dense_valu(B, false, gen, kPhiTail, R.grid, P->stream));
             HIPCHK(refine_merge(R.sel, o.n_reps, R.K, R.ratio, P->tiles_per_rep, R.tpr, R.part_m, R.part_e,

@@ -138,6 +138,12 @@ hipError_t sigma_chain(double* Ub, int n, const double* gd, double* X, hipStream
 void* blas_handle_void(hipStream_t s);
 int dense_mfma_tile(bool mixed);
 int dense_mfma_panel(bool mixed);
+// the FP64 re-evaluation of a mixed round's heaviest tiles: the FP64
+// matrix-core kernel with 128-sample tiles and 128-target panels, tail form,
+// on a tile list (latency-bound: a few tiles per shift)
+int dense_mfma_refine_tile();
+int dense_mfma_refine_panel();
+hipError_t launch_dense_mfma_refine(const DenseArgs& A, int gen, int grid, hipStream_t s);
 // mixed dense kernel with the panel contraction on FP16 matrix cores
 // (split operands; Richtmyer points)
 hipError_t launch_dense_split(const DenseArgs& A, int gen, const void* uc, int grid, hipStream_t s);

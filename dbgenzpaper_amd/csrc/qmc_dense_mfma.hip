@@ -29,7 +29,7 @@ namespace dbg {
 constexpr int kSB = 16;      // walk sub-block (dimensions)
 constexpr int kSXS = kSB + 4;  // LDS row stride of the accumulator hand-off
 
-template <bool MIXED>
+template <bool MIXED, bool NARROW = false>
 struct MfmaCfg;
 // TS samples x PW targets per workgroup; PFY / PFU: depths of the operand
 // rings of the panel contraction (Y rows stream from HBM, U rows are shared by
@@ -38,15 +38,21 @@ struct MfmaCfg;
 // ULDS: U rows of the panel contraction go through LDS in chunks of kUC rows,
 // loaded once per workgroup (instead of once per wavefront) one chunk ahead.
 template <>
-struct MfmaCfg<true> {   // FP32: 256 samples x 128 targets, 128 accumulators per lane
+struct MfmaCfg<true, false> {   // FP32: 256 samples x 128 targets, 128 accumulators per lane
     static constexpr int TS = 256, PW = 128, PFY = 4, PFU = 2, OCC = 2;
     static constexpr bool ULDS = true;
 };
 template <>
-struct MfmaCfg<false> {  // FP64: 256 samples x 64 targets, 128 accumulator registers per lane
+struct MfmaCfg<false, false> {  // FP64: 256 samples x 64 targets, 128 accumulator registers per lane
     // (one sample per thread: every wavefront walks; the narrower panel costs
     // twice the y re-reads of 128 targets, which the walk-bound kernel hides)
     static constexpr int TS = 256, PW = 64, PFY = 4, PFU = 4, OCC = 1;
+    static constexpr bool ULDS = false;
+};
+template <>
+struct MfmaCfg<false, true> {   // FP64, 128 samples x 128 targets: the re-evaluation of a few
+    // tiles, bound by one tile's latency (half the panels of the form above)
+    static constexpr int TS = 128, PW = 128, PFY = 4, PFU = 4, OCC = 1;
     static constexpr bool ULDS = false;
 };
 constexpr int kUC = 16;  // U rows per LDS chunk (four K-steps)
@@ -79,15 +85,17 @@ __device__ __forceinline__ void load_u(T (&bv)[NTB], const T* __restrict__ ub, i
 
 int dense_mfma_tile(bool mixed) { return mixed ? MfmaCfg<true>::TS : MfmaCfg<false>::TS; }
 int dense_mfma_panel(bool mixed) { return mixed ? MfmaCfg<true>::PW : MfmaCfg<false>::PW; }
+int dense_mfma_refine_tile() { return MfmaCfg<false, true>::TS; }
+int dense_mfma_refine_panel() { return MfmaCfg<false, true>::PW; }
 
 // MODE kModeIntegrate: the estimate.  kModeRecord: the same, keeping y and
 // the conditional means of every sample (rows of A.rs) and the per-sample
 // weights, for the adjoint.  kModeReverse: the reverse sweep in flipped index
 // order (Uk / Ucol hold the flipped factor, the "y" of a step is sbar); see
 // walk_block_rev.
-template <bool MIXED, int GEN, int FORM, int MODE,
+template <bool MIXED, int GEN, int FORM, int MODE, bool NARROW = false,
           typename T = typename std::conditional<MIXED, float, double>::type>
-__global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(DenseArgs A, const T* __restrict__ Uk,
+__global__ __launch_bounds__(256, (MfmaCfg<MIXED, NARROW>::OCC)) void dense_mfma_kernel(DenseArgs A, const T* __restrict__ Uk,
                                                          const T* __restrict__ Ucol,
                                                          const T* __restrict__ ak,
                                                          const T* __restrict__ bk,
@@ -100,11 +108,11 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
                                                          DBG_POINT_PARAMS)
 {
     using V = typename Mfma<T>::V;
-    constexpr int TS = MfmaCfg<MIXED>::TS;
-    constexpr int PW = MfmaCfg<MIXED>::PW;
+    constexpr int TS = MfmaCfg<MIXED, NARROW>::TS;
+    constexpr int PW = MfmaCfg<MIXED, NARROW>::PW;
     // the Sobol walk has no room in the FP32 register budget for the deeper Y ring
-    constexpr int kPFY = (MIXED && GEN == kGenSobol) ? 2 : MfmaCfg<MIXED>::PFY;
-    constexpr int kPFU = MfmaCfg<MIXED>::PFU;
+    constexpr int kPFY = (MIXED && GEN == kGenSobol) ? 2 : MfmaCfg<MIXED, NARROW>::PFY;
+    constexpr int kPFU = MfmaCfg<MIXED, NARROW>::PFU;
     constexpr int kUnroll = kPFY > kPFU ? kPFY : kPFU;
     static_assert(kUnroll % kPFY == 0 && kUnroll % kPFU == 0, "ring depths must divide the unroll");
     constexpr int WS = TS / 4;         // samples per wavefront
@@ -116,7 +124,7 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
     __shared__ T s_x[TS * kSXS];       // accumulator hand-off, [sample][16]
     __shared__ T s_y[kSB * TS];        // y of the current sub-block, [k][sample]
     __shared__ double s_red[REV ? 4 * kSB * 3 : 1];
-    constexpr bool ULDS = MfmaCfg<MIXED>::ULDS;
+    constexpr bool ULDS = MfmaCfg<MIXED, NARROW>::ULDS;
     // the in-panel update's factor rows staged in LDS during the walk (not for
     // Sobol: its LDS tables leave no registers for the staging addresses)
     constexpr bool UPDL = ULDS && GEN != kGenSobol;
@@ -372,12 +380,12 @@ __global__ __launch_bounds__(256, MfmaCfg<MIXED>::OCC) void dense_mfma_kernel(De
     }
 }
 
-template <bool MIXED, int GEN, int FORM, int MODE = kModeIntegrate>
+template <bool MIXED, int GEN, int FORM, int MODE = kModeIntegrate, bool NARROW = false>
 static void launch_m(const DenseArgs& A, int grid, hipStream_t s)
 {
     using T = typename std::conditional<MIXED, float, double>::type;
     T* Y = (MODE == kModeRecord) ? (T*)A.yrec : (T*)A.Y;
-    hipLaunchKernelGGL((dense_mfma_kernel<MIXED, GEN, FORM, MODE>), dim3(grid), dim3(256), 0, s, A,
+    hipLaunchKernelGGL((dense_mfma_kernel<MIXED, GEN, FORM, MODE, NARROW>), dim3(grid), dim3(256), 0, s, A,
                        (const T*)A.Uk, (const T*)A.Ucol, (const T*)A.ak, (const T*)A.bk,
                        (const T*)A.invk, Y, A.part_m, A.part_e, (T*)A.srec, (const T*)A.yrec,
                        DBG_POINT_ARGS(A.pts));
@@ -406,6 +414,15 @@ hipError_t launch_dense_mfma(const DenseArgs& A, bool mixed, int gen, int form, 
         else
             launch_m<false, kGenSobol, kPhiTail>(A, grid, s);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_dense_mfma_refine(const DenseArgs& A, int gen, int grid, hipStream_t s)
+{
+    if (gen == kGenRichtmyer)
+        launch_m<false, kGenRichtmyer, kPhiTail, kModeIntegrate, true>(A, grid, s);
+    else
+        launch_m<false, kGenSobol, kPhiTail, kModeIntegrate, true>(A, grid, s);
     return hipGetLastError();
 }
 
