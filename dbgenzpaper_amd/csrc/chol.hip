@@ -442,12 +442,60 @@ struct RegShared {
     double ci[kRegNb], xch[kRegNb];
     double rv[kRegCols / 64];
     int ri[kRegCols / 64];
-    double uii, yi;
-    int im, stop;
+    // a pivot's swap of two indices' entries: [0] index i's, [1] index im's
+    double x_dsum[2], x_sv[2], x_a[2], x_b[2], x_dg[2];
+    int x_perm[2];
 };
 
+// a 64-bit lane value moved by a DPP control (rows of 16 lanes and the row
+// broadcasts); lanes without a source, or outside row_mask, keep their own
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v)
+{
+    const int lo = (int)(uint32_t)v, hi = (int)(uint32_t)(v >> 32);
+    const int l = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xf, false);
+    const int h = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xf, false);
+    return ((uint64_t)(uint32_t)h << 32) | (uint32_t)l;
+}
+
+// minimum of a 64-bit key over the (full) wavefront, in every lane: inclusive
+// row minima (row_shr 1, 2, 4, 8), rows 1 and 3 take row 0 and 2 (row_bcast
+// 15), rows 2 and 3 take row 1 (row_bcast 31); lane 63 then holds the minimum
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t k)
+{
+    k = min(k, dpp_u64<0x111, 0xf>(k));
+    k = min(k, dpp_u64<0x112, 0xf>(k));
+    k = min(k, dpp_u64<0x114, 0xf>(k));
+    k = min(k, dpp_u64<0x118, 0xf>(k));
+    k = min(k, dpp_u64<0x142, 0xa>(k));
+    k = min(k, dpp_u64<0x143, 0xc>(k));
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)k, 63);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(k >> 32), 63);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// An unsigned key whose order is the pivot mode's preference on one
+// candidate's value: for the argmin of cholesky_genz! NaN first, then
+// ascending; for the argmax of cholesky_rowmax! descending (NaN is never a
+// candidate there).  -0 and +0 get one key, as they compare equal; equal keys
+// are then decided by the smaller index.
+__device__ __forceinline__ uint64_t pivot_key(int pivot, double v)
+{
+    if (isnan(v)) return 0;
+    const uint64_t b = (uint64_t)__double_as_longlong(v + 0.0);  // -0 -> +0
+    const uint64_t asc = (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+    return pivot == kPivotRowMax ? ~asc : asc;
+}
+
 // One pivot i = ib0 + 16 ng + E of the inner block; returns false when the
-// block ends (rank deficiency).  All branches are uniform.
+// block ends (rank deficiency).  All branches are uniform.  Two barriers per
+// pivot: after the search (each wavefront's best candidate in LDS) and after
+// the swap.  Every thread then reduces the wavefront minima and evaluates the
+// pivot's diagonal and conditional mean itself (the same operations on the
+// same LDS values, so all threads agree), and the per-index state in LDS is
+// only ever written by the thread that owns the index: the two threads of a
+// swap read each other's entries before the second barrier and write their
+// own after it, when nobody else reads them until the next pivot's first.
 template <int PIVOT, int E>
 __device__ __forceinline__ bool reg_pivot(const CholArgs& C, RegShared& S, ColRegs& R, int ng, int ib0, int w1,
                                           double eps0)
@@ -461,16 +509,15 @@ __device__ __forceinline__ bool reg_pivot(const CholArgs& C, RegShared& S, ColRe
     const int i = ib0 + q;
     const int c = ib0 + tid;
     const bool own = c < w1;
+    const int a0 = i - ib0;
     const double c2 = 1.0 / sqrt(2.0);
     const double is2pi = 2.0 / sqrt(2.0 * 3.14159265358979323846);
+    const double none_v = pivot == kPivotRowMax ? -INFINITY : INFINITY;
     // ---- pivot search over the rest of the window (src/cholesky.jl:500-516)
-    double bv = NAN;
-    int bi = INT32_MAX;
-    double best_order = 0.0;
     if (pivot) {
-        bv = pivot == kPivotRowMax ? -INFINITY : INFINITY;
+        double o = 0.0;
+        bool cand = false;
         if (own && c >= i) {
-            double o;
             if (pivot == kPivotRowMax) {
                 o = S.dg[tid] - S.dsum[tid];
             } else {
@@ -480,92 +527,159 @@ __device__ __forceinline__ bool reg_pivot(const CholArgs& C, RegShared& S, ColRe
                 const double b_k = (S.b[tid] - S.sv[tid]) / m_kk * c2;
                 o = erf(b_k) - erf(a_k);
             }
-            if (pivot_better(pivot, o, c, bv, bi)) {
-                bv = o;
-                bi = c;
-            }
+            // rowmax takes no NaN (src/cholesky.jl:319-327: a strict > from i)
+            cand = pivot != kPivotRowMax || !isnan(o);
         }
-        pivot_wave_reduce(pivot, bv, bi);
+        // the wavefront's best: the smallest key, then the smallest index
+        // (lane) among the candidates that hold it
+        const uint64_t key = cand ? pivot_key(pivot, o) : ~0ull;
+        const uint64_t kmin = wave_min_u64(key);
+        const uint64_t hit = __ballot(cand && key == kmin);
         if ((tid & 63) == 0) {
-            S.rv[tid >> 6] = bv;
-            S.ri[tid >> 6] = bi;
-        }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        int im = i;
-        if (pivot) {
-            bv = S.rv[0];
-            bi = S.ri[0];
-            for (int w = 1; w < kRegCols / 64; ++w)
-                if (pivot_better(pivot, S.rv[w], S.ri[w], bv, bi)) {
-                    bv = S.rv[w];
-                    bi = S.ri[w];
-                }
-            im = bi;
-            best_order = bv;
-            // rowmax: a NaN at i keeps i (nothing compares greater)
-            if (pivot == kPivotRowMax && (bi == INT32_MAX || isnan(S.dg[i - ib0] - S.dsum[i - ib0]))) im = i;
-        }
-        const int a0 = i - ib0;
-        const double m_ii = S.dg[im - ib0] - S.dsum[im - ib0];  // read before the swap
-        // rank: genz stops at m_ii <= eps0 after the swap (src/cholesky.jl:518-541),
-        // rowmax at m_ii <= eps0 before it, eps0 then being the rank tolerance
-        // (0 for the reference's m_ii <= 0, :329-350)
-        const bool stop = !(m_ii > eps0) && (pivot == kPivotGenz || pivot == kPivotRowMax);
-        if (stop && pivot == kPivotRowMax) im = i;
-        const int a1 = im - ib0;
-        if (im != i) {
-            int tp = S.perm[a0]; S.perm[a0] = S.perm[a1]; S.perm[a1] = tp;
-            double t;
-            t = S.dsum[a0]; S.dsum[a0] = S.dsum[a1]; S.dsum[a1] = t;
-            t = S.sv[a0]; S.sv[a0] = S.sv[a1]; S.sv[a1] = t;
-            if (pivot != kPivotRowMax) {  // rowmax leaves a and b in place
-                t = S.a[a0]; S.a[a0] = S.a[a1]; S.a[a1] = t;
-                t = S.b[a0]; S.b[a0] = S.b[a1]; S.b[a1] = t;
+            if (hit) {
+                const int lane = __builtin_ctzll(hit);
+                const uint64_t ob = (uint64_t)__double_as_longlong(o);
+                const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)ob, lane);
+                const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(ob >> 32), lane);
+                S.rv[tid >> 6] = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+                S.ri[tid >> 6] = ib0 + tid + lane;
+            } else {
+                S.rv[tid >> 6] = none_v;
+                S.ri[tid >> 6] = INT32_MAX;
             }
-            t = S.dg[a0]; S.dg[a0] = S.dg[a1]; S.dg[a1] = t;
-        }
-        S.im = im;
-        if (stop) {
-            S.stop = 1;
-            *C.rank = i;
-        } else {
-            const double uii = sqrt(m_ii);
-            S.uii = uii;
-            const double a_norm = (S.a[a0] - S.sv[a0]) / uii;
-            const double b_norm = (S.b[a0] - S.sv[a0]) / uii;
-            S.yi = pivot == kPivotGenz
-                       ? (exp(-(a_norm * a_norm) / 2) - exp(-(b_norm * b_norm) / 2)) * is2pi / best_order
-                       : 0.0;
         }
     }
+    // also orders the previous pivot's reads of S.ci / S.xch and its row
+    // stores before this pivot's swap
     __syncthreads();
-    const int im = S.im;
-    if (S.stop) return false;
-    // ---- symmetric swap i <-> im; each window thread keeps the new M[i, c]
+    int im = i;
+    double best_order = 0.0;
+    if (pivot) {
+        // the wavefront minima, in wavefront order (as one thread would):
+        // every thread reads the same eight entries
+        double bv = S.rv[0];
+        int bi = S.ri[0];
+#pragma unroll
+        for (int w = 1; w < kRegCols / 64; ++w)
+            if (pivot_better(pivot, S.rv[w], S.ri[w], bv, bi)) {
+                bv = S.rv[w];
+                bi = S.ri[w];
+            }
+        // uniform from here: kept in scalar registers
+        bi = __builtin_amdgcn_readfirstlane(bi);
+        const uint64_t bb = __double_as_longlong(bv);
+        bv = __longlong_as_double((long long)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(bb >> 32)) << 32) |
+                                              (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)bb)));
+        im = bi;
+        best_order = bv;
+        // rowmax: a NaN at i keeps i (nothing compares greater)
+        if (pivot == kPivotRowMax && (bi == INT32_MAX || isnan(S.dg[a0] - S.dsum[a0]))) im = i;
+    }
+    const double m_ii = S.dg[im - ib0] - S.dsum[im - ib0];  // read before the swap
+    // rank: genz stops at m_ii <= eps0 after the swap (src/cholesky.jl:518-541),
+    // rowmax at m_ii <= eps0 before it, eps0 then being the rank tolerance
+    // (0 for the reference's m_ii <= 0, :329-350)
+    const bool stop = !(m_ii > eps0) && (pivot == kPivotGenz || pivot == kPivotRowMax);
+    if (stop && pivot == kPivotRowMax) im = i;
+    const int a1 = im - ib0;
+    // the swap of the per-index state: the owners of i and im copy their
+    // entries to LDS staging now and take each other's after the barrier
+    // below (or at once when the block stops here: nobody reads them again in
+    // this launch); no registers hold them across the swap
+    const bool swp = im != i && (tid == a0 || tid == a1);
+    if (swp) {
+        const int k = tid == a0 ? 0 : 1;
+        S.x_perm[k] = S.perm[tid];
+        S.x_dsum[k] = S.dsum[tid];
+        S.x_sv[k] = S.sv[tid];
+        S.x_dg[k] = S.dg[tid];
+        S.x_a[k] = S.a[tid];
+        S.x_b[k] = S.b[tid];
+    }
+    auto take_swapped = [&]() {
+        if (swp) {
+            const int k = tid == a0 ? 1 : 0;
+            S.perm[tid] = S.x_perm[k];
+            S.dsum[tid] = S.x_dsum[k];
+            S.sv[tid] = S.x_sv[k];
+            S.dg[tid] = S.x_dg[k];
+            if (pivot != kPivotRowMax) {  // rowmax leaves a and b in place
+                S.a[tid] = S.x_a[k];
+                S.b[tid] = S.x_b[k];
+            }
+        }
+    };
+    if (stop) {
+        __syncthreads();  // every thread has read the entries of i and im, and the staging is complete
+        take_swapped();
+        if (tid == 0) *C.rank = i;
+        return false;
+    }
+    const double uii = sqrt(m_ii);
+    double yi = 0.0;
+    if (pivot == kPivotGenz) {
+        // position i now holds column im's bounds and conditional mean
+        const double a_norm = (S.a[a1] - S.sv[a1]) / uii;
+        const double b_norm = (S.b[a1] - S.sv[a1]) / uii;
+        yi = (exp(-(a_norm * a_norm) / 2) - exp(-(b_norm * b_norm) / 2)) * is2pi / best_order;
+    }
+    // ---- symmetric swap i <-> im; each window thread keeps the new M[i, c].
+    // Every element moves once; the loads of a thread's first rows and
+    // columns go out together, so the swap costs one memory round trip.
     double mic = 0.0;
     bool have = false;
     if (im != i) {
-        for (int r = tid; r < i; r += kRegCols) {  // rows r < i: columns i and im
+        constexpr int U = 2;
+        double ri[U], rm[U], ci_[U], cm[U];
+        const int rend = min(i, U * kRegCols), cend = min(n, w1 + U * kRegCols);
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const int r = tid + k * kRegCols, cc = w1 + tid + k * kRegCols;
+            if (r < rend) {  // rows r < i: columns i and im
+                ri[k] = MM(r, i);
+                rm[k] = MM(r, im);
+            }
+            if (cc < cend) {  // columns beyond the window: rows i and im
+                ci_[k] = MM(i, cc);
+                cm[k] = MM(im, cc);
+            }
+        }
+        double x = 0.0, y = 0.0;
+        const bool mid = own && c > i && c < im, past = own && c > im;
+        if (mid) {  // M[i, c] <-> M[c, im]
+            x = MM(i, c);
+            y = MM(c, im);
+        } else if (past) {  // M[i, c] <-> M[im, c]
+            x = MM(i, c);
+            y = MM(im, c);
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const int r = tid + k * kRegCols, cc = w1 + tid + k * kRegCols;
+            if (r < rend) {
+                MM(r, i) = rm[k];
+                MM(r, im) = ri[k];
+            }
+            if (cc < cend) {
+                MM(i, cc) = cm[k];
+                MM(im, cc) = ci_[k];
+            }
+        }
+        if (mid) {
+            MM(i, c) = y;
+            MM(c, im) = x;
+        } else if (past) {
+            MM(i, c) = y;
+            MM(im, c) = x;
+        }
+        mic = y;
+        have = mid || past;
+        for (int r = rend + tid; r < i; r += kRegCols) {  // the rest (i > U kRegCols)
             const double t = MM(r, i);
             MM(r, i) = MM(r, im);
             MM(r, im) = t;
         }
-        if (own && c > i && c < im) {  // M[i, c] <-> M[c, im]
-            const double x = MM(i, c), y = MM(c, im);
-            MM(i, c) = y;
-            MM(c, im) = x;
-            mic = y;
-            have = true;
-        } else if (own && c > im) {  // M[i, c] <-> M[im, c]
-            const double x = MM(i, c), y = MM(im, c);
-            MM(i, c) = y;
-            MM(im, c) = x;
-            mic = y;
-            have = true;
-        }
-        for (int cc = w1 + tid; cc < n; cc += kRegCols) {  // columns beyond the window
+        for (int cc = cend + tid; cc < n; cc += kRegCols) {
             const double t = MM(i, cc);
             MM(i, cc) = MM(im, cc);
             MM(im, cc) = t;
@@ -573,15 +687,14 @@ __device__ __forceinline__ bool reg_pivot(const CholArgs& C, RegShared& S, ColRe
         if (tid == 0) MM(im, im) = MM(i, i);  // MM(i, i) becomes U_ii below
         // the block rows of columns i and im are in registers as well: column
         // i's new rows come from the thread of im, which takes column i's old ones
-        if (tid == im - ib0) regs_store<E>(R, ng, S.ci);
-        if (tid == i - ib0) regs_store<E>(R, ng, S.xch);
-    } else if (tid == i - ib0) {
+        if (tid == a1) regs_store<E>(R, ng, S.ci);
+        if (tid == a0) regs_store<E>(R, ng, S.xch);
+    } else if (tid == a0) {
         regs_store<E>(R, ng, S.ci);
     }
     __syncthreads();
-    if (im != i && tid == im - ib0) regs_load<E>(R, ng, S.xch);
-    const double uii = S.uii;
-    const double yi = S.yi;
+    take_swapped();
+    if (im != i && tid == a1) regs_load<E>(R, ng, S.xch);
     if (tid == 0) MM(i, i) = uii;
     // ---- row i of U inside the window (src/cholesky.jl:547-570)
     if (own && c > i) {
@@ -592,7 +705,6 @@ __device__ __forceinline__ bool reg_pivot(const CholArgs& C, RegShared& S, ColRe
         S.dsum[tid] += u * u;
         S.sv[tid] += yi * u;
     }
-    __syncthreads();
     return true;
 }
 
@@ -630,7 +742,6 @@ __global__ __launch_bounds__(kRegCols, 1) void chol_window_reg_kernel(CholArgs C
         S.dg[tid] = MM(c, c);
         S.dsum[tid] = 0.0;  // d_vec reset at the block start (src/cholesky.jl:494-496)
     }
-    if (tid == 0) S.stop = 0;
     ColRegs R;
 #pragma unroll
     for (int e = 0; e < 16; ++e) R.g0[e] = R.g1[e] = R.g2[e] = R.cur[e] = 0.0;
