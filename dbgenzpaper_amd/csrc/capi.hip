@@ -137,10 +137,14 @@ int check_block_sizes(int bs1, int bs2, int bs3, int bs4)
 // Copy the inputs (host or device memory, hipMemcpyDefault) and factor.
 // pivot: 0 cholesky_classic!, 1 cholesky_genz!, 2 cholesky_rowmax!.
 // rowmax_tol: the rank check of the row-max pivoting is m_ii > rowmax_tol
-// (0 for cholesky_rowmax! as the reference has it)
+// (0 for cholesky_rowmax! as the reference has it).  keep_stop: leave the
+// solves after a rank stop undone, as the reference does, which reads the
+// rank back after every pivot window; a caller that discards the factor of a
+// rank-deficient matrix passes false and the windows follow each other
+// without a host round trip (the later pivot blocks then exit at once).
 int factor_on_device(CholBuffers& B, int n, const double* M_in, int64_t lda, const double* a_in,
                      const double* b_in, int bs1, int bs2, int bs3, int bs4, int pivot,
-                     hipStream_t s, int32_t* rank_out, double rowmax_tol = 0.0)
+                     hipStream_t s, int32_t* rank_out, double rowmax_tol = 0.0, bool keep_stop = true)
 {
     int rc = check_block_sizes(bs1, bs2, bs3, bs4);
     if (rc) return rc;
@@ -180,7 +184,7 @@ int factor_on_device(CholBuffers& B, int n, const double* M_in, int64_t lda, con
             for (int ii0 = iii0; ii0 < iii1; ii0 += bs2) {
                 const int ii1 = std::min(iii1, ii0 + bs2);
                 HIPCHK(chol_window(C, ii0, ii1, nb, pivot, s));
-                if (pivot) {
+                if (pivot && keep_stop) {
                     HIPCHK(hipMemcpyAsync(&rank, B.rank, sizeof(int32_t), hipMemcpyDeviceToHost, s));
                     HIPCHK(hipStreamSynchronize(s));
                     if (rank < n) {
@@ -525,8 +529,10 @@ int plan_factor(dbg_plan* P)
     hipStream_t s = P->stream;
     const bool pivot = (o.kind == DBG_KIND_DENSE);
     P->L.on = false;
+    // a rank stop sends the plan to lowrank_factor, which factors Sigma again:
+    // this factor is then discarded, so no host round trip per window
     int rc = factor_on_device(P->F, n, P->sig, n, P->a0, P->b0, o.chol_block_size, o.chol_block_size2,
-                              o.chol_block_size3, o.chol_block_size4, pivot ? 1 : 0, s, &P->rank);
+                              o.chol_block_size3, o.chol_block_size4, pivot ? 1 : 0, s, &P->rank, 0.0, false);
     if (rc) return rc;
     if (pivot && P->rank != n) return lowrank_factor(P);
     // truncate_matrixU!: |U_ij| < eps(T) / (9n) -> 0.  A refining mixed plan
