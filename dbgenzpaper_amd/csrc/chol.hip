@@ -817,9 +817,14 @@ rocblas_handle blas_handle(hipStream_t s)
     return handles[dev];
 }
 
-// trailing update M[x, y] -= sum_{l in [k0, k1)} U[l, x] U[l, y] for r0 <= x <= y < r1
+// trailing update M[x, y] -= sum_{l in [k0, k1)} U[l, x] U[l, y] for r0 <= x <= y < r1:
+// per element one FMA chain in ascending l from zero, then the subtraction.
+// 64 x 64 tiles, 4 x 4 per thread; stages of 32 l in two LDS buffers, the
+// next stage's loads held in registers while the current one is multiplied,
+// so a stage costs one barrier and its load latency hides behind the products
 constexpr int kSyTile = 64;
-constexpr int kSyK = 16;
+constexpr int kSyK = 32;
+constexpr int kSyLd = kSyK * kSyTile / 256;  // elements of each operand a thread stages
 __global__ __launch_bounds__(256) void chol_syrk_kernel(double* M, int64_t lda, int n, int k0, int k1,
                                                         int r0, int r1, const int32_t* rank)
 {
@@ -827,7 +832,7 @@ __global__ __launch_bounds__(256) void chol_syrk_kernel(double* M, int64_t lda, 
     const int tx = blockIdx.x, ty = blockIdx.y;
     if (tx > ty) return;
     const int x0 = r0 + tx * kSyTile, y0 = r0 + ty * kSyTile;
-    __shared__ double Ax[kSyK][kSyTile + 1], Ay[kSyK][kSyTile + 1];
+    __shared__ double Ax[2][kSyK][kSyTile + 1], Ay[2][kSyK][kSyTile + 1];
     const int t = threadIdx.x;
     const int tr = t % 16, tc = t / 16;
     double acc[4][4];
@@ -835,27 +840,48 @@ __global__ __launch_bounds__(256) void chol_syrk_kernel(double* M, int64_t lda, 
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
-    for (int l0 = k0; l0 < k1; l0 += kSyK) {
-        for (int e = t; e < kSyK * kSyTile; e += 256) {
-            const int ll = e % kSyK, cc = e / kSyK;
-            const int l = l0 + ll;
-            Ax[ll][cc] = (l < k1 && x0 + cc < r1) ? MM(l, x0 + cc) : 0.0;
-            Ay[ll][cc] = (l < k1 && y0 + cc < r1) ? MM(l, y0 + cc) : 0.0;
+    // staging: element e = t + 256 q is (l0 + e mod kSyK, column e / kSyK)
+    double ra[kSyLd], rb[kSyLd];
+    auto fetch = [&](int l0) {
+#pragma unroll
+        for (int q = 0; q < kSyLd; ++q) {
+            const int e = t + 256 * q;
+            const int l = l0 + e % kSyK, cc = e / kSyK;
+            ra[q] = (l < k1 && x0 + cc < r1) ? MM(l, x0 + cc) : 0.0;
+            rb[q] = (l < k1 && y0 + cc < r1) ? MM(l, y0 + cc) : 0.0;
         }
-        __syncthreads();
+    };
+    auto put = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < kSyLd; ++q) {
+            const int e = t + 256 * q;
+            Ax[buf][e % kSyK][e / kSyK] = ra[q];
+            Ay[buf][e % kSyK][e / kSyK] = rb[q];
+        }
+    };
+    fetch(k0);
+    put(0);
+    __syncthreads();
+    int buf = 0;
+    for (int l0 = k0; l0 < k1; l0 += kSyK) {
+        const bool more = l0 + kSyK < k1;  // uniform
+        if (more) fetch(l0 + kSyK);
 #pragma unroll 4
         for (int ll = 0; ll < kSyK; ++ll) {
             double xa[4], ya[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) xa[i] = Ax[ll][tr * 4 + i];
+            for (int i = 0; i < 4; ++i) xa[i] = Ax[buf][ll][tr * 4 + i];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) ya[j] = Ay[ll][tc * 4 + j];
+            for (int j = 0; j < 4; ++j) ya[j] = Ay[buf][ll][tc * 4 + j];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) acc[i][j] = fma(xa[i], ya[j], acc[i][j]);
         }
+        // the other buffer was last read before the previous stage's barrier
+        if (more) put(buf ^ 1);
         __syncthreads();
+        buf ^= 1;
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
