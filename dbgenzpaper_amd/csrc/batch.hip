@@ -32,6 +32,40 @@ hipError_t queue_reset(const BatchArgs& B, hipStream_t s)
     return hipGetLastError();
 }
 
+// One work item's walk over its problem: the register-blocked contraction of
+// each block of W dimensions, then walk_block; FORM kPhiTailLower for a
+// problem whose lower bounds are all -inf (mixed only)
+template <bool MIXED, int W, int FORM, typename T>
+__device__ __forceinline__ Weight<T> batch_item(const PointArgs& pts, const BatchProblem& P, const double* f,
+                                                int r, int64_t j, int tid, T* __restrict__ s_x,
+                                                T* __restrict__ s_y, T* __restrict__ Ys,
+                                                const T* __restrict__ Uk, const T* __restrict__ Ucol,
+                                                const T* __restrict__ ak, const T* __restrict__ bk,
+                                                const T* __restrict__ invk)
+{
+    constexpr int SXS = W + 4;
+    const int d = P.d, nd = d - 1;
+    typename Path<MIXED, FORM>::State st;
+    Weight<T> w;
+    Path<MIXED, FORM>::init(st, w, f);
+    for (int g0 = 0; g0 < nd; g0 += W) {
+        T s[W];
+#pragma unroll
+        for (int c = 0; c < W; ++c) s[c] = T(0);
+        for (int l = 0; l < g0; ++l) {
+            const T y = Ys[(size_t)l * kTileB + tid];
+            const T* __restrict__ ur = Uk + (size_t)l * P.ldu + g0 + 1;
+#pragma unroll
+            for (int c = 0; c < W; ++c) s[c] = fma(y, ur[c], s[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < W; ++c) s_x[tid * SXS + c] = s[c];
+        walk_block<MIXED, kGenRichtmyer, FORM, kTileB, SXS, W>(pts, g0, min(W, nd - g0), r, j, tid, st, w, s_x,
+                                                               s_y, Ys, kTileB, Ucol, d, ak, bk, invk);
+    }
+    return w;
+}
+
 template <bool MIXED, int W, typename T = typename std::conditional<MIXED, float, double>::type>
 __global__ __launch_bounds__(kTileB) void batch_integrate_kernel(BatchArgs B, const T* __restrict__ Ukall,
                                                                  const T* __restrict__ Ucall,
@@ -70,7 +104,7 @@ __global__ __launch_bounds__(kTileB) void batch_integrate_kernel(BatchArgs B, co
         const int p = wi.x;
         const int lt = wi.y;
         const BatchProblem P = B.probs[p];
-        const int d = P.d, nd = d - 1;
+        const int nd = P.d - 1;
         const int r = lt / P.tiles_per_rep;
         const int t = lt - r * P.tiles_per_rep;
         const int64_t j = (int64_t)t * kTileB + tid + 1;
@@ -89,24 +123,12 @@ __global__ __launch_bounds__(kTileB) void batch_integrate_kernel(BatchArgs B, co
         pts.q64 = B.q64;
         pts.shift32 = B.shift32 + P.shift_off;
 
-        typename Path<MIXED, kPhiErf>::State st;
         Weight<T> w;
-        Path<MIXED, kPhiErf>::init(st, w, f);
-        for (int g0 = 0; g0 < nd; g0 += W) {
-            T s[W];
-#pragma unroll
-            for (int c = 0; c < W; ++c) s[c] = T(0);
-            for (int l = 0; l < g0; ++l) {
-                const T y = Ys[(size_t)l * kTileB + tid];
-                const T* __restrict__ ur = Uk + (size_t)l * P.ldu + g0 + 1;
-#pragma unroll
-                for (int c = 0; c < W; ++c) s[c] = fma(y, ur[c], s[c]);
-            }
-#pragma unroll
-            for (int c = 0; c < W; ++c) s_x[tid * SXS + c] = s[c];
-            walk_block<MIXED, kGenRichtmyer, kPhiErf, kTileB, SXS, W>(pts, g0, min(W, nd - g0), r, j, tid, st,
-                                                                     w, s_x, s_y, Ys, kTileB, Ucol, d, ak, bk, invk);
-        }
+        if (MIXED && P.lower)  // uniform: one problem per item
+            w = batch_item<MIXED, W, MIXED ? kPhiTailLower : kPhiErf>(pts, P, f, r, j, tid, s_x, s_y, Ys, Uk, Ucol,
+                                                                      ak, bk, invk);
+        else
+            w = batch_item<MIXED, W, kPhiErf>(pts, P, f, r, j, tid, s_x, s_y, Ys, Uk, Ucol, ak, bk, invk);
         const double pm = valid ? (double)w.pm : 0.0;
         double S;
         int E;

@@ -767,6 +767,7 @@ int run_loop(dbg_plan* P, int64_t block, int64_t n_pts0)
         A.y_slot = P->y_slot;
         A.part_m = P->part_m;
         A.part_e = P->part_e;
+        A.a_inf = P->a_inf ? 1 : 0;
         if (P->split)
             HIPCHK(launch_dense_split(A, gen, P->uc16, P->grid, P->stream));
         else if (P->use_mfma)
@@ -1620,7 +1621,8 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
         P.d = d;
         P.tiles_per_rep = (int32_t)tpr;
         P.ldu = d + 32;
-        P.pad = 0;
+        P.lower = 1;
+        for (int i = 0; i < d; ++i) P.lower = P.lower && std::isinf(a[vec + i]) && a[vec + i] < 0;
         P.sig_off = sig;
         P.ucol_off = sig;
         P.vec_off = vec;

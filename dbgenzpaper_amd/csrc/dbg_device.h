@@ -256,7 +256,12 @@ __device__ __forceinline__ uint32_t sobol_u32_from_x(const PointArgs& P, uint32_
 // from the fixed-point u; u = 0 gives w = 1.
 __device__ __forceinline__ void tent_w(uint32_t u, float& w, float& wc)
 {
-    const uint32_t tw = (u & 0x80000000u) ? (u << 1) : (0u - (u << 1));
+    // tw = 2u mod 2^32 when u >= 1/2, else 2^32 - 2u (mod 2^32), without a
+    // branch: nm is all ones exactly when the top bit of u is clear, and
+    // (v ^ nm) - nm negates v then
+    const uint32_t u2 = u << 1;
+    const uint32_t nm = ~(uint32_t)((int32_t)u >> 31);
+    const uint32_t tw = (u2 ^ nm) - nm;
     constexpr float kS = 2.3283064365386963e-10f;  // 2^-32
     w = (u == 0u) ? 1.0f : (float)tw * kS;
     wc = (u == 0u) ? 0.0f : (tw == 0u ? 1.0f : (float)(0u - tw) * kS);
@@ -392,12 +397,37 @@ __device__ __forceinline__ void tail_from_z(StateTail<float>& st, float za, floa
     float mb = tail_mass_sel<CALLS>(fabsf(zb), kb);
     if (isinf(za)) ma = 0.0f;
     if (isinf(zb)) mb = 0.0f;
-    const int mode = (zb <= 0.0f) ? 0 : (za >= 0.0f) ? 1 : 2;
-    const int k = (mode == 0) ? kb : (mode == 1) ? ka : 0;
-    st.ea = ldexpf(ma, k - ka);
-    st.eb = ldexpf(mb, k - kb);
-    st.dc = (mode == 0) ? st.eb - st.ea : (mode == 1) ? st.ea - st.eb : (1.0f - st.ea) - st.eb;
-    st.mode = mode;
+    const bool lo = zb <= 0.0f, hi = !lo && za >= 0.0f;
+    const int k = lo ? kb : (hi ? ka : 0);
+    const float ea = ldexpf(ma, k - ka), eb = ldexpf(mb, k - kb);
+    // the three widths are formed and one selected, so that the step has no
+    // divergent branch here
+    const float d_lo = eb - ea, d_hi = ea - eb, d_mid = (1.0f - ea) - eb;
+    st.ea = ea;
+    st.eb = eb;
+    st.dc = lo ? d_lo : (hi ? d_hi : d_mid);
+    st.mode = lo ? 0 : (hi ? 1 : 2);
+    st.k = k;
+}
+
+// The same state when the lower bound is -inf (za = -inf): ea = 0, and the
+// interval is either below 0 (mode 0, dc = eb) or straddles it (mode 2,
+// dc = 1 - eb).  tail_from_z gives the same values for za = -inf (its lower
+// mass is set to 0 and every select above reduces to these); here the lower
+// bound's mass is never formed.
+template <bool CALLS = false>
+__device__ __forceinline__ void tail_from_zb(StateTail<float>& st, float zb)
+{
+    int kb;
+    float mb = tail_mass_sel<CALLS>(fabsf(zb), kb);
+    if (isinf(zb)) mb = 0.0f;
+    const bool lo = zb <= 0.0f;
+    const int k = lo ? kb : 0;
+    const float eb = ldexpf(mb, k - kb);
+    st.ea = 0.0f;
+    st.eb = eb;
+    st.dc = lo ? eb : 1.0f - eb;
+    st.mode = lo ? 0 : 2;
     st.k = k;
 }
 
@@ -492,7 +522,8 @@ struct Path<false, kPhiTail> {
     static __device__ __forceinline__ void weigh(Weight<T>& w, const State& st) { w.mul(st.dc); }
 };
 
-// the FP32 mixed path (tail form, any FORM)
+// the FP32 mixed path (tail form, any FORM; kPhiTailLower when every lower
+// bound is -inf)
 template <int FORM>
 struct Path<true, FORM> {
     using T = float;
@@ -506,7 +537,8 @@ struct Path<true, FORM> {
     static __device__ __forceinline__ T sample_w(const State& st, double w) { return tail_sample_w(st, w); }
     static __device__ __forceinline__ void phi(State& st, T s, T a, T b, T inv)
     {
-        tail_from_z(st, (a - s) * inv, (b - s) * inv);
+        if constexpr (FORM == kPhiTailLower) tail_from_zb(st, (b - s) * inv);
+        else tail_from_z(st, (a - s) * inv, (b - s) * inv);
     }
     static __device__ __forceinline__ void weigh(Weight<T>& w, const State& st)
     {

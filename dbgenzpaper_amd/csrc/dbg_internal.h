@@ -7,7 +7,10 @@
 namespace dbg {
 
 enum { kGenRichtmyer = 0, kGenSobol = 1 };
-enum { kPhiErf = 0, kPhiErfc = 1, kPhiTail = 2 };
+// kPhiTailLower: the tail form for a lower orthant (every lower bound -inf),
+// an internal specialisation of the mixed walk that forms only the upper
+// bound's tail mass (the lower one is exactly 0); same values as kPhiTail
+enum { kPhiErf = 0, kPhiErfc = 1, kPhiTail = 2, kPhiTailLower = 3 };
 
 // Where the points of one round (one block of m points per shift) come from.
 struct PointArgs {
@@ -67,7 +70,7 @@ struct DenseArgs {
     double* ga;                // reverse: d log P / d a', b', U_rr accumulators (n each)
     double* gb;
     double* gd;
-    int a_inf;                 // reverse: every lower bound is -inf (d log P / d a' is 0)
+    int a_inf;                 // every lower bound is -inf (mixed walk: kPhiTailLower; reverse: d log P / d a' is 0)
 };
 enum { kModeIntegrate = 0, kModeRecord = 1, kModeReverse = 2 };
 
@@ -177,7 +180,7 @@ struct BatchProblem {
     int64_t shift_off;  // offset of the shifts ((d - 1) x n_reps doubles)
     int64_t part_off;   // offset of the problem's tile partials (n_reps x tiles_per_rep)
     int32_t ldu;
-    int32_t pad;
+    int32_t lower;      // 1 when every lower bound of the problem is -inf (the mixed walk's kPhiTailLower)
 };
 
 struct BatchArgs {

@@ -5,11 +5,10 @@ mass:      erfc(z) = t exp(-z^2 + P(t)) with t = 2 / (2 + z) defines P on t in
            as z -> inf.  P is fitted as a polynomial in u = 2t - 1 by least
            squares on 600 Chebyshev nodes of t, checked on 6001 uniform points.
 quantile:  y = Phi^{-1}(p) for p <= 1/2 as a function of w = -log(4p(1 - p)),
-           in three regions, each fitted for least relative error on 500
+           in two regions, each fitted for least relative error on 500
            Chebyshev nodes:  y / (2p - 1) as a polynomial in w - 3.125 for
-           w < 6.25;  -y as a polynomial in s - 4.25, s = sqrt(w), for
-           s in [2.5, 6);  -y / s as a polynomial in 1/s - 0.09333 for s in
-           [6, 50] (p down to about 2^-3600).
+           w < 6.25;  -y / s as a polynomial in 1/s - 0.21, s = sqrt(w), for
+           s in [2.5, 50] (p from 4.8e-4 down to about 2^-3600).
 Values come from mpmath at 50-60 digits.  Prints the maximum relative fit
 error and the coefficients, highest power first, rounded to FP32.
 Run: python fit_tailq.py mass | quantile
@@ -75,13 +74,9 @@ def fit_quantile(nodes=500):
         y, p = quantile_of_w(w)
         f1.append(float(y / (2 * p - 1)))
     show("C1, region w < 6.25", *relfit(ws - 3.125, np.array(f1), 10))
-    s2 = cheb(2.5, 6.0, nodes)
-    g2 = np.array([float(-quantile_of_w(s * s)[0]) for s in s2])
-    show("C2, region s in [2.5, 6)", *relfit(s2 - 4.25, g2, 9))
-    s3 = cheb(6.0, 50.0, nodes)
-    g3 = np.array([float(-quantile_of_w(s * s)[0] / s) for s in s3])
-    show("C3, region s in [6, 50]", *relfit(1 / s3 - (1 / 6 + 1 / 50) / 2, g3, 9))
-
+    s2 = cheb(2.5, 50.0, nodes)
+    g2 = np.array([float(-quantile_of_w(s * s)[0] / s) for s in s2])
+    show("CB, region s in [2.5, 50]", *relfit(1 / s2 - 0.21, g2, 12))
 
 if __name__ == "__main__":
     which = sys.argv[1] if len(sys.argv) > 1 else "mass"

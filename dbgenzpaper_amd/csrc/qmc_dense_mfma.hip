@@ -395,10 +395,14 @@ hipError_t launch_dense_mfma(const DenseArgs& A, bool mixed, int gen, int form, 
                              hipStream_t s)
 {
     if (mixed) {
-        if (gen == kGenRichtmyer)
-            launch_m<true, kGenRichtmyer, kPhiTail>(A, grid, s);
-        else
-            launch_m<true, kGenSobol, kPhiTail>(A, grid, s);
+        // a lower orthant takes the walk that forms only the upper bound's tail mass
+        if (gen == kGenRichtmyer) {
+            if (A.a_inf) launch_m<true, kGenRichtmyer, kPhiTailLower>(A, grid, s);
+            else launch_m<true, kGenRichtmyer, kPhiTail>(A, grid, s);
+        } else {
+            if (A.a_inf) launch_m<true, kGenSobol, kPhiTailLower>(A, grid, s);
+            else launch_m<true, kGenSobol, kPhiTail>(A, grid, s);
+        }
     } else if (gen == kGenRichtmyer) {
         if (form == kPhiErf)
             launch_m<false, kGenRichtmyer, kPhiErf>(A, grid, s);

@@ -115,8 +115,9 @@ int64_t dense_split_slot(int nd) { return (int64_t)((nd + kSB - 1) / kSB) * kSB 
 // matrix-core kernel, with each sample's normalised weight wt scaled to
 // [1/2, 1) by a power of two so that its sbar values (all proportional to
 // wt) suit the FP16 split; the scale is undone where values leave the sample
-// (recorded sbar, per-index sums).
-template <int GEN, int MODE>
+// (recorded sbar, per-index sums).  LOWER: every lower bound is -inf (the
+// walk forms only the upper bound's tail mass; integrate and record modes).
+template <int GEN, int MODE, bool LOWER>
 __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const _Float16* __restrict__ uc,
                                                             int64_t ucs, const float* __restrict__ Uk,
                                                             const float* __restrict__ Ucol,
@@ -331,7 +332,7 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
                         yv, pts, n, gb, cnt, k, j, tid, ybar, wt, s_x, s_y, (float*)nullptr, Ucol, A.ldc, ak, bk,
                         invk, srec, yrec, A.rs, s_red, A.a_inf != 0, unsc, s_nx, y_nx);
                 else
-                    walk_block_yv<true, GEN, kPhiTail, kTS, kSXS, kSB, false, REC>(
+                    walk_block_yv<true, GEN, LOWER ? kPhiTailLower : kPhiTail, kTS, kSXS, kSB, false, REC>(
                         yv, pts, gb, cnt, k, j, tid, st, w, s_x, s_y, (float*)nullptr, 0, Ucol, A.ldc, ak, bk, invk,
                         (float*)nullptr, SOBT ? s_sob : nullptr, sl);
                 store_y16(Ys16, gb, tid, yv, cnt, kOpScale);
@@ -446,10 +447,10 @@ hipError_t pack_split16(const float* Ucol, int n, void* uc, hipStream_t s)
     return hipGetLastError();
 }
 
-template <int GEN, int MODE>
+template <int GEN, int MODE, bool LOWER = false>
 static void launch_s(const DenseArgs& A, const void* uc, int64_t ucs, int grid, hipStream_t s)
 {
-    hipLaunchKernelGGL((dense_split_kernel<GEN, MODE>), dim3(grid), dim3(256), 0, s, A, (const _Float16*)uc, ucs,
+    hipLaunchKernelGGL((dense_split_kernel<GEN, MODE, LOWER>), dim3(grid), dim3(256), 0, s, A, (const _Float16*)uc, ucs,
                        (const float*)A.Uk, (const float*)A.Ucol, (const float*)A.ak, (const float*)A.bk,
                        (const float*)A.invk, (float*)A.Y, A.part_m, A.part_e, (float*)A.yrec, (float*)A.srec,
                        DBG_POINT_ARGS(A.pts));
@@ -458,14 +459,27 @@ static void launch_s(const DenseArgs& A, const void* uc, int64_t ucs, int grid, 
 hipError_t launch_dense_split(const DenseArgs& A, int gen, const void* uc, int grid, hipStream_t s)
 {
     const int64_t ucs = split16_col_stride(A.n);
+    const bool lower = A.a_inf != 0;
     if (gen == kGenRichtmyer) {
-        if (A.mode == kModeRecord) launch_s<kGenRichtmyer, kModeRecord>(A, uc, ucs, grid, s);
-        else if (A.mode == kModeReverse) launch_s<kGenRichtmyer, kModeReverse>(A, uc, ucs, grid, s);
-        else launch_s<kGenRichtmyer, kModeIntegrate>(A, uc, ucs, grid, s);
+        if (A.mode == kModeRecord) {
+            if (lower) launch_s<kGenRichtmyer, kModeRecord, true>(A, uc, ucs, grid, s);
+            else launch_s<kGenRichtmyer, kModeRecord>(A, uc, ucs, grid, s);
+        } else if (A.mode == kModeReverse) {
+            launch_s<kGenRichtmyer, kModeReverse>(A, uc, ucs, grid, s);
+        } else {
+            if (lower) launch_s<kGenRichtmyer, kModeIntegrate, true>(A, uc, ucs, grid, s);
+            else launch_s<kGenRichtmyer, kModeIntegrate>(A, uc, ucs, grid, s);
+        }
     } else {
-        if (A.mode == kModeRecord) launch_s<kGenSobol, kModeRecord>(A, uc, ucs, grid, s);
-        else if (A.mode == kModeReverse) launch_s<kGenSobol, kModeReverse>(A, uc, ucs, grid, s);
-        else launch_s<kGenSobol, kModeIntegrate>(A, uc, ucs, grid, s);
+        if (A.mode == kModeRecord) {
+            if (lower) launch_s<kGenSobol, kModeRecord, true>(A, uc, ucs, grid, s);
+            else launch_s<kGenSobol, kModeRecord>(A, uc, ucs, grid, s);
+        } else if (A.mode == kModeReverse) {
+            launch_s<kGenSobol, kModeReverse>(A, uc, ucs, grid, s);
+        } else {
+            if (lower) launch_s<kGenSobol, kModeIntegrate, true>(A, uc, ucs, grid, s);
+            else launch_s<kGenSobol, kModeIntegrate>(A, uc, ucs, grid, s);
+        }
     }
     return hipGetLastError();
 }

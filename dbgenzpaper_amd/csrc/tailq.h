@@ -111,14 +111,32 @@ DBG_HD inline float log2_f32(float x)
 #endif
 }
 
+// 1/sqrt(x) for the positive, normal arguments of tail_quantile_f32: the
+// device's approximate v_rsq_f32 (within 1 ulp), the host's correctly rounded
+// quotient (or one ulp off it, for the host check of that tolerance)
+DBG_HD inline float rsqrt_f32(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rsqf(x);
+#elif defined(DBG_RCP_ULP_TEST)
+    return nextafterf(1.0f / sqrtf(x), DBG_RCP_ULP_TEST > 0 ? INFINITY : 0.0f);
+#else
+    return 1.0f / sqrtf(x);
+#endif
+}
+
 // Phi^{-1}(p), p = t 2^-k <= 1/2, in terms of w = -log(4p(1 - p)) (the
 // variable of Giles, "Approximating the erfinv function", GPU Computing Gems
-// 2010; coefficients fitted here): y / (2p - 1) is a polynomial in w for
-// w < 6.25 (p > 4.8e-4, the common case, so a wavefront of samples seldom
-// takes two branches); -y a polynomial in s = sqrt(w) for s < 6 (p > 5.8e-17);
-// -y / s a polynomial in 1/s beyond (fitted to s = 50, p near 2^-3600).
-// w is formed as (k - log2(4t(1 - p))) log 2, so no step leaves the FP32
-// range.  Within 1.5 FP32 eps of max(|y|, 1) on the host.
+// 2010; coefficients fitted here) in two regions: y / (2p - 1) is a
+// polynomial in w for w < 6.25 (p > 4.8e-4, the common case at moderate
+// dimension), and beyond it -y / s is a polynomial in x = 1/s, s = sqrt(w),
+// fitted for s in [2.5, 50] (p from 4.8e-4 down to about 2^-3600), with x from
+// one reciprocal square root and s = w x refined by one Newton step (the
+// factor s carries its rounding into y; x enters only through the flat
+// polynomial).  w is formed as
+// (k - log2(4t(1 - p))) log 2, so no step leaves the FP32 range.  Within 1.7
+// FP32 eps of max(|y|, 1) on the host, with the reciprocal square root exact
+// or one ulp off.
 DBG_HD inline float tail_quantile_f32(float t, int k)
 {
     const float p = ldexpf(t, -k);  // 0 when far below the FP32 range
@@ -130,17 +148,14 @@ DBG_HD inline float tail_quantile_f32(float t, int k)
                               -8.532967418e-03f, 3.396349549e-01f, 2.338620663e+00f};
         return fmaf(2.0f, p, -1.0f) * horner_f32(w - 3.125f, C1);
     }
-    const float s = sqrtf(w);
-    if (s < 6.0f) {  // p > 5.8e-17
-        const float C2[10] = {4.801153182e-07f, -1.164755830e-07f, -6.827907328e-06f, 2.369199137e-05f,
-                              -7.816802827e-05f, 2.896706865e-04f, -8.585309261e-04f, 1.008118619e-03f,
-                              1.428325772e+00f, 5.787277699e+00f};
-        return -horner_f32(s - 4.25f, C2);
-    }
-    const float C3[10] = {1.666337969e+05f, -1.617340527e+04f, -1.590158749e+01f, -9.795043945e+01f,
-                          3.148544884e+01f, -6.921745300e+00f, 2.482273579e+00f, -5.426022410e-01f,
-                          -2.322116643e-01f, 1.400314808e+00f};
-    return -s * horner_f32(rcp_f32(s) - 0.0933333333f, C3);
+    const float x = rsqrt_f32(w);
+    const float s0 = w * x;
+    const float s = fmaf(0.5f * x, fmaf(-s0, s0, w), s0);  // one Newton step: s to about half an ulp
+    const float CB[13] = {-1.301323203e+05f, 1.896112891e+04f, 1.034813477e+04f, -9.892526855e+02f,
+                          -4.395959778e+02f, 4.312902832e+01f, 4.628243446e-01f, 2.000564814e+00f,
+                          -1.639978766e+00f, 1.008415103e+00f, 6.785751786e-03f, -2.852775156e-01f,
+                          1.368908286e+00f};
+    return -s * horner_f32(x - 0.21f, CB);
 }
 
 }  // namespace dbg
