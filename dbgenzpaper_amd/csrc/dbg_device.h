@@ -780,13 +780,19 @@ struct RevOut {
     double ga, gb, gd;
 };
 
-template <bool MIXED, int GEN, typename T = typename std::conditional<MIXED, float, double>::type>
+// LOWER (mixed only): every lower bound is -inf, so alpha_r = -inf, phi(alpha_r)
+// = 0 and the alpha terms vanish; the lower bound's mass and density are not
+// formed (the same values as the general step with a_r = -inf).
+template <bool MIXED, int GEN, bool LOWER = false, typename T = typename std::conditional<MIXED, float, double>::type>
 __device__ __forceinline__ T rev_step(const PointArgs& pts, const PointOp& op, int n, int r, int64_t j, T ybar,
                                       T wt, T s_r, T y_r, T a_r, T b_r, T inv_r, RevOut& o)
 {
-    const T za = (a_r - s_r) * inv_r, zb = (b_r - s_r) * inv_r;
+    static_assert(MIXED || !LOWER, "the lower-orthant step is a form of the mixed path");
+    const T zb = (b_r - s_r) * inv_r;
+    const T za = LOWER ? T(0) : (a_r - s_r) * inv_r;
     typename Path<MIXED, kPhiTail>::State st;
-    tail_from_z(st, za, zb);
+    if constexpr (LOWER) tail_from_zb(st, zb);
+    else tail_from_z(st, za, zb);
     const T sq2 = T(1.4142135623730951);
     const T al = za * sq2, be = zb * sq2;
     T dcb = (wt > T(0) && st.dc > T(0)) ? wt / st.dc : T(0);
@@ -805,23 +811,23 @@ __device__ __forceinline__ T rev_step(const PointArgs& pts, const PointOp& op, i
         cb = iy;
         dcb += iy * w;
     }
-    const T albar = (cb - dcb) * scaled_pdf(al, st.k);
+    const T albar = LOWER ? T(0) : (cb - dcb) * scaled_pdf(al, st.k);
     const T bebar = dcb * scaled_pdf(be, st.k);
     const T invU = inv_r * sq2;
     o.ga = (double)(albar * invU);
     o.gb = (double)(bebar * invU);
-    const T ta = isinf(al) ? T(0) : albar * al;
+    const T ta = (LOWER || isinf(al)) ? T(0) : albar * al;
     const T tb = isinf(be) ? T(0) : bebar * be;
     o.gd = (double)(-(ta + tb) * invU);
     return -(albar + bebar) * invU;
 }
 
-template <bool MIXED, int GEN, typename T = typename std::conditional<MIXED, float, double>::type>
+template <bool MIXED, int GEN, bool LOWER = false, typename T = typename std::conditional<MIXED, float, double>::type>
 __device__ __forceinline__ T rev_step(const PointArgs& pts, int n, int r, int k, int64_t j, T ybar, T wt,
                                       T s_r, T y_r, T a_r, T b_r, T inv_r, RevOut& o)
 {
     const PointOp op = (r <= n - 2) ? point_op<MIXED, GEN>(pts, r, k) : PointOp{};
-    return rev_step<MIXED, GEN>(pts, op, n, r, j, ybar, wt, s_r, y_r, a_r, b_r, inv_r, o);
+    return rev_step<MIXED, GEN, LOWER>(pts, op, n, r, j, ybar, wt, s_r, y_r, a_r, b_r, inv_r, o);
 }
 
 // Scalar operands of one reverse step (flipped index g = gb + CC, index
@@ -854,8 +860,10 @@ struct RevOps {
 // sbar and ybar of the sample) where values leave the sample: the recorded
 // sbar and the per-index sums (1 without such a scale).  YST = false leaves
 // the sub-block's sbar in sv for the caller instead of storing them to Ys;
-// SREC = false leaves the sbar records to the caller as well.
-template <bool MIXED, int GEN, int TS, int SXS, int SB, int CC, bool YST = true, bool SREC = true, typename T>
+// SREC = false leaves the sbar records to the caller as well.  LOWER: the
+// lower-orthant step (rev_step).
+template <bool MIXED, int GEN, int TS, int SXS, int SB, int CC, bool YST = true, bool SREC = true,
+          bool LOWER = false, typename T>
 __device__ __forceinline__ void rev_steps(const PointArgs& pts, int n, int gb, int cnt, int k, int64_t j, int t,
                                           T& ybar, T wt, const T* __restrict__ s_x, T* __restrict__ s_y,
                                           T* __restrict__ Ys, const T* __restrict__ Uf, int64_t ldc,
@@ -880,7 +888,8 @@ __device__ __forceinline__ void rev_steps(const PointArgs& pts, int n, int gb, i
                 }
             }
             RevOut o;
-            const T sbar = rev_step<MIXED, GEN>(pts, cur.pt, n, r, j, ybar, wt, s_r, y_r, cur.a, cur.b, cur.inv, o);
+            const T sbar =
+                rev_step<MIXED, GEN, LOWER>(pts, cur.pt, n, r, j, ybar, wt, s_r, y_r, cur.a, cur.b, cur.inv, o);
             sv[CC] = sbar;
             s_y[CC * TS + t] = sbar;
             if constexpr (YST) Ys[(size_t)g * TS + t] = sbar;
@@ -889,7 +898,7 @@ __device__ __forceinline__ void rev_steps(const PointArgs& pts, int n, int gb, i
             // terms on the mixed path, FP64 on the FP64 path
             double a0 = 0.0, b0, d0;
             if constexpr (MIXED) {
-                if (!a_inf) a0 = wave_sum_f32((float)o.ga * unsc);  // uniform; 0 when a' = -inf throughout
+                if (!LOWER && !a_inf) a0 = wave_sum_f32((float)o.ga * unsc);  // uniform; 0 when a' = -inf throughout
                 b0 = wave_sum_f32((float)o.gb * unsc);
                 d0 = wave_sum_f32((float)o.gd * unsc);
             } else {
@@ -907,9 +916,9 @@ __device__ __forceinline__ void rev_steps(const PointArgs& pts, int n, int gb, i
 #pragma unroll
             for (int l = 0; l <= CC; ++l) acc = fma(sv[l], cur.u[l], acc);
             ybar = acc;
-            rev_steps<MIXED, GEN, TS, SXS, SB, CC + 1, YST, SREC>(pts, n, gb, cnt, k, j, t, ybar, wt, s_x, s_y, Ys,
-                                                                  Uf, ldc, ak, bk, invk, srec, yrec, rs, red, sv,
-                                                                  s_nx, y_nx, ops, a_inf, unsc);
+            rev_steps<MIXED, GEN, TS, SXS, SB, CC + 1, YST, SREC, LOWER>(pts, n, gb, cnt, k, j, t, ybar, wt, s_x,
+                                                                         s_y, Ys, Uf, ldc, ak, bk, invk, srec, yrec,
+                                                                         rs, red, sv, s_nx, y_nx, ops, a_inf, unsc);
         }
     }
 }
@@ -927,7 +936,7 @@ __device__ __forceinline__ void rev_steps(const PointArgs& pts, int n, int gb, i
 // step cc overwrites record row r - 1 only after reading it, and the next
 // step reads row r - 2.  The scalar operands are fetched one step ahead as
 // well.
-template <bool MIXED, int GEN, int TS, int SXS, int SB, bool YST, bool SREC = true, typename T>
+template <bool MIXED, int GEN, int TS, int SXS, int SB, bool YST, bool SREC = true, bool LOWER = false, typename T>
 __device__ __forceinline__ void walk_block_rev_sv(T (&sv)[SB], const PointArgs& pts, int n, int gb, int cnt, int k,
                                                   int64_t j, int t, T& ybar, T wt, const T* __restrict__ s_x,
                                                   T* __restrict__ s_y, T* __restrict__ Ys,
@@ -939,8 +948,9 @@ __device__ __forceinline__ void walk_block_rev_sv(T (&sv)[SB], const PointArgs& 
 {
     RevOps<MIXED, GEN, SB, T> ops[2];
     if (cnt > 0) ops[0].template fetch<0>(pts, n, gb, k, Uf, ldc, ak, bk, invk);
-    rev_steps<MIXED, GEN, TS, SXS, SB, 0, YST, SREC>(pts, n, gb, cnt, k, j, t, ybar, wt, s_x, s_y, Ys, Uf, ldc, ak,
-                                                     bk, invk, srec, yrec, rs, red, sv, s_nx, y_nx, ops, a_inf, unsc);
+    rev_steps<MIXED, GEN, TS, SXS, SB, 0, YST, SREC, LOWER>(pts, n, gb, cnt, k, j, t, ybar, wt, s_x, s_y, Ys, Uf, ldc,
+                                                            ak, bk, invk, srec, yrec, rs, red, sv, s_nx, y_nx, ops,
+                                                            a_inf, unsc);
 }
 
 // The records of the first step of the reverse block at flipped index gb

@@ -116,7 +116,7 @@ int64_t dense_split_slot(int nd) { return (int64_t)((nd + kSB - 1) / kSB) * kSB 
 // [1/2, 1) by a power of two so that its sbar values (all proportional to
 // wt) suit the FP16 split; the scale is undone where values leave the sample
 // (recorded sbar, per-index sums).  LOWER: every lower bound is -inf (the
-// walk forms only the upper bound's tail mass; integrate and record modes).
+// walk and the reverse step form only the upper bound's tail mass).
 template <int GEN, int MODE, bool LOWER>
 __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const _Float16* __restrict__ uc,
                                                             int64_t ucs, const float* __restrict__ Uk,
@@ -328,7 +328,7 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
                 // from s_y) are stored after the wait below, which would
                 // otherwise wait for them as well: they drain during the next walk
                 if constexpr (REV)
-                    walk_block_rev_sv<true, GEN, kTS, kSXS, kSB, false, false>(
+                    walk_block_rev_sv<true, GEN, kTS, kSXS, kSB, false, false, LOWER>(
                         yv, pts, n, gb, cnt, k, j, tid, ybar, wt, s_x, s_y, (float*)nullptr, Ucol, A.ldc, ak, bk,
                         invk, srec, yrec, A.rs, s_red, A.a_inf != 0, unsc, s_nx, y_nx);
                 else
@@ -394,8 +394,8 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
         if constexpr (REV) {
             // index 0: s_0 = 0, no sbar to pass on
             RevOut o{0.0, 0.0, 0.0};
-            (void)rev_step<true, GEN>(pts, n, 0, k, j, ybar, wt, 0.0f, n >= 2 ? yrec[tid] : 0.0f, ak[0], bk[0],
-                                      invk[0], o);
+            (void)rev_step<true, GEN, LOWER>(pts, n, 0, k, j, ybar, wt, 0.0f, n >= 2 ? yrec[tid] : 0.0f, ak[0],
+                                             bk[0], invk[0], o);
             const double a0 = wave_sum_f64(o.ga * unsc), b0 = wave_sum_f64(o.gb * unsc),
                          d0 = wave_sum_f64(o.gd * unsc);
             if (lane == 0) {
@@ -465,7 +465,8 @@ hipError_t launch_dense_split(const DenseArgs& A, int gen, const void* uc, int g
             if (lower) launch_s<kGenRichtmyer, kModeRecord, true>(A, uc, ucs, grid, s);
             else launch_s<kGenRichtmyer, kModeRecord>(A, uc, ucs, grid, s);
         } else if (A.mode == kModeReverse) {
-            launch_s<kGenRichtmyer, kModeReverse>(A, uc, ucs, grid, s);
+            if (lower) launch_s<kGenRichtmyer, kModeReverse, true>(A, uc, ucs, grid, s);
+            else launch_s<kGenRichtmyer, kModeReverse>(A, uc, ucs, grid, s);
         } else {
             if (lower) launch_s<kGenRichtmyer, kModeIntegrate, true>(A, uc, ucs, grid, s);
             else launch_s<kGenRichtmyer, kModeIntegrate>(A, uc, ucs, grid, s);
@@ -475,7 +476,8 @@ hipError_t launch_dense_split(const DenseArgs& A, int gen, const void* uc, int g
             if (lower) launch_s<kGenSobol, kModeRecord, true>(A, uc, ucs, grid, s);
             else launch_s<kGenSobol, kModeRecord>(A, uc, ucs, grid, s);
         } else if (A.mode == kModeReverse) {
-            launch_s<kGenSobol, kModeReverse>(A, uc, ucs, grid, s);
+            if (lower) launch_s<kGenSobol, kModeReverse, true>(A, uc, ucs, grid, s);
+            else launch_s<kGenSobol, kModeReverse>(A, uc, ucs, grid, s);
         } else {
             if (lower) launch_s<kGenSobol, kModeIntegrate, true>(A, uc, ucs, grid, s);
             else launch_s<kGenSobol, kModeIntegrate>(A, uc, ucs, grid, s);
