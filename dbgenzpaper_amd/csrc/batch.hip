@@ -7,8 +7,9 @@
 // the queue is empty, so long and short problems interleave without a tail.
 // Each item runs the vector-ALU dense integration (one sample per lane,
 // dimension blocks of W with the register-blocked contraction, then
-// walk_block) on its problem's factor.  W is a template parameter picked per
-// batch by the autotuner of the C ABI layer.
+// walk_block_reg, which keeps the hand-off in registers) on its problem's
+// factor.  W is a template parameter picked per batch by the autotuner of the
+// C ABI layer; the grid is the kernel's resident workgroups (batch_occupancy).
 #include <type_traits>
 
 #include "dbg_device.h"
@@ -33,17 +34,16 @@ hipError_t queue_reset(const BatchArgs& B, hipStream_t s)
 }
 
 // One work item's walk over its problem: the register-blocked contraction of
-// each block of W dimensions, then walk_block; FORM kPhiTailLower for a
-// problem whose lower bounds are all -inf (mixed only)
+// each block of W dimensions into the lane's own registers, then the walk of
+// the block from those registers (no LDS: occupancy is set by registers);
+// FORM kPhiTailLower for a problem whose lower bounds are all -inf (mixed only)
 template <bool MIXED, int W, int FORM, typename T>
 __device__ __forceinline__ Weight<T> batch_item(const PointArgs& pts, const BatchProblem& P, const double* f,
-                                                int r, int64_t j, int tid, T* __restrict__ s_x,
-                                                T* __restrict__ s_y, T* __restrict__ Ys,
+                                                int r, int64_t j, int tid, T* __restrict__ Ys,
                                                 const T* __restrict__ Uk, const T* __restrict__ Ucol,
                                                 const T* __restrict__ ak, const T* __restrict__ bk,
                                                 const T* __restrict__ invk)
 {
-    constexpr int SXS = W + 4;
     const int d = P.d, nd = d - 1;
     typename Path<MIXED, FORM>::State st;
     Weight<T> w;
@@ -58,10 +58,8 @@ __device__ __forceinline__ Weight<T> batch_item(const PointArgs& pts, const Batc
 #pragma unroll
             for (int c = 0; c < W; ++c) s[c] = fma(y, ur[c], s[c]);
         }
-#pragma unroll
-        for (int c = 0; c < W; ++c) s_x[tid * SXS + c] = s[c];
-        walk_block<MIXED, kGenRichtmyer, FORM, kTileB, SXS, W>(pts, g0, min(W, nd - g0), r, j, tid, st, w, s_x,
-                                                               s_y, Ys, kTileB, Ucol, d, ak, bk, invk);
+        walk_block_reg<MIXED, kGenRichtmyer, FORM, W>(pts, g0, min(W, nd - g0), r, j, tid, st, w, s, Ys, kTileB,
+                                                      Ucol, d, ak, bk, invk);
     }
     return w;
 }
@@ -73,9 +71,6 @@ __global__ __launch_bounds__(kTileB) void batch_integrate_kernel(BatchArgs B, co
                                                                  double* __restrict__ part_m,
                                                                  int* __restrict__ part_e)
 {
-    constexpr int SXS = W + 4;
-    __shared__ T s_x[kTileB * SXS];
-    __shared__ T s_y[W * kTileB];
     __shared__ int64_t s_item;
     const int tid = threadIdx.x;
     T* __restrict__ Ys = Yall + (size_t)blockIdx.x * (size_t)B.y_slot;
@@ -125,10 +120,10 @@ __global__ __launch_bounds__(kTileB) void batch_integrate_kernel(BatchArgs B, co
 
         Weight<T> w;
         if (MIXED && P.lower)  // uniform: one problem per item
-            w = batch_item<MIXED, W, MIXED ? kPhiTailLower : kPhiErf>(pts, P, f, r, j, tid, s_x, s_y, Ys, Uk, Ucol,
-                                                                      ak, bk, invk);
+            w = batch_item<MIXED, W, MIXED ? kPhiTailLower : kPhiErf>(pts, P, f, r, j, tid, Ys, Uk, Ucol, ak, bk,
+                                                                      invk);
         else
-            w = batch_item<MIXED, W, kPhiErf>(pts, P, f, r, j, tid, s_x, s_y, Ys, Uk, Ucol, ak, bk, invk);
+            w = batch_item<MIXED, W, kPhiErf>(pts, P, f, r, j, tid, Ys, Uk, Ucol, ak, bk, invk);
         const double pm = valid ? (double)w.pm : 0.0;
         double S;
         int E;
@@ -150,6 +145,22 @@ static void launch_bi(const BatchArgs& B, int grid, const void* Uk, const void* 
     using T = typename std::conditional<MIXED, float, double>::type;
     hipLaunchKernelGGL((batch_integrate_kernel<MIXED, W>), dim3(grid), dim3(kTileB), 0, s, B,
                        (const T*)Uk, (const T*)Ucol, (T*)Y, pm, pe);
+}
+
+template <bool MIXED, int W>
+static int occ_bi()
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_integrate_kernel<MIXED, W>, kTileB, 0) != hipSuccess)
+        return 0;
+    return n;
+}
+
+// resident workgroups per CU of the integration kernel at width W (0 on error)
+int batch_occupancy(bool mixed, int W)
+{
+    if (mixed) return W == 8 ? occ_bi<true, 8>() : (W == 32 ? occ_bi<true, 32>() : occ_bi<true, 16>());
+    return W == 8 ? occ_bi<false, 8>() : (W == 32 ? occ_bi<false, 32>() : occ_bi<false, 16>());
 }
 
 hipError_t batch_integrate(const BatchArgs& B, bool mixed, int W, int grid, const void* Uk,

@@ -1524,7 +1524,8 @@ struct dbg_batch {
     int* part_e = nullptr;
     int2* sample_items = nullptr;
     int64_t n_sample = 0;
-    int grid = 0;
+    int grid = 0;              // workspace slots: the largest resident grid over the widths
+    int grid_w[3] = {0, 0, 0};  // resident grid of the integration kernel at width 8, 16, 32
     int W = 16;
     float tune_ms[3] = {0, 0, 0};
 
@@ -1706,7 +1707,15 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
     int dev = 0, ncu = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    Bt->grid = ncu * 2;
+    // the persistent integration launch fills the device: as many workgroups as
+    // are resident at once for each width (registers, no LDS beyond the queue word)
+    const int widths[3] = {8, 16, 32};
+    Bt->grid = 0;
+    for (int w = 0; w < 3; ++w) {
+        const int occ = std::max(1, std::min(8, batch_occupancy(Bt->mixed, widths[w])));
+        Bt->grid_w[w] = ncu * occ;
+        Bt->grid = std::max(Bt->grid, Bt->grid_w[w]);
+    }
     B.y_slot = (int64_t)std::max(dmax - 1, 1) * tile;
     BALLOC(Bt->Y, (size_t)Bt->grid * B.y_slot * T);
     BALLOC(Bt->part_m, std::max<int64_t>(part, 1) * sizeof(double));
@@ -1744,8 +1753,8 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
 
     // ---- autotune the dimension-block width on a sample of the work items
     HIPCHK(batch_factor(B, Bt->mixed, Bt->Uk, Bt->Ucol, s));
-    const int widths[3] = {8, 16, 32};
     float best = 1e30f;
+    int wsel = 1;
     BatchArgs Bs = B;
     Bs.items = Bt->sample_items;
     Bs.n_items = Bt->n_sample;
@@ -1754,7 +1763,7 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
         for (int rep = 0; rep < 2; ++rep) {  // first pass warms up
             HIPCHK(queue_reset(B, s));
             HIPCHK(hipEventRecord(Bt->ev0, s));
-            HIPCHK(batch_integrate(Bs, Bt->mixed, widths[w], Bt->grid, Bt->Uk, Bt->Ucol, Bt->Y, Bt->part_m,
+            HIPCHK(batch_integrate(Bs, Bt->mixed, widths[w], Bt->grid_w[w], Bt->Uk, Bt->Ucol, Bt->Y, Bt->part_m,
                                    Bt->part_e, s));
             HIPCHK(hipEventRecord(Bt->ev1, s));
             HIPCHK(hipEventSynchronize(Bt->ev1));
@@ -1768,6 +1777,7 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
         if (msum < best) {
             best = msum;
             Bt->W = widths[w];
+            wsel = w;
         }
     }
     // ---- the whole job as one graph: factor, queue reset, integration, reductions
@@ -1775,7 +1785,8 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
     hipError_t e = batch_factor(B, Bt->mixed, Bt->Uk, Bt->Ucol, s);
     if (e == hipSuccess) e = queue_reset(B, s);
     if (e == hipSuccess)
-        e = batch_integrate(B, Bt->mixed, Bt->W, Bt->grid, Bt->Uk, Bt->Ucol, Bt->Y, Bt->part_m, Bt->part_e, s);
+        e = batch_integrate(B, Bt->mixed, Bt->W, Bt->grid_w[wsel], Bt->Uk, Bt->Ucol, Bt->Y, Bt->part_m, Bt->part_e,
+                            s);
     if (e == hipSuccess) e = batch_reduce(B, Bt->part_m, Bt->part_e, s);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(s, &g);

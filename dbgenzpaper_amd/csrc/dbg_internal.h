@@ -219,6 +219,7 @@ struct BatchArgs {
 
 constexpr int kBatchMaxDim = 512;
 hipError_t batch_factor(const BatchArgs& B, bool mixed, void* Uk, void* Ucol, hipStream_t s);
+int batch_occupancy(bool mixed, int W);
 hipError_t batch_integrate(const BatchArgs& B, bool mixed, int W, int grid, const void* Uk,
                            const void* Ucol, void* Y, double* part_m, int* part_e, hipStream_t s);
 hipError_t batch_reduce(const BatchArgs& B, const double* part_m, const int* part_e, hipStream_t s);
