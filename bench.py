@@ -41,7 +41,7 @@ PEAK_HBM_GBPS = 8000.0
 # (contraction, d, N, shifts): rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00
 # (factors from a plain 1 GiB device copy), profiles/dense_d2500_mixed_counters.md.
 # Other shapes report null.
-TRAFFIC_BYTES = {("split16", 2500, 2 ** 18, 10): 2.864e11, ("fp32", 2500, 2 ** 18, 10): 2.839e11}
+TRAFFIC_BYTES = {("split16", 2500, 2 ** 18, 10): 2.860e11, ("fp32", 2500, 2 ** 18, 10): 2.839e11}
 
 
 def _log(msg):

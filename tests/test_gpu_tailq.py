@@ -1,8 +1,8 @@
 """The mixed walk's FP32 tail functions (tailq.h) evaluated on the GPU, where
-they use the device's approximate v_rcp_f32, v_exp_f32 and v_log_f32,
+they use the device's approximate v_rcp_f32, v_rsq_f32, v_exp_f32 and v_log_f32,
 against 40-digit values: the tail mass erfc(z)/2 = m 2^-k over z in [0, 30]
 and the quantile Phi^{-1}(t 2^-k) from p = 1/2 down to 2^-3000, with both
-sides of the quantile's region switches.  Same bounds as the host-build
+sides of the quantile's region switch.  Same bounds as the host-build
 tests: 5e-7 relative for the mass, 4 FP32 ulps of max(|y|, 1) for the
 quantile."""
 import ctypes

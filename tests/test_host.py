@@ -119,12 +119,12 @@ def _host_build(tmp_path, src, name):
 
 @pytest.mark.parametrize("rcp_ulp", [0, 1, -1])
 def test_fp32_tail_quantile_against_mpmath(tmp_path, rcp_ulp):
-    """The mixed path's FP32 inverse CDF (three fitted polynomials in
+    """The mixed path's FP32 inverse CDF (two fitted polynomials in
     w = -log(4p(1-p)), w formed from t and k) against a 40-digit quantile,
     from p = 0.5 down to p = 2^-3000 given as scaled masses t 2^-k.  Bound:
-    4 FP32 ulps of max(|y|, 1).  The deepest region takes 1/sqrt(w) from a
-    reciprocal; rcp_ulp = +-1 builds it one ulp off, as the device's
-    approximate reciprocal may be."""
+    4 FP32 ulps of max(|y|, 1).  The tail region takes 1/sqrt(w) from a
+    reciprocal square root; rcp_ulp = +-1 builds it one ulp off, as the
+    device's approximate v_rsq_f32 may be."""
     mpmath = pytest.importorskip("mpmath")
     pre = f"#define DBG_RCP_ULP_TEST {rcp_ulp}\n" if rcp_ulp else ""
     lib = _host_build(tmp_path, pre + '#include "tailq.h"\n'
@@ -144,7 +144,7 @@ def test_fp32_tail_quantile_against_mpmath(tmp_path, rcp_ulp):
         return y
 
     worst = 0.0
-    # plus both sides of the region switches, w = 6.25 (p = 4.83e-4) and sqrt(w) = 6 (p = 5.80e-17)
+    # plus both sides of the region switch, w = 6.25 (p = 4.83e-4), and p near 5.80e-17 (sqrt(w) = 6)
     edges = np.log2(np.concatenate([4.8303e-4 * (1 + np.linspace(-1e-3, 1e-3, 21)),
                                     5.7988e-17 * (1 + np.linspace(-1e-3, 1e-3, 21))]))
     log2p = np.concatenate([np.linspace(-1.0, -40.0, 157), np.linspace(-40.0, -3000.0, 97), edges])
