@@ -5,11 +5,16 @@
 // Work items are (problem, shift, tile of 256 points), ordered largest d
 // first; resident workgroups take the next item from a global counter until
 // the queue is empty, so long and short problems interleave without a tail.
-// Each item runs the vector-ALU dense integration (one sample per lane,
-// dimension blocks of W with the register-blocked contraction, then
-// walk_block_reg, which keeps the hand-off in registers) on its problem's
-// factor.  W is a template parameter picked per batch by the autotuner of the
-// C ABI layer; the grid is the kernel's resident workgroups (batch_occupancy).
+// Each item runs the dense integration (one sample per lane) over dimension
+// blocks of W on its problem's factor: the block's conditional means from the
+// earlier blocks, then walk_block_reg, which keeps them in registers.  For
+// W = 8, 16 (and FP64) the contraction is register-blocked vector ALU work; for
+// the mixed path at W = 32 it runs on the FP16 matrix cores in the split form
+// of the dense split kernel (high and low FP16 parts of 2^6 y and 2^13 u,
+// three products, FP32 accumulation), with the tiles transposed back to one
+// sample per lane by lane-row swaps.  W is picked per batch by the autotuner
+// of the C ABI layer; the grid is the kernel's resident workgroups
+// (batch_occupancy).
 #include <type_traits>
 
 #include "dbg_device.h"
@@ -31,6 +36,151 @@ hipError_t queue_reset(const BatchArgs& B, hipStream_t s)
 {
     hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(1), 0, s, B.counter, B.status);
     return hipGetLastError();
+}
+
+namespace {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef unsigned int uint2v __attribute__((ext_vector_type(2)));
+constexpr int kYGB = 2 * kTileB * 8;          // halves per k-group of 8 dimensions in the split y slot
+constexpr float kYScaleB = 64.0f;             // 2^6
+constexpr float kUScaleB = 8192.0f;           // 2^13
+constexpr float kAccScaleB = 1.0f / 524288.0f;  // 2^-19
+constexpr float kHalfMaxB = 65504.0f;
+
+// v -> (hi, lo), hi = fp16(v), lo = fp16(v - hi); |v| clamped to the FP16 range
+__device__ __forceinline__ void split16b(float v, _Float16& hi, _Float16& lo)
+{
+    v = fabsf(v) > kHalfMaxB ? copysignf(kHalfMaxB, v) : v;
+    hi = (_Float16)v;
+    lo = (_Float16)(v - (float)hi);
+}
+
+// 4 x 4 transpose of (register, row of 16 lanes): afterwards register q at
+// lane lr + 16 lq holds what register lq held at lane lr + 16 q
+__device__ __forceinline__ void transpose_rows4(float& r0, float& r1, float& r2, float& r3)
+{
+    unsigned a0 = __float_as_uint(r0), a1 = __float_as_uint(r1), a2 = __float_as_uint(r2), a3 = __float_as_uint(r3);
+    uint2v x = __builtin_amdgcn_permlane32_swap(a0, a2, false, false);
+    a0 = x[0];
+    a2 = x[1];
+    x = __builtin_amdgcn_permlane32_swap(a1, a3, false, false);
+    a1 = x[0];
+    a3 = x[1];
+    x = __builtin_amdgcn_permlane16_swap(a0, a1, false, false);
+    a0 = x[0];
+    a1 = x[1];
+    x = __builtin_amdgcn_permlane16_swap(a2, a3, false, false);
+    a2 = x[0];
+    a3 = x[1];
+    r0 = __uint_as_float(a0);
+    r1 = __uint_as_float(a1);
+    r2 = __uint_as_float(a2);
+    r3 = __uint_as_float(a3);
+}
+
+}  // namespace
+
+// The mixed item on the FP16 matrix cores (W a multiple of 32).  Layouts:
+//   factor uc[j * ucs + 64 c + e] (j = target column < d + 32): e < 32 the high
+//          parts of 2^13 U[32 c + e, j], e >= 32 the low parts; zero for rows
+//          32 c + e >= j and for j >= d (batch_pack_split);
+//   y      the workgroup's slot, k-group g of 8 dimensions: 256 samples x 8
+//          high parts, then 256 x 8 low parts (the walk writes its sample's
+//          block of W dimensions as W / 8 groups).
+// The contraction of a block at g0 forms S^T (targets x samples) tiles: lane
+// (lr, lq) supplies the A operand of target g0 + 1 + 16 tb + lr, rows 32 c +
+// 8 lq .. + 7, and the B operand of sample 64 wv + 16 sb + lr, k-group 4 c +
+// lq; each output lane holds targets 16 tb + 4 lq + r of sample 16 sb + lr,
+// and transpose_rows4 over sb gives lane lr + 16 lq the targets of its own
+// sample.  Each wave reads only its own 64 samples' y.
+template <int W, int FORM>
+__device__ __forceinline__ Weight<float> batch_item_split(const PointArgs& pts, const BatchProblem& P,
+                                                          const double* f, int r, int64_t j, int tid,
+                                                          _Float16* __restrict__ Ys16,
+                                                          const _Float16* __restrict__ uc,
+                                                          const float* __restrict__ Ucol,
+                                                          const float* __restrict__ ak, const float* __restrict__ bk,
+                                                          const float* __restrict__ invk)
+{
+    static_assert(W % 32 == 0, "whole chunks of 32 dimensions");
+    using V = floatx4;
+    constexpr int NTB = W / 16;
+    const int lane = tid & 63, lr = lane & 15, lq = lane >> 4, wv = tid >> 6;
+    const int d = P.d, nd = d - 1;
+    const int64_t ucs = P.ucs;
+    typename Path<true, FORM>::State st;
+    Weight<float> w;
+    Path<true, FORM>::init(st, w, f);
+    for (int g0 = 0; g0 < nd; g0 += W) {
+        V acc[NTB][4];
+#pragma unroll
+        for (int tb = 0; tb < NTB; ++tb)
+#pragma unroll
+            for (int sb = 0; sb < 4; ++sb) acc[tb][sb] = V{0, 0, 0, 0};
+        const int nch = g0 / 32;
+        const _Float16* __restrict__ ua = uc + (size_t)(g0 + 1 + lr) * ucs + 8 * lq;
+        const _Float16* __restrict__ yb = Ys16 + (size_t)lq * kYGB + (size_t)(wv * 64 + lr) * 8;
+        for (int c = 0; c < nch; ++c) {
+            half8 ah[NTB], al[NTB], bh[4], bl[4];
+#pragma unroll
+            for (int tb = 0; tb < NTB; ++tb) {
+                const _Float16* __restrict__ up = ua + (size_t)(16 * tb) * ucs + 64 * c;
+                ah[tb] = *(const half8*)up;
+                al[tb] = *(const half8*)(up + 32);
+            }
+#pragma unroll
+            for (int sb = 0; sb < 4; ++sb) {
+                const _Float16* __restrict__ yp = yb + (size_t)(4 * c) * kYGB + 16 * sb * 8;
+                bh[sb] = *(const half8*)yp;
+                bl[sb] = *(const half8*)(yp + kTileB * 8);
+            }
+#pragma unroll
+            for (int tb = 0; tb < NTB; ++tb)
+#pragma unroll
+                for (int sb = 0; sb < 4; ++sb) {
+                    acc[tb][sb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[tb], bh[sb], acc[tb][sb], 0, 0, 0);
+                    acc[tb][sb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[tb], bl[sb], acc[tb][sb], 0, 0, 0);
+                    acc[tb][sb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[tb], bh[sb], acc[tb][sb], 0, 0, 0);
+                }
+        }
+        float s[W];
+#pragma unroll
+        for (int tb = 0; tb < NTB; ++tb)
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr) {
+                float v0 = acc[tb][0][rr] * kAccScaleB, v1 = acc[tb][1][rr] * kAccScaleB;
+                float v2 = acc[tb][2][rr] * kAccScaleB, v3 = acc[tb][3][rr] * kAccScaleB;
+                transpose_rows4(v0, v1, v2, v3);
+                s[16 * tb + rr] = v0;
+                s[16 * tb + 4 + rr] = v1;
+                s[16 * tb + 8 + rr] = v2;
+                s[16 * tb + 12 + rr] = v3;
+            }
+        const int cnt = min(W, nd - g0);
+        float yv[W];
+        walk_block_reg<true, kGenRichtmyer, FORM, W, false>(pts, g0, cnt, r, j, tid, st, w, s, yv,
+                                                            (float*)nullptr, 0, Ucol, d, ak, bk, invk);
+        // the block's y, split, as W / 8 k-groups of this sample
+#pragma unroll
+        for (int h = 0; h < W / 8; ++h) {
+            half8 hi, lo;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float v = (8 * h + e < cnt) ? yv[8 * h + e] * kYScaleB : 0.0f;
+                _Float16 a, b;
+                split16b(v, a, b);
+                hi[e] = a;
+                lo[e] = b;
+            }
+            _Float16* dst = Ys16 + (size_t)(g0 / 8 + h) * kYGB + (size_t)tid * 8;
+            *(half8*)dst = hi;
+            *(half8*)(dst + kTileB * 8) = lo;
+        }
+        // the next block's contraction reads these groups from other lanes of the wave
+        __syncthreads();
+    }
+    return w;
 }
 
 // One work item's walk over its problem: the register-blocked contraction of
@@ -58,15 +208,20 @@ __device__ __forceinline__ Weight<T> batch_item(const PointArgs& pts, const Batc
 #pragma unroll
             for (int c = 0; c < W; ++c) s[c] = fma(y, ur[c], s[c]);
         }
-        walk_block_reg<MIXED, kGenRichtmyer, FORM, W>(pts, g0, min(W, nd - g0), r, j, tid, st, w, s, Ys, kTileB,
-                                                      Ucol, d, ak, bk, invk);
+        T yv[W];
+        walk_block_reg<MIXED, kGenRichtmyer, FORM, W>(pts, g0, min(W, nd - g0), r, j, tid, st, w, s, yv, Ys,
+                                                      kTileB, Ucol, d, ak, bk, invk);
     }
     return w;
 }
 
+// the matrix-core form is held to four resident workgroups per CU (128
+// registers): the walk, latency-bound, needs the waves more than the
+// contraction needs registers for loads in flight
 template <bool MIXED, int W, typename T = typename std::conditional<MIXED, float, double>::type>
-__global__ __launch_bounds__(kTileB) void batch_integrate_kernel(BatchArgs B, const T* __restrict__ Ukall,
+__global__ __launch_bounds__(kTileB, (MIXED && W % 32 == 0) ? 4 : 1) void batch_integrate_kernel(BatchArgs B, const T* __restrict__ Ukall,
                                                                  const T* __restrict__ Ucall,
+                                                                 const _Float16* __restrict__ ucall,
                                                                  T* __restrict__ Yall,
                                                                  double* __restrict__ part_m,
                                                                  int* __restrict__ part_e)
@@ -119,11 +274,19 @@ __global__ __launch_bounds__(kTileB) void batch_integrate_kernel(BatchArgs B, co
         pts.shift32 = B.shift32 + P.shift_off;
 
         Weight<T> w;
-        if (MIXED && P.lower)  // uniform: one problem per item
+        if constexpr (MIXED && W % 32 == 0) {
+            _Float16* Ys16 = reinterpret_cast<_Float16*>(Ys);
+            const _Float16* uc = ucall + P.uc_off;
+            if (P.lower)  // uniform: one problem per item
+                w = batch_item_split<W, kPhiTailLower>(pts, P, f, r, j, tid, Ys16, uc, Ucol, ak, bk, invk);
+            else
+                w = batch_item_split<W, kPhiErf>(pts, P, f, r, j, tid, Ys16, uc, Ucol, ak, bk, invk);
+        } else if (MIXED && P.lower) {
             w = batch_item<MIXED, W, MIXED ? kPhiTailLower : kPhiErf>(pts, P, f, r, j, tid, Ys, Uk, Ucol, ak, bk,
                                                                       invk);
-        else
+        } else {
             w = batch_item<MIXED, W, kPhiErf>(pts, P, f, r, j, tid, Ys, Uk, Ucol, ak, bk, invk);
+        }
         const double pm = valid ? (double)w.pm : 0.0;
         double S;
         int E;
@@ -139,12 +302,12 @@ __global__ __launch_bounds__(kTileB) void batch_integrate_kernel(BatchArgs B, co
 }
 
 template <bool MIXED, int W>
-static void launch_bi(const BatchArgs& B, int grid, const void* Uk, const void* Ucol, void* Y,
+static void launch_bi(const BatchArgs& B, int grid, const void* Uk, const void* Ucol, const void* uc16, void* Y,
                       double* pm, int* pe, hipStream_t s)
 {
     using T = typename std::conditional<MIXED, float, double>::type;
     hipLaunchKernelGGL((batch_integrate_kernel<MIXED, W>), dim3(grid), dim3(kTileB), 0, s, B,
-                       (const T*)Uk, (const T*)Ucol, (T*)Y, pm, pe);
+                       (const T*)Uk, (const T*)Ucol, (const _Float16*)uc16, (T*)Y, pm, pe);
 }
 
 template <bool MIXED, int W>
@@ -164,16 +327,16 @@ int batch_occupancy(bool mixed, int W)
 }
 
 hipError_t batch_integrate(const BatchArgs& B, bool mixed, int W, int grid, const void* Uk,
-                           const void* Ucol, void* Y, double* part_m, int* part_e, hipStream_t s)
+                           const void* Ucol, const void* uc16, void* Y, double* part_m, int* part_e, hipStream_t s)
 {
     if (mixed) {
-        if (W == 8) launch_bi<true, 8>(B, grid, Uk, Ucol, Y, part_m, part_e, s);
-        else if (W == 32) launch_bi<true, 32>(B, grid, Uk, Ucol, Y, part_m, part_e, s);
-        else launch_bi<true, 16>(B, grid, Uk, Ucol, Y, part_m, part_e, s);
+        if (W == 8) launch_bi<true, 8>(B, grid, Uk, Ucol, uc16, Y, part_m, part_e, s);
+        else if (W == 32) launch_bi<true, 32>(B, grid, Uk, Ucol, uc16, Y, part_m, part_e, s);
+        else launch_bi<true, 16>(B, grid, Uk, Ucol, uc16, Y, part_m, part_e, s);
     } else {
-        if (W == 8) launch_bi<false, 8>(B, grid, Uk, Ucol, Y, part_m, part_e, s);
-        else if (W == 32) launch_bi<false, 32>(B, grid, Uk, Ucol, Y, part_m, part_e, s);
-        else launch_bi<false, 16>(B, grid, Uk, Ucol, Y, part_m, part_e, s);
+        if (W == 8) launch_bi<false, 8>(B, grid, Uk, Ucol, uc16, Y, part_m, part_e, s);
+        else if (W == 32) launch_bi<false, 32>(B, grid, Uk, Ucol, uc16, Y, part_m, part_e, s);
+        else launch_bi<false, 16>(B, grid, Uk, Ucol, uc16, Y, part_m, part_e, s);
     }
     return hipGetLastError();
 }
@@ -259,6 +422,37 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(BatchArgs B, const do
 hipError_t batch_reduce(const BatchArgs& B, const double* part_m, const int* part_e, hipStream_t s)
 {
     hipLaunchKernelGGL(batch_reduce_kernel, dim3(B.K), dim3(256), 0, s, B, part_m, part_e);
+    return hipGetLastError();
+}
+
+// The split-FP16 factor of every problem (mixed batch): one workgroup per
+// problem writes uc[j * ucs + 64 c + e] for all j < d + 32 and rows 32 c + e <
+// ucs / 2 from the FP32 column-major factor (ldc = d): the high and low FP16
+// parts of 2^13 U[32 c + e, j] for 32 c + e < j < d, else zero.
+__global__ __launch_bounds__(256) void batch_pack_split_kernel(BatchArgs B, const float* __restrict__ Ucall,
+                                                               _Float16* __restrict__ ucall)
+{
+    const BatchProblem P = B.probs[blockIdx.x];
+    const int d = P.d;
+    const int64_t ucs = P.ucs, rows = ucs / 2;
+    const float* __restrict__ Ucol = Ucall + P.ucol_off;
+    _Float16* __restrict__ uc = ucall + P.uc_off;
+    const int64_t total = (int64_t)(d + 32) * rows;
+    for (int64_t idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        const int jc = (int)(idx / rows);
+        const int l = (int)(idx - (int64_t)jc * rows);
+        const float v = (jc < d && l < jc) ? Ucol[(size_t)jc * d + l] * kUScaleB : 0.0f;
+        _Float16 hi, lo;
+        split16b(v, hi, lo);
+        const int c = l / 32, e = l % 32;
+        uc[(size_t)jc * ucs + 64 * c + e] = hi;
+        uc[(size_t)jc * ucs + 64 * c + 32 + e] = lo;
+    }
+}
+
+hipError_t batch_pack_split(const BatchArgs& B, const void* Ucol, void* uc16, hipStream_t s)
+{
+    hipLaunchKernelGGL(batch_pack_split_kernel, dim3(B.K), dim3(256), 0, s, B, (const float*)Ucol, (_Float16*)uc16);
     return hipGetLastError();
 }
 

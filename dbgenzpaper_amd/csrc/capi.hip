@@ -1519,6 +1519,7 @@ struct dbg_batch {
     std::vector<void*> owned;
     void* Uk = nullptr;
     void* Ucol = nullptr;
+    void* uc16 = nullptr;      // mixed: the split-FP16 factors of the matrix-core contraction
     void* Y = nullptr;
     double* part_m = nullptr;
     int* part_e = nullptr;
@@ -1563,7 +1564,7 @@ int batch_status(dbg_batch* Bt)
 // Extents of the batch's device buffers (elements), for the host-side check
 // of every offset the kernels derive from the problem table and the items.
 struct BatchExtents {
-    int64_t sig, vec, uk, sh, part, y_slot, nq;
+    int64_t sig, vec, uk, sh, part, y_slot, nq, uc;
     int K, R, dmax, tile;
     int64_t tpr;
 };
@@ -1573,8 +1574,11 @@ struct BatchExtents {
 // packed rows (d - 1 rows of ldu; the contraction reads W <= 32 entries from
 // column g0 + 1 with g0 <= d - 2, so ldu >= d + 31), the per-index vectors
 // (d at vec_off), the shifts ((d - 1) x R at shift_off), the tile partials
-// (tpr x R at part_off), the per-workgroup y slot ((d - 1) x tile) and the
-// lattice table (d - 1 entries); each item names a problem and a tile of it.
+// (tpr x R at part_off), the per-workgroup y slot ((d - 1) x tile, and for
+// the split y layout whole k-groups of the blocks of 32 dimensions), the
+// split-FP16 factor (d + 32 columns of ucs halves at uc_off, ucs covering every
+// row < d in chunks of 32) and the lattice table (d - 1 entries); each item
+// names a problem and a tile of it.
 int batch_validate(const std::vector<BatchProblem>& probs, const std::vector<int2>& items,
                    const std::vector<int2>& sample, const BatchExtents& X)
 {
@@ -1591,6 +1595,9 @@ int batch_validate(const std::vector<BatchProblem>& probs, const std::vector<int
         if (P.tiles_per_rep != X.tpr || P.part_off < 0 || P.part_off + X.tpr * X.R > X.part)
             return bad("partials extent");
         if (nd * X.tile > X.y_slot || nd > X.nq) return bad("workspace extent");
+        if ((nd + 31) / 32 * 32 * X.tile > X.y_slot) return bad("split workspace extent");
+        if (X.uc > 0 && (P.ucs != (d + 31) / 32 * 64 || P.uc_off < 0 || P.uc_off + (d + 32) * P.ucs > X.uc))
+            return bad("split factor extent");
     }
     for (const std::vector<int2>* v : {&items, &sample})
         for (const int2& it : *v)
@@ -1608,7 +1615,7 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
     hipStream_t s = Bt->stream;
     BatchArgs& B = Bt->B;
     std::vector<BatchProblem> probs(K);
-    int64_t sig = 0, vec = 0, uk = 0, sh = 0, part = 0;
+    int64_t sig = 0, vec = 0, uk = 0, sh = 0, part = 0, ucn = 0;
     int dmax = 1;
     const int tile = batch_tile();
     const int64_t tpr = (o.m + tile - 1) / tile;
@@ -1624,6 +1631,10 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
         P.ldu = d + 32;
         P.lower = 1;
         for (int i = 0; i < d; ++i) P.lower = P.lower && std::isinf(a[vec + i]) && a[vec + i] < 0;
+        P.ucs = (d + 31) / 32 * 64;
+        P.uc_off = ucn;
+        P.pad = 0;
+        ucn += (int64_t)(d + 32) * P.ucs;
         P.sig_off = sig;
         P.ucol_off = sig;
         P.vec_off = vec;
@@ -1654,7 +1665,9 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
         X.uk = uk;
         X.sh = sh;
         X.part = part;
-        X.y_slot = (int64_t)std::max(dmax - 1, 1) * tile;
+        // whole blocks of 32 dimensions: the split y layout of the matrix-core contraction
+        X.y_slot = (int64_t)((std::max(dmax - 1, 1) + 31) / 32 * 32) * tile;
+        X.uc = Bt->mixed ? ucn : 0;
         X.nq = std::max(dmax - 1, 1);
         X.K = K;
         X.R = R;
@@ -1691,6 +1704,7 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
     BALLOC(B.first, (size_t)8 * K * sizeof(double));
     BALLOC(Bt->Uk, std::max<int64_t>(uk, 1) * T);
     BALLOC(Bt->Ucol, sig * T);
+    if (Bt->mixed) BALLOC(Bt->uc16, std::max<int64_t>(ucn, 1) * sizeof(_Float16));
     int2* items_d;
     BALLOC(items_d, items.size() * sizeof(int2));
     BALLOC(Bt->sample_items, std::max<size_t>(sample.size(), 1) * sizeof(int2));
@@ -1716,7 +1730,7 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
         Bt->grid_w[w] = ncu * occ;
         Bt->grid = std::max(Bt->grid, Bt->grid_w[w]);
     }
-    B.y_slot = (int64_t)std::max(dmax - 1, 1) * tile;
+    B.y_slot = (int64_t)((std::max(dmax - 1, 1) + 31) / 32 * 32) * tile;
     BALLOC(Bt->Y, (size_t)Bt->grid * B.y_slot * T);
     BALLOC(Bt->part_m, std::max<int64_t>(part, 1) * sizeof(double));
     BALLOC(Bt->part_e, std::max<int64_t>(part, 1) * sizeof(int));
@@ -1753,6 +1767,7 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
 
     // ---- autotune the dimension-block width on a sample of the work items
     HIPCHK(batch_factor(B, Bt->mixed, Bt->Uk, Bt->Ucol, s));
+    if (Bt->mixed) HIPCHK(batch_pack_split(B, Bt->Ucol, Bt->uc16, s));
     float best = 1e30f;
     int wsel = 1;
     BatchArgs Bs = B;
@@ -1763,8 +1778,8 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
         for (int rep = 0; rep < 2; ++rep) {  // first pass warms up
             HIPCHK(queue_reset(B, s));
             HIPCHK(hipEventRecord(Bt->ev0, s));
-            HIPCHK(batch_integrate(Bs, Bt->mixed, widths[w], Bt->grid_w[w], Bt->Uk, Bt->Ucol, Bt->Y, Bt->part_m,
-                                   Bt->part_e, s));
+            HIPCHK(batch_integrate(Bs, Bt->mixed, widths[w], Bt->grid_w[w], Bt->Uk, Bt->Ucol, Bt->uc16, Bt->Y,
+                                   Bt->part_m, Bt->part_e, s));
             HIPCHK(hipEventRecord(Bt->ev1, s));
             HIPCHK(hipEventSynchronize(Bt->ev1));
             rc = batch_status(Bt);
@@ -1783,10 +1798,11 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
     // ---- the whole job as one graph: factor, queue reset, integration, reductions
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipError_t e = batch_factor(B, Bt->mixed, Bt->Uk, Bt->Ucol, s);
+    if (e == hipSuccess && Bt->mixed) e = batch_pack_split(B, Bt->Ucol, Bt->uc16, s);
     if (e == hipSuccess) e = queue_reset(B, s);
     if (e == hipSuccess)
-        e = batch_integrate(B, Bt->mixed, Bt->W, Bt->grid_w[wsel], Bt->Uk, Bt->Ucol, Bt->Y, Bt->part_m, Bt->part_e,
-                            s);
+        e = batch_integrate(B, Bt->mixed, Bt->W, Bt->grid_w[wsel], Bt->Uk, Bt->Ucol, Bt->uc16, Bt->Y, Bt->part_m,
+                            Bt->part_e, s);
     if (e == hipSuccess) e = batch_reduce(B, Bt->part_m, Bt->part_e, s);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(s, &g);

@@ -751,7 +751,7 @@ __device__ __forceinline__ void walk_block(const PointArgs& pts, int gb, int cnt
 // hand-off sx[c] (target gb + 1 + c) is a register array and y goes only to
 // the global workspace Ys (row g, stride ys), so the walk needs no LDS.  Same
 // arithmetic and operation order as walk_steps.
-template <bool MIXED, int GEN, int FORM, int SB, int CC, typename T, typename State>
+template <bool MIXED, int GEN, int FORM, int SB, int CC, bool YST, typename T, typename State>
 __device__ __forceinline__ void walk_steps_reg(const PointArgs& pts, int gb, int cnt, int k, int64_t j, int t,
                                                State& st, Weight<T>& w, T (&yv)[SB], const T (&sx)[SB],
                                                StepOps<MIXED, GEN, SB, T>& cur, T* __restrict__ Ys, int64_t ys,
@@ -768,31 +768,32 @@ __device__ __forceinline__ void walk_steps_reg(const PointArgs& pts, int gb, int
             const int g = gb + CC;
             const T y = path_draw<MIXED, GEN, FORM>(pts, cur.pt, st, g, j);
             yv[CC] = y;
-            Ys[(size_t)g * ys + t] = y;
+            if constexpr (YST) Ys[(size_t)g * ys + t] = y;
             T acc = sx[CC];
 #pragma unroll
             for (int l = 0; l <= CC; ++l) acc = fma(yv[l], cur.u[l], acc);
             PT::phi(st, acc, cur.a, cur.b, cur.inv);
             PT::weigh(w, st);
             if constexpr (CC + 1 < SB)
-                walk_steps_reg<MIXED, GEN, FORM, SB, CC + 1>(pts, gb, cnt, k, j, t, st, w, yv, sx, nxt, Ys, ys, Ucol,
-                                                             ldc, ak, bk, invk);
+                walk_steps_reg<MIXED, GEN, FORM, SB, CC + 1, YST>(pts, gb, cnt, k, j, t, st, w, yv, sx, nxt, Ys, ys,
+                                                                  Ucol, ldc, ak, bk, invk);
         }
     }
 }
 
-template <bool MIXED, int GEN, int FORM, int SB, typename T, typename State>
+// YST = false leaves the block's y in yv for the caller instead of storing
+// them to Ys.
+template <bool MIXED, int GEN, int FORM, int SB, bool YST = true, typename T, typename State>
 __device__ __forceinline__ void walk_block_reg(const PointArgs& pts, int gb, int cnt, int k, int64_t j, int t,
-                                               State& st, Weight<T>& w, const T (&sx)[SB], T* __restrict__ Ys,
-                                               int64_t ys, const T* __restrict__ Ucol, int64_t ldc,
-                                               const T* __restrict__ ak, const T* __restrict__ bk,
+                                               State& st, Weight<T>& w, const T (&sx)[SB], T (&yv)[SB],
+                                               T* __restrict__ Ys, int64_t ys, const T* __restrict__ Ucol,
+                                               int64_t ldc, const T* __restrict__ ak, const T* __restrict__ bk,
                                                const T* __restrict__ invk)
 {
-    T yv[SB];
     StepOps<MIXED, GEN, SB, T> first;
     if (cnt > 0) first.template fetch<0>(pts, gb, k, Ucol, ldc, ak, bk, invk);
-    walk_steps_reg<MIXED, GEN, FORM, SB, 0>(pts, gb, cnt, k, j, t, st, w, yv, sx, first, Ys, ys, Ucol, ldc, ak, bk,
-                                            invk);
+    walk_steps_reg<MIXED, GEN, FORM, SB, 0, YST>(pts, gb, cnt, k, j, t, st, w, yv, sx, first, Ys, ys, Ucol, ldc, ak,
+                                                 bk, invk);
 }
 
 // ---------------------------------------------------------------------------

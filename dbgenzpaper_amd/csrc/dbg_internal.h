@@ -181,6 +181,9 @@ struct BatchProblem {
     int64_t part_off;   // offset of the problem's tile partials (n_reps x tiles_per_rep)
     int32_t ldu;
     int32_t lower;      // 1 when every lower bound of the problem is -inf (the mixed walk's kPhiTailLower)
+    int64_t uc_off;     // mixed: offset (halves) of the split-FP16 factor, d + 32 columns of ucs halves
+    int32_t ucs;        // mixed: halves per split column, 64 per chunk of 32 rows
+    int32_t pad;
 };
 
 struct BatchArgs {
@@ -221,7 +224,8 @@ constexpr int kBatchMaxDim = 512;
 hipError_t batch_factor(const BatchArgs& B, bool mixed, void* Uk, void* Ucol, hipStream_t s);
 int batch_occupancy(bool mixed, int W);
 hipError_t batch_integrate(const BatchArgs& B, bool mixed, int W, int grid, const void* Uk,
-                           const void* Ucol, void* Y, double* part_m, int* part_e, hipStream_t s);
+                           const void* Ucol, const void* uc16, void* Y, double* part_m, int* part_e, hipStream_t s);
+hipError_t batch_pack_split(const BatchArgs& B, const void* Ucol, void* uc16, hipStream_t s);
 hipError_t batch_reduce(const BatchArgs& B, const double* part_m, const int* part_e, hipStream_t s);
 hipError_t queue_reset(const BatchArgs& B, hipStream_t s);
 int batch_tile();
