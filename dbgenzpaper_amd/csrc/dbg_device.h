@@ -166,21 +166,40 @@ __device__ __forceinline__ SobolLane sobol_lane(const PointArgs& P, int64_t j0, 
     return sl;
 }
 
-// NT >= 256 threads; dimensions [gb, gb + cnt), cnt <= SB; j0: the tile's
-// first point index
+// Sobol tables of a block of dimensions [gb, gb + cnt), cnt <= SB, for a tile
+// whose first point index is j0 (NT >= 256 threads).  The Sobol integer of a
+// lane's point in dimension gb + cc is the XOR of V[k] over the set bits k of
+// its Gray code, split as bits 0-3, 4-7 (the low byte, sl.lo()) and 8-31 (the
+// high part, common to the tile up to one carry, sl.hi()):
+//   s_sob[cc * 32 + e]       XOR of V[k], k in 0..3, over the bits of e < 16
+//   s_sob[cc * 32 + 16 + e]  XOR of V[4 + k], k in 0..3, over the bits of e
+//   s_sob[SB * 32 + 2 cc + h] the high part for H0 + h (h = 0, 1)
+// so that x = t[lo & 15] ^ t[16 + (lo >> 4)] ^ hi: the same XOR as one
+// 256-entry table of the low byte, at a sixteenth of the table work.
+constexpr int kSobTab = 32;  // words per dimension of the low tables
+
+template <int SB>
+constexpr int sobol_table_words()
+{
+    return SB * kSobTab + 2 * SB;
+}
+
 template <int SB, int NT>
 __device__ __forceinline__ void sobol_tables(const PointArgs& P, int gb, int cnt, int64_t j0,
                                              uint32_t* __restrict__ s_sob, int tid)
 {
-    static_assert(NT >= 256 && 2 * SB <= NT, "one thread per low entry");
-    if (tid < 256) {
-        const uint32_t e = (uint32_t)tid;
-        for (int cc = 0; cc < cnt; ++cc) {  // uniform
-            const uint32_t* __restrict__ V = P.V + (size_t)(gb + cc) * 32;
+    static_assert(NT >= 256 && SB * kSobTab <= 2 * NT && 2 * SB <= NT, "two low entries per thread");
+#pragma unroll
+    for (int rep = 0; rep < 2; ++rep) {
+        const int ei = tid + rep * NT;  // entry: dimension cc, part (bits 0-3 or 4-7), index e
+        const int cc = ei / kSobTab;
+        if (ei < SB * kSobTab && cc < cnt) {
+            const uint32_t e = (uint32_t)(ei & 15);
+            const uint32_t* __restrict__ V = P.V + (size_t)(gb + cc) * 32 + 4 * ((ei >> 4) & 1);
             uint32_t x = 0u;
 #pragma unroll
-            for (int k = 0; k < 8; ++k) x ^= ((e >> k) & 1u) ? V[k] : 0u;
-            s_sob[cc * 256 + e] = x;
+            for (int k = 0; k < 4; ++k) x ^= ((e >> k) & 1u) ? V[k] : 0u;
+            s_sob[ei] = x;
         }
     }
     if (tid < 2 * cnt) {
@@ -191,8 +210,17 @@ __device__ __forceinline__ void sobol_tables(const PointArgs& P, int gb, int cnt
         uint32_t x = 0u;
 #pragma unroll
         for (int b = 0; b < 24; ++b) x ^= ((gh >> b) & 1u) ? V[b] : 0u;
-        s_sob[SB * 256 + tid] = x;
+        s_sob[SB * kSobTab + tid] = x;
     }
+}
+
+// the lane's Sobol integer in dimension gb + cc from the block's tables
+template <int SB>
+__device__ __forceinline__ uint32_t sobol_from_tables(const uint32_t* __restrict__ s_sob, int cc, SobolLane sl)
+{
+    const uint32_t lo = sl.lo();
+    return s_sob[cc * kSobTab + (lo & 15u)] ^ s_sob[cc * kSobTab + 16 + (lo >> 4)] ^
+           s_sob[SB * kSobTab + 2 * cc + sl.hi()];
 }
 
 // The point-set tables enter each kernel as separate __restrict__ arguments:
@@ -690,7 +718,7 @@ __device__ __forceinline__ void walk_steps(const PointArgs& pts, int gb, int cnt
             T y;
             if (GEN == kGenSobol && s_sob != nullptr)
                 y = path_draw_sobol<MIXED, FORM>(pts, cur.pt, st,
-                                                 s_sob[CC * 256 + sl.lo()] ^ s_sob[SB * 256 + 2 * CC + sl.hi()]);
+                                                 sobol_from_tables<SB>(s_sob, CC, sl));
             else
                 y = path_draw<MIXED, GEN, FORM>(pts, cur.pt, st, g, j);
             yv[CC] = y;

@@ -131,7 +131,7 @@ __global__ __launch_bounds__(256, (MfmaCfg<MIXED, NARROW>::OCC)) void dense_mfma
     constexpr int kUS = PW + 16;       // LDS row stride of a U chunk (conflict-free reads)
     __shared__ T s_u[ULDS ? 2 * kUC * kUS : 1];
     constexpr bool SOBT = GEN == kGenSobol && !REV;  // walk points from LDS Sobol tables
-    __shared__ uint32_t s_sob[SOBT ? kSB * 256 + 2 * kSB : 1];
+    __shared__ uint32_t s_sob[SOBT ? sobol_table_words<kSB>() : 1];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;

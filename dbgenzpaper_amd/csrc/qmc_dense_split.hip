@@ -155,7 +155,7 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
     constexpr bool SOBT = GEN == kGenSobol && !REV;
     __shared__ double s_red[REV ? 4 * kSB * 3 : 1];
     uint32_t* const s_sob = reinterpret_cast<uint32_t*>(s_u + kSB * kUS);
-    static_assert(kSB * kUS * sizeof(float) + (kSB * 256 + 2 * kSB) * sizeof(uint32_t) <= sizeof(s_uc),
+    static_assert(kSB * kUS * sizeof(float) + sobol_table_words<kSB>() * sizeof(uint32_t) <= sizeof(s_uc),
                   "Sobol tables fit the chunk buffers");
 
     const int tid = threadIdx.x;
