@@ -33,7 +33,8 @@ EXPORTS = (
     "dbg_plan_set_sobol_directions", "dbg_qmc_pnorm", "dbg_qmc_loop", "dbg_plan_sync",
     "dbg_plan_last_kernel_ms", "dbg_plan_summary", "dbg_plan_get_factor", "dbg_plan_get_reps",
     "dbg_plan_get_state", "dbg_plan_set_state", "dbg_plan_refactor", "dbg_batch_create",
-    "dbg_batch_run", "dbg_batch_last_ms", "dbg_batch_tuning", "dbg_batch_destroy", "dbg_qmc_grad",
+    "dbg_batch_run", "dbg_batch_last_ms", "dbg_batch_tuning", "dbg_batch_set_width", "dbg_batch_destroy",
+    "dbg_qmc_grad",
     "dbg_plan_refine_info", "dbg_chol_genz_f64_dev_noalloc", "dbg_plan_set_grad_budget", "dbg_plan_grad_info",
     "dbg_plan_lowrank_info",
 )
@@ -112,6 +113,7 @@ def lib():
     L.dbg_batch_run.argtypes = [vp, dp]
     L.dbg_batch_last_ms.argtypes = [vp, C.POINTER(C.c_float)]
     L.dbg_batch_tuning.argtypes = [vp, ip32, C.POINTER(C.c_float)]
+    L.dbg_batch_set_width.argtypes = [vp, C.c_int32]
     L.dbg_batch_destroy.argtypes = [vp]
     for name in EXPORTS:
         if name not in ("dbg_last_error", "dbg_abi_version", "dbg_opts_default"):

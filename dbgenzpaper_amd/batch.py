@@ -71,6 +71,12 @@ class Batch:
         L.check(L.lib().dbg_batch_tuning(self._h, C.byref(w), t))
         return int(w.value), list(t)
 
+    def set_width(self, width: int) -> None:
+        """Override the autotuner's dimension-block width (8, 16, or 32: for
+        the mixed path the FP16 matrix-core contraction); the job's graph is
+        captured again."""
+        L.check(L.lib().dbg_batch_set_width(self._h, int(width)))
+
     def close(self):
         if getattr(self, "_h", None):
             L.lib().dbg_batch_destroy(self._h)

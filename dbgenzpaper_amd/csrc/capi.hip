@@ -1606,6 +1606,39 @@ int batch_validate(const std::vector<BatchProblem>& probs, const std::vector<int
     return DBG_OK;
 }
 
+// The whole job as one graph: factor, (mixed) split factors, queue reset,
+// integration at width Bt->W, reductions.  Replaces an earlier graph.
+int batch_capture(dbg_batch* Bt)
+{
+    const BatchArgs& B = Bt->B;
+    hipStream_t s = Bt->stream;
+    const int wsel = Bt->W == 8 ? 0 : (Bt->W == 16 ? 1 : 2);
+    if (Bt->exec) {
+        HIPCHK(hipGraphExecDestroy(Bt->exec));
+        Bt->exec = nullptr;
+    }
+    if (Bt->graph) {
+        HIPCHK(hipGraphDestroy(Bt->graph));
+        Bt->graph = nullptr;
+    }
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t e = batch_factor(B, Bt->mixed, Bt->Uk, Bt->Ucol, s);
+    if (e == hipSuccess && Bt->mixed) e = batch_pack_split(B, Bt->Ucol, Bt->uc16, s);
+    if (e == hipSuccess) e = queue_reset(B, s);
+    if (e == hipSuccess)
+        e = batch_integrate(B, Bt->mixed, Bt->W, Bt->grid_w[wsel], Bt->Uk, Bt->Ucol, Bt->uc16, Bt->Y, Bt->part_m,
+                            Bt->part_e, s);
+    if (e == hipSuccess) e = batch_reduce(B, Bt->part_m, Bt->part_e, s);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(s, &g);
+    Bt->graph = g;
+    HIPCHK(e);
+    HIPCHK(e2);
+    HIPCHK(hipGraphInstantiate(&Bt->exec, Bt->graph, nullptr, nullptr, 0));
+    HIPCHK(hipStreamSynchronize(s));
+    return DBG_OK;
+}
+
 int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const double* a,
                 const double* b, const double* shifts)
 {
@@ -1769,7 +1802,6 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
     HIPCHK(batch_factor(B, Bt->mixed, Bt->Uk, Bt->Ucol, s));
     if (Bt->mixed) HIPCHK(batch_pack_split(B, Bt->Ucol, Bt->uc16, s));
     float best = 1e30f;
-    int wsel = 1;
     BatchArgs Bs = B;
     Bs.items = Bt->sample_items;
     Bs.n_items = Bt->n_sample;
@@ -1792,26 +1824,9 @@ int batch_build(dbg_batch* Bt, const int32_t* dims, const double* sigmas, const 
         if (msum < best) {
             best = msum;
             Bt->W = widths[w];
-            wsel = w;
         }
     }
-    // ---- the whole job as one graph: factor, queue reset, integration, reductions
-    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    hipError_t e = batch_factor(B, Bt->mixed, Bt->Uk, Bt->Ucol, s);
-    if (e == hipSuccess && Bt->mixed) e = batch_pack_split(B, Bt->Ucol, Bt->uc16, s);
-    if (e == hipSuccess) e = queue_reset(B, s);
-    if (e == hipSuccess)
-        e = batch_integrate(B, Bt->mixed, Bt->W, Bt->grid_w[wsel], Bt->Uk, Bt->Ucol, Bt->uc16, Bt->Y, Bt->part_m,
-                            Bt->part_e, s);
-    if (e == hipSuccess) e = batch_reduce(B, Bt->part_m, Bt->part_e, s);
-    hipGraph_t g = nullptr;
-    hipError_t e2 = hipStreamEndCapture(s, &g);
-    Bt->graph = g;
-    HIPCHK(e);
-    HIPCHK(e2);
-    HIPCHK(hipGraphInstantiate(&Bt->exec, Bt->graph, nullptr, nullptr, 0));
-    HIPCHK(hipStreamSynchronize(s));
-    return DBG_OK;
+    return batch_capture(Bt);
 }
 
 }  // namespace
@@ -1870,6 +1885,16 @@ int dbg_batch_last_ms(dbg_batch* Bt, float* ms)
     if (!Bt || !ms) return fail(DBG_ERR_ARG, "null argument");
     HIPCHK(hipEventElapsedTime(ms, Bt->ev0, Bt->ev1));
     return DBG_OK;
+}
+
+int dbg_batch_set_width(dbg_batch* Bt, int32_t width)
+{
+    g_err.clear();
+    if (!Bt) return fail(DBG_ERR_ARG, "null batch");
+    if (width != 8 && width != 16 && width != 32) return fail(DBG_ERR_ARG, "batch width must be 8, 16 or 32");
+    HIPCHK(hipStreamSynchronize(Bt->stream));
+    Bt->W = width;
+    return batch_capture(Bt);
 }
 
 int dbg_batch_tuning(dbg_batch* Bt, int32_t* width, float* ms_w8_w16_w32)

@@ -315,6 +315,10 @@ int dbg_batch_run(dbg_batch* batch, double* results);
 int dbg_batch_last_ms(dbg_batch* batch, float* ms);
 /* chosen block width and the trial times of widths 8, 16, 32 (ms) */
 int dbg_batch_tuning(dbg_batch* batch, int32_t* width, float* ms_w8_w16_w32);
+/* Override of the autotuner's dimension-block width (8 or 16: vector-ALU
+ * contraction; 32: for DBG_PREC_MIXED the FP16 matrix-core contraction): the
+ * job's graph is captured again at that width (no reference counterpart) */
+int dbg_batch_set_width(dbg_batch* batch, int32_t width);
 int dbg_batch_destroy(dbg_batch* batch);
 
 #ifdef __cplusplus

@@ -6,6 +6,7 @@ import pytest
 
 import dbgenzpaper_amd as dbg
 from dbgenzpaper_amd import workloads as W
+from dbgenzpaper_amd import _lib as L
 from dbgenzpaper_amd.batch import Batch
 from tests._helpers import rel
 
@@ -52,14 +53,56 @@ def test_batch_fp64_matches_single_plans():
         assert rel(r.value[k], v) < 1e-12, (k, p.d, r.value[k], v)
 
 
-def test_batch_mixed_close_to_fp64():
+@pytest.mark.parametrize("width", [16, 32])
+def test_batch_mixed_close_to_fp64(width):
+    """The mixed batch at a fixed block width (16: vector-ALU contraction;
+    32: FP16 matrix cores) against the FP64 batch."""
     probs = W.batch_uneven(32, d_min=2, d_max=256, m=2048, n_reps=4, seed=32)
-    print("batch mixed K=32 m=2048 dims", sorted(p.d for p in probs), flush=True)
-    r32 = Batch(*_args(probs), T=np.float32, m=2048, n_reps=4).run()
+    print(f"batch mixed K=32 m=2048 width {width} dims", sorted(p.d for p in probs), flush=True)
+    B32 = Batch(*_args(probs), T=np.float32, m=2048, n_reps=4)
+    B32.set_width(width)
+    assert B32.tuning()[0] == width
+    r32 = B32.run()
+    B32.close()
     r64 = Batch(*_args(probs), T=np.float64, m=2048, n_reps=4).run()
     err = np.abs(np.expm1(r32.log_value - r64.log_value))
     print("batch mixed worst rel err", err.max(), flush=True)
     assert np.all(np.isfinite(r32.log_value)) and err.max() < 1e-4
+
+
+def test_batch_matrix_core_block_edges(oracle):
+    """The matrix-core contraction (mixed, width 32) on dimensions around the
+    32-wide blocks (d - 1 = 31, 32, 33, 63, 64, 65, 96, 128, 255), one and two
+    points past a tile, against the oracle in FP64 on the same factor inputs
+    (the mixed path's bound, 1e-4 relative) and against the vector-ALU width
+    16 (the same walk on differently rounded sums)."""
+    rng = np.random.default_rng(36)
+    probs = []
+    for d in (2, 32, 33, 34, 64, 65, 66, 97, 129, 256):
+        P = W.dense_random(d, m=513, n_reps=2, seed=int(rng.integers(1 << 30)))
+        P.shifts = np.asfortranarray(rng.random((d - 1, 2)))
+        probs.append(P)
+    out = {}
+    for width in (16, 32):
+        B = Batch(*_args(probs), T=np.float32, m=513, n_reps=2)
+        B.set_width(width)
+        out[width] = B.run()
+        r2 = B.run()
+        assert np.array_equal(out[width].value, r2.value)  # graph replay
+        B.close()
+    for k, p in enumerate(probs):
+        ro = _oracle(oracle, p)
+        for width in (16, 32):
+            assert rel(out[width].value[k], ro.value) < 1e-4, (width, p.d, out[width].value[k], ro.value)
+        assert rel(out[32].value[k], out[16].value[k]) < 1e-4
+
+
+def test_batch_set_width_rejects():
+    probs = W.batch_uneven(2, d_min=3, d_max=5, m=256, n_reps=1, seed=37)
+    B = Batch(*_args(probs), T=np.float32, m=256, n_reps=1)
+    with pytest.raises(L.DbgError):
+        B.set_width(12)
+    B.close()
 
 
 def test_batch_ragged_edges(oracle):

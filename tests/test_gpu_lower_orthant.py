@@ -79,7 +79,8 @@ def test_lower_orthant_gradient_record_pass():
         assert np.abs(x - y).max() <= 1e-12 * np.abs(y).max(), np.abs(x - y).max()
 
 
-def test_lower_orthant_batch():
+@pytest.mark.parametrize("width", [16, 32])
+def test_lower_orthant_batch(width):
     """A batch mixing lower-orthant problems with problems whose lower bounds
     are finite: each problem's estimate equals the one with a = -1e30 (the
     batch factors in one pivot window; these random-correlation problems with
@@ -94,6 +95,7 @@ def test_lower_orthant_batch():
     def run(ps):
         B = Batch([p.Sigma for p in ps], [p.a for p in ps], [p.b for p in ps], [p.shifts for p in ps],
                   T=np.float32, m=1024, n_reps=2)
+        B.set_width(width)  # both batches at the same width (16: vector ALU, 32: matrix cores)
         r = B.run()
         B.close()
         return r
