@@ -63,17 +63,21 @@ def test_grad_fp64_matches_oracle(dbg, oracle, n, m, R, gen):
 @pytest.mark.parametrize("n,gen", [(64, "Richtmyer"), (200, "Richtmyer"), (300, "Sobol")])
 def test_grad_mixed_close_to_oracle(dbg, oracle, n, gen):
     """Mixed forward and sweep (FP64 sums; from d = 96 the split-FP16 matrix-core
-    kernel in record and reverse modes and the split Ubar product): within
-    2e-3 of the FP64 oracle in norm."""
+    kernel in record and reverse modes and the split Ubar product) against
+    the FP64 oracle on the same factor, points and shifts: log P within 1e-6
+    relative, the gradients within 1e-5 in norm (measured: log P 6e-9 to
+    3e-8, gradients 9e-7 to 2.1e-6)."""
     print(f"grad mixed n={n} {gen}", flush=True)
     P = W.factor_model(n, 4, m=1024, n_reps=2, seed=n, u_range=(0.05, 0.3), gen=gen)
     D = _plan(dbg, P.Sigma, P.a, P.b, np.float32, 1024, 2, gen)
     D.set_shifts(P.shifts)
     g = D.grad()
     ro = _oracle_grad(oracle, D, gen)
-    assert abs(g.log_value - ro.log_value) < 1e-4 * abs(ro.log_value)
-    assert _relnorm(g.b_std, ro.gb) < 2e-3
-    assert _relnorm(np.triu(g.U), np.triu(ro.gU)) < 2e-3
+    print(f"  log P rel {abs(g.log_value - ro.log_value) / abs(ro.log_value):.2e}, "
+          f"b {_relnorm(g.b_std, ro.gb):.2e}, U {_relnorm(np.triu(g.U), np.triu(ro.gU)):.2e}", flush=True)
+    assert abs(g.log_value - ro.log_value) < 1e-6 * abs(ro.log_value)
+    assert _relnorm(g.b_std, ro.gb) < 1e-5
+    assert _relnorm(np.triu(g.U), np.triu(ro.gU)) < 1e-5
 
 
 @pytest.mark.parametrize("n,T,m,R", [(40, np.float64, 700, 3), (150, np.float64, 1000, 3),
@@ -131,7 +135,8 @@ def test_grad_c4_central_differences(dbg):
     the test): the mixed and the FP64 gradients against FP64 central
     differences (h = 1e-5) of the device estimate on the same points, for 32
     random coordinates of b and 32 of Sigma (pivot order held: a coordinate
-    whose perturbation changes it is skipped)."""
+    whose perturbation changes it is skipped).  Bounds: 2e-6 (FP64) and
+    1e-4 (mixed) of max(|fd|, 1); measured 2.7e-7 and 9.4e-6."""
     d, m, R, h = 1000, 2 ** 13, 8, 1e-5
     P = W.factor_model(d, 16, m=m, n_reps=R, seed=4, u_range=(0.02, 0.20))
     print("grad c4 fd", flush=True)
@@ -182,5 +187,5 @@ def test_grad_c4_central_differences(dbg):
         checked += 1
     print(f"c4 fd: {checked} coordinates, worst fp64 {worst64:.2e}, mixed {worst32:.2e}", flush=True)
     assert checked >= 48
-    assert worst64 < 1e-4
-    assert worst32 < 2e-2
+    assert worst64 < 2e-6
+    assert worst32 < 1e-4
