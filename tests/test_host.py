@@ -271,9 +271,9 @@ def _julia_struct_fields(md, name):
 
 
 def test_julia_binding_matches_the_header():
-    """INTEGRATION.md's Julia structs have the C structs' fields in order and
-    type, the ctypes mirror too, and the routing function builds DbgOpts with
-    one argument per field."""
+    """The Julia structs of the binding guide at the repository root have the
+    C structs' fields in order and type, the ctypes mirror too, and the
+    guide's routing function builds DbgOpts with one argument per field."""
     from dbgenzpaper_amd import _lib
 
     hdr = open(os.path.join(ROOT, "include", "dbgenz.h")).read()
