@@ -325,6 +325,15 @@ struct Weight {
         pm = frexp_(pm * dc, &e);
         pe += (pm == T(0)) ? 0 : e;
     }
+    // pm back to [1/2, 1) after products taken without normalization (the
+    // exponent moves to pe; a power-of-two scale changes no product's rounding
+    // while it stays normal)
+    __device__ __forceinline__ void settle()
+    {
+        int e = 0;
+        pm = frexp_(pm, &e);
+        pe += (pm == T(0)) ? 0 : e;
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -529,6 +538,7 @@ struct Path<false, FORM> {
         }
     }
     static __device__ __forceinline__ void weigh(Weight<T>& w, const State& st) { w.mul(st.dc); }
+    static __device__ __forceinline__ void settle(Weight<T>&) {}
 };
 
 // tail form in FP64
@@ -548,6 +558,7 @@ struct Path<false, kPhiTail> {
     }
     static __device__ __forceinline__ void phi_z(State& st, double za, double zb) { tail_from_z(st, za, zb); }
     static __device__ __forceinline__ void weigh(Weight<T>& w, const State& st) { w.mul(st.dc); }
+    static __device__ __forceinline__ void settle(Weight<T>&) {}
 };
 
 // the FP32 mixed path (tail form, any FORM; kPhiTailLower when every lower
@@ -568,10 +579,20 @@ struct Path<true, FORM> {
         if constexpr (FORM == kPhiTailLower) tail_from_zb(st, (b - s) * inv);
         else tail_from_z(st, (a - s) * inv, (b - s) * inv);
     }
+    // The lower-orthant form multiplies without normalizing: each factor
+    // (m in [0.02, 0.71] of a tail mass m 2^-k, or 1 - m 2^-k >= 1/2, or 0)
+    // keeps 16 products of a weight in [1/2, 1) above 2^-91, so the walks
+    // settle the weight every 16 steps and at the end of each block, with
+    // the products rounded exactly as with normalization at every step.
     static __device__ __forceinline__ void weigh(Weight<T>& w, const State& st)
     {
-        w.mul(st.dc);
+        if constexpr (FORM == kPhiTailLower) w.pm *= st.dc;
+        else w.mul(st.dc);
         w.pe -= st.k;
+    }
+    static __device__ __forceinline__ void settle(Weight<T>& w)
+    {
+        if constexpr (FORM == kPhiTailLower) w.settle();
     }
 };
 
@@ -733,6 +754,7 @@ __device__ __forceinline__ void walk_steps(const PointArgs& pts, int gb, int cnt
             if constexpr (SX) const_cast<T*>(s_x)[t * SXS + CC] = acc;
             PT::phi(st, acc, cur.a, cur.b, cur.inv);
             PT::weigh(w, st);
+            if constexpr (CC % 16 == 15) PT::settle(w);
             if constexpr (CC + 1 < SB)
                 walk_steps<MIXED, GEN, FORM, TS, SXS, SB, CC + 1, YST, SX>(pts, gb, cnt, k, j, t, st, w, yv, nxt, s_x,
                                                                        s_y, Ys, ys, Ucol, ldc, ak, bk, invk, srec,
@@ -758,6 +780,8 @@ __device__ __forceinline__ void walk_block_yv(T (&yv)[SB], const PointArgs& pts,
     if (cnt > 0) first.template fetch<0>(pts, gb, k, Ucol, ldc, ak, bk, invk);
     walk_steps<MIXED, GEN, FORM, TS, SXS, SB, 0, YST, SX>(pts, gb, cnt, k, j, t, st, w, yv, first, s_x, s_y, Ys, ys,
                                                       Ucol, ldc, ak, bk, invk, srec, s_sob, sl);
+    if constexpr (SB % 16 != 0) Path<MIXED, FORM>::settle(w);
+    else if (cnt % 16 != 0) Path<MIXED, FORM>::settle(w);  // uniform: a short last block
 }
 
 template <bool MIXED, int GEN, int FORM, int TS, int SXS, int SB, typename T, typename State>
@@ -802,6 +826,7 @@ __device__ __forceinline__ void walk_steps_reg(const PointArgs& pts, int gb, int
             for (int l = 0; l <= CC; ++l) acc = fma(yv[l], cur.u[l], acc);
             PT::phi(st, acc, cur.a, cur.b, cur.inv);
             PT::weigh(w, st);
+            if constexpr (CC % 16 == 15) PT::settle(w);
             if constexpr (CC + 1 < SB)
                 walk_steps_reg<MIXED, GEN, FORM, SB, CC + 1, YST>(pts, gb, cnt, k, j, t, st, w, yv, sx, nxt, Ys, ys,
                                                                   Ucol, ldc, ak, bk, invk);
@@ -822,6 +847,8 @@ __device__ __forceinline__ void walk_block_reg(const PointArgs& pts, int gb, int
     if (cnt > 0) first.template fetch<0>(pts, gb, k, Ucol, ldc, ak, bk, invk);
     walk_steps_reg<MIXED, GEN, FORM, SB, 0, YST>(pts, gb, cnt, k, j, t, st, w, yv, sx, first, Ys, ys, Ucol, ldc, ak,
                                                  bk, invk);
+    if constexpr (SB % 16 != 0) Path<MIXED, FORM>::settle(w);
+    else if (cnt % 16 != 0) Path<MIXED, FORM>::settle(w);  // uniform: a short last block
 }
 
 // ---------------------------------------------------------------------------
