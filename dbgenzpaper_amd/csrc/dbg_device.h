@@ -284,15 +284,16 @@ __device__ __forceinline__ uint32_t sobol_u32_from_x(const PointArgs& P, uint32_
 // from the fixed-point u; u = 0 gives w = 1.
 __device__ __forceinline__ void tent_w(uint32_t u, float& w, float& wc)
 {
-    // tw = 2u mod 2^32 when u >= 1/2, else 2^32 - 2u (mod 2^32), without a
-    // branch: nm is all ones exactly when the top bit of u is clear, and
-    // (v ^ nm) - nm negates v then
+    // with u2 = 2u mod 2^32: for u >= 1/2, w = u2 2^-32 and 1 - w = (2^32 - u2)
+    // 2^-32; for u < 1/2 the two swap.  2^32 - u2 is -u2 mod 2^32 except at
+    // u2 = 0 (u = 0 or 1/2), where it is 2^32: the value 1.
     const uint32_t u2 = u << 1;
-    const uint32_t nm = ~(uint32_t)((int32_t)u >> 31);
-    const uint32_t tw = (u2 ^ nm) - nm;
     constexpr float kS = 2.3283064365386963e-10f;  // 2^-32
-    w = (u == 0u) ? 1.0f : (float)tw * kS;
-    wc = (u == 0u) ? 0.0f : (tw == 0u ? 1.0f : (float)(0u - tw) * kS);
+    const float a = (float)u2 * kS;
+    const float b = (u2 == 0u) ? 1.0f : (float)(0u - u2) * kS;
+    const bool hi = (int32_t)u < 0;
+    w = hi ? a : b;
+    wc = hi ? b : a;
 }
 
 // w = u and 1 - u without the tent map (the SparseQMC module)
