@@ -280,30 +280,6 @@ __device__ __forceinline__ uint32_t sobol_u32_from_x(const PointArgs& P, uint32_
     return (uint32_t)(P.sobol_bits <= 32 ? (v << (32 - P.sobol_bits)) : (v >> (P.sobol_bits - 32)));
 }
 
-// w = |2u - 1| (the tent map of src/qmc_generators.jl:160) and 1 - w in FP32
-// from the fixed-point u; u = 0 gives w = 1.
-__device__ __forceinline__ void tent_w(uint32_t u, float& w, float& wc)
-{
-    // with u2 = 2u mod 2^32: for u >= 1/2, w = u2 2^-32 and 1 - w = (2^32 - u2)
-    // 2^-32; for u < 1/2 the two swap.  2^32 - u2 is -u2 mod 2^32 except at
-    // u2 = 0 (u = 0 or 1/2), where it is 2^32: the value 1.
-    const uint32_t u2 = u << 1;
-    constexpr float kS = 2.3283064365386963e-10f;  // 2^-32
-    const float a = (float)u2 * kS;
-    const float b = (u2 == 0u) ? 1.0f : (float)(0u - u2) * kS;
-    const bool hi = (int32_t)u < 0;
-    w = hi ? a : b;
-    wc = hi ? b : a;
-}
-
-// w = u and 1 - u without the tent map (the SparseQMC module)
-__device__ __forceinline__ void plain_w(uint32_t u, float& w, float& wc)
-{
-    constexpr float kS = 2.3283064365386963e-10f;
-    w = (float)u * kS;
-    wc = (u == 0u) ? 1.0f : (float)(0u - u) * kS;
-}
-
 // ---------------------------------------------------------------------------
 // Scaled weight product: p = pm * 2^pe with pm in [0.5, 1) (or 0).
 // ---------------------------------------------------------------------------

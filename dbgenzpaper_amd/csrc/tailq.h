@@ -2,16 +2,22 @@
 // carries probability masses as p = t 2^-k (t a normal FP32 number, k >= 0 an
 // integer scale) so that masses far below the FP32 range need no FP64:
 //   tail_mass_f32      erfc(z)/2 for z >= 0, as m 2^-k;
-//   tail_quantile_f32  Phi^{-1}(p) for p = t 2^-k <= 1/2.
-// Both are polynomials fitted for this code (fit_tailq.py beside this file
-// reproduces every coefficient) and are branch-free or nearly so, since the
-// integration evaluates them once per sample and dimension in SIMD.
+//   tail_quantile_f32  Phi^{-1}(p) for p = t 2^-k <= 1/2;
+// and the map from a point's fixed-point coordinate u (32 fraction bits) to
+// the FP32 pair (w, 1 - w) that the walk samples with:
+//   tent_w             w = |2u - 1| (the tent map);
+//   plain_w            w = u (no tent map: the SparseQMC module).
+// The two tail functions are polynomials fitted for this code (fit_tailq.py
+// beside this file reproduces every coefficient) and are branch-free or nearly
+// so, since the integration evaluates them once per sample and dimension in
+// SIMD.
 //
 // The same source compiles for the host (plain C++) so that the CPU test
-// suite can check both against high-precision values.
+// suite can check these functions against exact or high-precision values.
 #pragma once
 
 #include <math.h>
+#include <stdint.h>
 
 #if defined(__HIPCC__)
 #define DBG_HD __host__ __device__
@@ -156,6 +162,30 @@ DBG_HD inline float tail_quantile_f32(float t, int k)
                           -1.639978766e+00f, 1.008415103e+00f, 6.785751786e-03f, -2.852775156e-01f,
                           1.368908286e+00f};
     return -s * horner_f32(x - 0.21f, CB);
+}
+
+// w = |2u - 1| (the tent map of src/qmc_generators.jl:160) and 1 - w in FP32
+// from the fixed-point u; u = 0 gives w = 1.
+DBG_HD inline void tent_w(uint32_t u, float& w, float& wc)
+{
+    // with u2 = 2u mod 2^32: for u >= 1/2, w = u2 2^-32 and 1 - w = (2^32 - u2)
+    // 2^-32; for u < 1/2 the two swap.  2^32 - u2 is -u2 mod 2^32 except at
+    // u2 = 0 (u = 0 or 1/2), where it is 2^32: the value 1.
+    const uint32_t u2 = u << 1;
+    constexpr float kS = 2.3283064365386963e-10f;  // 2^-32
+    const float a = (float)u2 * kS;
+    const float b = (u2 == 0u) ? 1.0f : (float)(0u - u2) * kS;
+    const bool hi = (int32_t)u < 0;
+    w = hi ? a : b;
+    wc = hi ? b : a;
+}
+
+// w = u and 1 - u without the tent map (the SparseQMC module)
+DBG_HD inline void plain_w(uint32_t u, float& w, float& wc)
+{
+    constexpr float kS = 2.3283064365386963e-10f;
+    w = (float)u * kS;
+    wc = (u == 0u) ? 1.0f : (float)(0u - u) * kS;
 }
 
 }  // namespace dbg

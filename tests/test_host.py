@@ -188,6 +188,41 @@ def test_fp32_tail_mass_against_mpmath(tmp_path, rcp_ulp):
     assert lib.tm(45.0, ctypes.byref(k)) == lib.tm(30.0, ctypes.byref(kc)) and k.value == kc.value
 
 
+def test_tent_and_plain_maps_exact(tmp_path):
+    """The walk's map from a point's fixed-point coordinate u (32 fraction
+    bits) to FP32 w and 1 - w: the tent map w = |2u - 1|
+    (src/qmc_generators.jl:160) and, for the SparseQMC module, w = u, each of
+    w and 1 - w the correctly rounded FP32 value of the exact fraction.  Exact
+    values from integer arithmetic (|2u - 2^32| and its complement are
+    integers below 2^33, exact in FP64 and scaled by 2^-32 exactly, then
+    rounded once to FP32), on every u with 18 leading bits fixed at random
+    plus the edges 0, 1/2 and the values next to them."""
+    lib = _host_build(tmp_path, '#include "tailq.h"\n'
+                      'extern "C" void maps(const unsigned* u, long n, float* tw, float* tc, float* pw, float* pc) {\n'
+                      '  for (long i = 0; i < n; ++i) { dbg::tent_w(u[i], tw[i], tc[i]); dbg::plain_w(u[i], pw[i], pc[i]); }\n'
+                      '}\n', "maps")
+    rng = np.random.default_rng(11)
+    lo = np.arange(1 << 14, dtype=np.uint64)
+    hi = rng.integers(0, 1 << 18, 64, dtype=np.uint64) << np.uint64(14)
+    edges = np.array([0, 1, 2, 3, 2**31 - 2, 2**31 - 1, 2**31, 2**31 + 1, 2**31 + 2, 2**32 - 2, 2**32 - 1],
+                     dtype=np.uint64)
+    u = np.concatenate([(hi[:, None] + lo[None, :]).ravel(), edges]).astype(np.uint32)
+    n = u.size
+    tw, tc, pw, pc = (np.empty(n, np.float32) for _ in range(4))
+    f = ctypes.POINTER(ctypes.c_float)
+    lib.maps(u.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), ctypes.c_long(n), tw.ctypes.data_as(f),
+             tc.ctypes.data_as(f), pw.ctypes.data_as(f), pc.ctypes.data_as(f))
+    ui = u.astype(np.int64)
+    a = np.abs(2 * ui - 2**32)  # 2^32 |2u - 1|
+    s = 2.0 ** -32
+    np.testing.assert_array_equal(tw, (a * s).astype(np.float32))
+    np.testing.assert_array_equal(tc, ((2**32 - a) * s).astype(np.float32))
+    np.testing.assert_array_equal(pw, (ui * s).astype(np.float32))
+    np.testing.assert_array_equal(pc, ((2**32 - ui) * s).astype(np.float32))
+    # u = 0 and u = 1/2: w = 1 and 0
+    assert (tw[-11], tc[-11], tw[-5], tc[-5]) == (1.0, 0.0, 0.0, 1.0)
+
+
 def test_plan_option_validation(dbg):
     """dbg_plan_create rejects invalid options before touching the device:
     the contraction option (DBG_CONTRACT_*) is checked against the path."""
