@@ -496,13 +496,22 @@ __global__ __launch_bounds__(UbSplitCfg<TT>::NT, UbSplitCfg<TT>::OCC) void ubar_
                 const half8 bh = *(const half8*)(rb + 16 * j * kUsRow + rh);
                 const half8 bl = *(const half8*)(rb + 16 * j * kUsRow + rl);
                 const float sc = s_sc[buf][wc * Cfg::WC + 16 * j + lr];
+                {
+                    // the four row blocks' products as independent chains, so
+                    // that each matrix instruction's result is not waited on
+                    // by the next one
+                    V a[4];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    V acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh, V{0, 0, 0, 0}, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl, acc, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh, acc, 0, 0, 0);
+                    for (int i = 0; i < 4; ++i)
+                        a[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh, V{0, 0, 0, 0}, 0, 0, 0);
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) tot[i][j][r] = fmaf(acc[r], sc, tot[i][j][r]);
+                    for (int i = 0; i < 4; ++i) a[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl, a[i], 0, 0, 0);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) a[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh, a[i], 0, 0, 0);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) tot[i][j][r] = fmaf(a[i][r], sc, tot[i][j][r]);
                 }
             }
         }
