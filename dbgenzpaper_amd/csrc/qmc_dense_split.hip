@@ -95,9 +95,11 @@ __device__ __forceinline__ void store_y16(_Float16* __restrict__ Ys16, int gb, i
             hi[e] = a;
             lo[e] = b;
         }
+        // non-temporal: the slot is re-read from HBM by later panels, and
+        // cached writes would evict the factor chunks that all workgroups share
         _Float16* dst = Ys16 + (size_t)(gb / 8 + h) * kYG + (size_t)t * 8;
-        *(half8*)dst = hi;
-        *(half8*)(dst + kTS * 8) = lo;
+        __builtin_nontemporal_store(hi, (half8*)dst);
+        __builtin_nontemporal_store(lo, (half8*)(dst + kTS * 8));
     }
 }
 
@@ -343,8 +345,8 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
 #pragma unroll
                     for (int c = 0; c < kSB; ++c) {
                         if (c < cnt) {  // uniform
-                            yrec[(size_t)(gb + c) * A.rs + tid] = s_y[c * kTS + tid];
-                            srec[(size_t)(gb + c) * A.rs + tid] = s_x[tid * kSXS + c];
+                            __builtin_nontemporal_store(s_y[c * kTS + tid], &yrec[(size_t)(gb + c) * A.rs + tid]);
+                            __builtin_nontemporal_store(s_x[tid * kSXS + c], &srec[(size_t)(gb + c) * A.rs + tid]);
                         }
                     }
                 }
@@ -357,7 +359,8 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
 #pragma unroll
                     for (int c = 0; c < kSB; ++c) {
                         if (c < cnt)  // uniform
-                            srec[(size_t)(n - 2 - (gb + c)) * A.rs + tid] = s_y[c * kTS + tid] * unsc;
+                            __builtin_nontemporal_store(s_y[c * kTS + tid] * unsc,
+                                                        &srec[(size_t)(n - 2 - (gb + c)) * A.rs + tid]);
                     }
                     // per-index sums of the four waves -> the global accumulators
                     if (tid < 3 * cnt) {
