@@ -64,6 +64,7 @@ constexpr float kYScale = 64.0f;     // 2^6
 constexpr float kUScale = 8192.0f;   // 2^13
 constexpr float kAccScale = 1.0f / 524288.0f;  // 2^-19
 constexpr float kHalfMax = 65504.0f;
+constexpr int kLoadNT = 2;           // cache policy of a direct load: non-temporal (nt)
 
 __device__ __forceinline__ floatx4 mfma16(half8 a, half8 b, floatx4 c)
 {
@@ -244,8 +245,11 @@ __global__ __launch_bounds__(256, 2) void dense_split_kernel(DenseArgs A, const 
                                                          s_uc + buf * kChunk + (32 * wv + 8 * q) * 64, 16, 0, 0);
 #pragma unroll
                     for (int q = 0; q < 8; ++q)
+                        // nt: the y rows stream from HBM (each is read once per
+                        // panel); the non-temporal hint keeps them from evicting
+                        // the factor chunks in L2
                         __builtin_amdgcn_global_load_lds(yd + (size_t)(4 * c + (q >> 1)) * kYG + (q & 1) * kTS * 8,
-                                                         ywv + q * 512, 16, 0, 0);
+                                                         ywv + q * 512, 16, 0, kLoadNT);
                 };
                 issue(0, 0);
                 asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
