@@ -964,8 +964,9 @@ __device__ __forceinline__ void rev_steps(const PointArgs& pts, int n, int gb, i
             const RevOps<MIXED, GEN, SB, T>& cur = ops[CC & 1];
             if constexpr (CC + 1 < SB) {
                 if (CC + 1 < cnt) {
-                    s_nx = srec[(size_t)(r - 2) * rs + t];
-                    y_nx = yrec[(size_t)(r - 1) * rs + t];  // r - 1 <= n - 2
+                    // records stream from HBM once: non-temporal loads
+                    s_nx = __builtin_nontemporal_load(&srec[(size_t)(r - 2) * rs + t]);
+                    y_nx = __builtin_nontemporal_load(&yrec[(size_t)(r - 1) * rs + t]);  // r - 1 <= n - 2
                     ops[(CC + 1) & 1].template fetch<CC + 1>(pts, n, gb, k, Uf, ldc, ak, bk, invk);
                 }
             }
@@ -1043,8 +1044,8 @@ __device__ __forceinline__ void record_first_step(int n, int gb, int t, const T*
                                                   const T* __restrict__ yrec, int64_t rs, T& s_nx, T& y_nx)
 {
     const int r0 = n - 1 - gb;
-    s_nx = srec[(size_t)(r0 - 1) * rs + t];
-    y_nx = (r0 <= n - 2) ? yrec[(size_t)r0 * rs + t] : T(0);
+    s_nx = __builtin_nontemporal_load(&srec[(size_t)(r0 - 1) * rs + t]);
+    y_nx = (r0 <= n - 2) ? __builtin_nontemporal_load(&yrec[(size_t)r0 * rs + t]) : T(0);
 }
 
 template <bool MIXED, int GEN, int TS, int SXS, int SB, typename T>
