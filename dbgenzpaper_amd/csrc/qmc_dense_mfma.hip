@@ -65,11 +65,23 @@ struct alignas(16) YVec {
     T v[NSB];
 };
 
-// Y rows k0 + lq of the lane's NSB samples
+// Y rows k0 + lq of the lane's NSB samples; non-temporal loads (the rows
+// stream from HBM once per panel and would otherwise crowd the factor rows
+// out of L2)
 template <typename T, int NSB>
 __device__ __forceinline__ void load_y(T (&av)[NSB], const T* __restrict__ ya, int64_t ys, int k0)
 {
     static_assert(sizeof(YVec<T, NSB>) % 16 == 0, "16-byte loads per lane and K-step");
+    typedef T VT __attribute__((ext_vector_type(NSB)));
+    const VT v = __builtin_nontemporal_load((const VT*)(ya + (size_t)k0 * ys));
+#pragma unroll
+    for (int sb = 0; sb < NSB; ++sb) av[sb] = v[sb];
+}
+
+// the same operands from the LDS copy of a sub-block's y
+template <typename T, int NSB>
+__device__ __forceinline__ void load_y_lds(T (&av)[NSB], const T* __restrict__ ya, int64_t ys, int k0)
+{
     const YVec<T, NSB> v = *(const YVec<T, NSB>*)(ya + (size_t)k0 * ys);
 #pragma unroll
     for (int sb = 0; sb < NSB; ++sb) av[sb] = v.v[sb];
@@ -333,7 +345,7 @@ __global__ __launch_bounds__(256, (MfmaCfg<MIXED, NARROW>::OCC)) void dense_mfma
 #pragma unroll
                     for (int ks = 0; ks < kSB / 4; ++ks) {
                         T av[NSB];
-                        load_y<T, NSB>(av, s_y + (size_t)lq * TS + wv * WS + NSB * lr, TS, 4 * ks);
+                        load_y_lds<T, NSB>(av, s_y + (size_t)lq * TS + wv * WS + NSB * lr, TS, 4 * ks);
                         const T* __restrict__ ub = UPDL ? s_u + (4 * ks + lq) * kUS + lr
                                                         : Uk + (size_t)(gb + 4 * ks + lq) * ldu + g0 + 1 + lr;
 #pragma unroll
