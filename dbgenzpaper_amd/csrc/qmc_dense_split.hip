@@ -40,7 +40,10 @@
 // granules of row r permuted by r mod 8 so that the operand reads are free of
 // bank conflicts.  The y operands of the next chunk (8 KB per wavefront) go to
 // LDS by direct loads as well, into the memory the walk uses for its hand-off
-// buffers, so no registers hold data in flight.
+// buffers, so no registers hold data in flight.  The y slot is written and
+// read with the non-temporal cache policy: each row passes through HBM once
+// per later panel, while the factor chunks, read by every workgroup, are to
+// stay in L2.
 #include "dbg_device.h"
 
 namespace dbg {
